@@ -1,0 +1,176 @@
+#!/usr/bin/env python3
+"""Headline benchmark of the LBM3 hot path on one MI355X.
+
+Workload: flow around a sphere on a uniform 256x128x128 grid (D = 32 cells at
+(64,64,64), inlet 0.05, Re = 100, K17 cumulant with limiter 0.1, velocity inlet,
+non-reflective pressure outlet, far-field sides, interpolated bounceback on the
+STL sphere), synthetic data.  A step is one time step of updateAllGrids over the
+whole grid.  `value` is MLUPS in the reference's own count
+(sum over levels of (cells - walls) * 2^level per step) for FP32 population
+storage; FP16 / BF16 shifted storage are reported as variants with their
+velocity error against an FP64 run of the same steps.
+
+The roofline entry is the fused collide-stream kernel: algorithmic bytes per
+launch (fluid cells x 2 x 27 x bytes per population) over its average duration,
+measured with HIP events around each launch on the simulation's stream.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM_GBS = 8000.0
+SBYTES = {"f32": 4, "f16": 2, "bf16": 2, "f64": 8}
+
+
+def workload():
+    from lbm3_amd import cases
+    return cases.sphere(256, 128, 128, diameter=32.0, center=(64.0, 64.0, 64.0), u_inlet=0.05, reynolds=100.0,
+                        limiter=0.1)
+
+
+def measure(storage, steps, warmup, reference=None):
+    from lbm3_amd import lib
+    case, dom, tris = workload()
+    sim = lib.Sim(case, storage)
+    t0 = time.time()
+    fluid_updates = sim.build(dom, tris)
+    build_s = time.time() - t0
+    info = sim.info(0)
+    walls = int((sim.array("wallMap") == -3).sum())
+    sim.init()
+    sim.run(warmup)
+    sim.sync()
+    total_ms, _, _ = sim.run_timed(steps, kernel_events=False)
+    _, kernel_ms, launches = sim.run_timed(steps, kernel_events=True)
+    fluid_cells = info.cellCount - walls
+    bytes_per_launch = fluid_cells * 2 * 27 * SBYTES[storage]
+    avg_kernel_s = kernel_ms / launches / 1e3
+    achieved = bytes_per_launch / avg_kernel_s / 1e9
+    out = {
+        "storage": storage,
+        "mlups": fluid_updates * steps / (total_ms / 1e3) / 1e6,
+        "ms_per_step": total_ms / steps,
+        "kernel_us": avg_kernel_s * 1e6,
+        "kernel_mlups": fluid_cells / avg_kernel_s / 1e6,
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None},
+        "cells": info.cellCount, "walls": walls, "fluid_updates_per_step": fluid_updates,
+        "bytes_per_launch": bytes_per_launch, "build_s": round(build_s, 3),
+        "steps_done": warmup + 2 * steps,
+    }
+    u = sim.macros(0)[:, 1:].astype(np.float64)
+    sim.close()
+    return out, u
+
+
+def cpu_baseline(threads, steps=2):
+    """The CPU oracle (C, OpenMP over cells) on the same sphere workload, `steps` time steps."""
+    os.environ.setdefault("OMP_NUM_THREADS", str(threads))
+    from tests.oracle_lib import OracleSim
+    case, dom, tris = workload()
+    o = OracleSim(case, dom, tris)
+    o.init()
+    o.run(1)
+    t0 = time.time()
+    o.run(steps)
+    dt = time.time() - t0
+    return {"value": round(o.fluid_updates * steps / dt / 1e6, 3), "unit": "MLUPS", "cores": threads, "kind": "port",
+            "sample": "oracle (C restatement, OpenMP over cells) on the full 256x128x128 sphere grid, %d time steps "
+                      "after 1 untimed step, %.1f s" % (steps, dt)}
+
+
+def traffic_from_profiles(storage):
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_sphere.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        d = json.load(fh)
+    e = d.get(storage)
+    return None if e is None else e.get("bytes_per_launch")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--variants", default="f16,bf16")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16")))
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (requested --gpus %d)" % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+    from lbm3_amd import lib
+    name, mem, cus = lib.device_info()
+    main_run, u32 = measure("f32", args.steps, args.warmup)
+    variants = {}
+    errs = {}
+    for v in [x for x in args.variants.split(",") if x]:
+        r, u = measure(v, args.steps, args.warmup)
+        variants[v] = r
+        errs[v] = u
+    # velocity error of each storage precision against an FP64 run of the same number of steps
+    accuracy = None
+    if not args.no_accuracy:
+        from lbm3_amd import lib as L2
+        case, dom, tris = workload()
+        s64 = L2.Sim(case, "f64")
+        s64.build(dom, tris)
+        s64.init()
+        s64.run(main_run["steps_done"])
+        s64.sync()
+        u64 = s64.macros(0)[:, 1:].astype(np.float64)
+        s64.close()
+        mask = np.isfinite(u64).all(axis=1)
+        accuracy = {"steps": main_run["steps_done"],
+                    "f32_rel_l2_u_vs_f64": float(np.linalg.norm((u32 - u64)[mask]) / np.linalg.norm(u64[mask]))}
+        for v, u in errs.items():
+            accuracy["%s_rel_l2_u_vs_f64" % v] = float(np.linalg.norm((u - u64)[mask]) / np.linalg.norm(u64[mask]))
+    copy_bytes = 1 << 30
+    copy_ms = lib.copy_probe(copy_bytes, 20)
+    for run in [main_run] + list(variants.values()):
+        run["roofline"]["traffic"] = traffic_from_profiles(run["storage"])
+    cpu = None if args.no_cpu else cpu_baseline(args.cpu_threads)
+    line = {
+        "metric": "MLUPS (sphere 256x128x128, FP32 population storage)",
+        "value": round(main_run["mlups"], 1),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(main_run["ms_per_step"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "flow around a sphere, uniform 256x128x128, D=32 at (64,64,64), u_in=0.05, Re=100, "
+                               "K17 lambda=0.1, IBB on a 57120-triangle STL sphere, FP32 storage",
+                   "storage": "f32", "cells": main_run["cells"], "walls": main_run["walls"],
+                   "fluid_updates_per_step": main_run["fluid_updates_per_step"]},
+        "roofline": main_run["roofline"],
+        "cpu_baseline": cpu,
+        "kernel_us": round(main_run["kernel_us"], 2),
+        "kernel_mlups": round(main_run["kernel_mlups"], 1),
+        "variants": {k: {"mlups": round(v["mlups"], 1), "ms_per_step": round(v["ms_per_step"], 4),
+                         "kernel_us": round(v["kernel_us"], 2), "roofline": v["roofline"],
+                         "speedup_vs_f32": round(v["mlups"] / main_run["mlups"], 3)} for k, v in variants.items()},
+        "accuracy": accuracy,
+        "copy_probe_gbs": round(2 * copy_bytes / (copy_ms / 1e3) / 1e9, 1),
+        "device": {"name": name, "hbm_bytes": mem, "cus": cus},
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,130 @@
+/*
+ * liblbm3mi -- C ABI of the LBM3 per-timestep hot path on MI355X (gfx950).
+ *
+ * The reference (stepanmuller/LBM3) has no FFI: a case file includes the
+ * library headers and calls C++ functions on GridStruct vectors.  This header
+ * is the thin C layer those calls map onto; every entry point names the
+ * reference function it replaces.  Conventions:
+ *   - an opaque lbm3_sim owns every device buffer of all grid levels and one
+ *     HIP stream; calls on one sim are not thread safe;
+ *   - host pointers are caller-owned and only read/written during the call;
+ *   - every function returns 0 on success or a negative LBM3_E* code, with a
+ *     message in lbm3_last_error(); no exception crosses the ABI.
+ */
+#ifndef LBM3MI_H
+#define LBM3MI_H
+
+#include <stdint.h>
+#include "lbm3mi_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LBM3_OK 0
+#define LBM3_E_ARG -1      /* invalid argument */
+#define LBM3_E_HIP -2      /* HIP runtime error */
+#define LBM3_E_BUILD -3    /* grid builder failed (missing periodic/open-BC source, empty STL) */
+#define LBM3_E_STATE -4    /* call out of order (e.g. step before build) */
+#define LBM3_E_MEMORY -5   /* device allocation failed or exceeds the budget */
+
+typedef struct lbm3_sim lbm3_sim;
+
+/* Host-side arrays of one built level, as produced by the reference builder
+ * gridBuilderToGrid (gridBuilderFunctions.h:1032-1569).  Used by
+ * lbm3_import_level and filled by lbm3_export_level. */
+typedef struct lbm3_level_arrays {
+    lbm3_info info;
+    int nRows;
+    const int *shifter, *rowI, *rowJ, *rowK, *rowJPlus, *rowKPlus, *rowJKPlus;
+    int W;                        /* wall-adjacent cells */
+    const int *wallMap;           /* [cellCount] */
+    const int *wallIndex;         /* [W] */
+    const uint32_t *wallData;     /* [W] */
+    const uint32_t *linkLength;   /* [7][W] */
+    int nOpenIDs;                 /* open boundary ids 0..nOpenIDs-1 */
+    const int *openCount;         /* [nOpenIDs] */
+    const int *openIndex;         /* concatenated cell lists, BCID order */
+    const int *openSource;        /* concatenated source cells */
+    int nPeriodic;
+    const int *periodicIndex, *periodicSource;
+    int wallCount;                /* cells marked as wall */
+} lbm3_level_arrays;
+
+/* Library. */
+const char *lbm3_last_error(void);
+int lbm3_device_info(char *name, int nameLen, long long *hbmBytes, int *cuCount);
+
+/* Simulation object: the case constants and hooks (lbm3_case) plus the
+ * population storage precision (LBM3_STORE_*).  Replaces the
+ * std::vector<GridStruct> grids of a case main(), e.g.
+ * taylorGreenVortexMain.cu:185. */
+int lbm3_sim_create(const lbm3_case *c, int storage, lbm3_sim **out);
+int lbm3_sim_destroy(lbm3_sim *s);
+
+/* buildGrids (gridBuilderFunctions.h:1577): voxelize the static STL bodies
+ * (triangles as 9 floats ax,ay,az,bx,...,cz; triCounts[b] triangles per body),
+ * build every level on the GPU and allocate the populations.  Returns the
+ * reference's fluidUpdatesPerIteration (sum over levels of (cells - walls)*2^L). */
+int lbm3_build_grids(lbm3_sim *s, const float *triangles, const int *triCounts, int nBodies, lbm3_bounds domain,
+                     long long *fluidUpdatesPerIteration);
+
+/* Upload a level built elsewhere (same layout as gridBuilderToGrid output). */
+int lbm3_import_level(lbm3_sim *s, int level, const lbm3_level_arrays *a);
+
+/* applyInitialCondition (boundaryConditions/applyInitialCondition.h:8) on one level. */
+int lbm3_apply_initial_condition(lbm3_sim *s, int level);
+/* Per-cell initial state (dRho, ux, uy, uz) for every cell of the level, in cell order. */
+int lbm3_apply_initial_fields(lbm3_sim *s, int level, const float *fields4);
+
+/* updateSingleGrid (updateGrid.h:16): collide+stream, open BCs, periodic copy. */
+int lbm3_update_single_grid(lbm3_sim *s, int level);
+/* updateAllGrids (updateGrid.h:388): recursive 2:1 sub-cycling from `level`. */
+int lbm3_update_all_grids(lbm3_sim *s, int level);
+/* `steps` coarse time steps of updateAllGrids(grids, 0). */
+int lbm3_run(lbm3_sim *s, int steps);
+int lbm3_synchronize(lbm3_sim *s);
+
+/* Timed run: `steps` coarse steps between HIP events on the sim's stream.
+ * out[0] = total ms; with kernelEvents != 0 also out[1] = summed ms of the
+ * level-0 collide-stream kernel (events around each launch) and out[2] = the
+ * number of those launches.  No counterpart in the reference (TNL::Timer). */
+int lbm3_run_timed(lbm3_sim *s, int steps, int kernelEvents, double *out);
+
+/* State access. */
+int lbm3_get_info(lbm3_sim *s, int level, lbm3_info *out);
+int lbm3_level_count(lbm3_sim *s);
+/* (dRho, ux, uy, uz) of every cell from its pre-collision populations
+ * (getDRhoUxUyUz, cellFunctions.h:257); out is float or double (FP64 store). */
+int lbm3_download_macros(lbm3_sim *s, int level, void *out);
+/* Populations [27][cellCount] in storage precision. */
+int lbm3_download_populations(lbm3_sim *s, int level, void *out, long long bytes);
+int lbm3_upload_populations(lbm3_sim *s, int level, const void *in, long long bytes);
+/* Named built arrays (shifter, rowI.., wallMap, wallIndex, wallData, linkLength,
+ * periodicIndex, periodicSource, openIndex, openSource, openMeta, rayHitScan,
+ * rayMap, f2cIndex, ...).  With out == NULL only *count is returned. */
+int lbm3_download_array(lbm3_sim *s, int level, const char *name, void *out, long long capacity, long long *count);
+
+/* FP64 observables. */
+/* kinetic energy sum over counted cells (taylorGreenVortexMain.cu:116-180, lattice units,
+ * level L weighted by 8^-L) */
+int lbm3_kinetic_energy(lbm3_sim *s, double *out);
+/* per-body sums of the accumulated momentum-exchange force and its torque about
+ * `origin` (trackerFunctions.h:356-474, FP64): out[body*6 + {Fx,Fy,Fz,Tx,Ty,Tz}] */
+int lbm3_wall_forces(lbm3_sim *s, int level, const double *origin, int nBodies, double *out);
+int lbm3_reset_wall_forces(lbm3_sim *s);
+
+/* applyCollision (applyCollision.h:17) on n independent cells, 27 shifted
+ * populations each (cell-major), on the device. */
+int lbm3_collide_cells_f32(float *f, int n, const lbm3_bc *bc, float nu);
+int lbm3_collide_cells_f64(double *f, int n, const lbm3_bc *bc, double nu);
+
+/* Plain device-to-device streaming copy of `bytes` (read + write), timed over
+ * `reps` launches: the achievable-bandwidth probe and the reference kernel for
+ * calibrating HBM byte counters.  No counterpart in the reference. */
+int lbm3_copy_probe(long long bytes, int reps, double *msPerCopy);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

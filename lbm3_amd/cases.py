@@ -1,0 +1,158 @@
+"""Case descriptions: the reference's case files expressed as lbm3_case data.
+
+Each builder returns (case, domain_bounds, static_triangles) where
+static_triangles is a list of float32 arrays of shape (n, 9) (one per STL
+body).  Float arithmetic that sets up the grid follows the reference case
+files in float32, so the builder sees the same numbers:
+
+* taylor_green  cases/taylorGreenVortex/taylorGreenVortexMain.cu:1-44, 82-102, 182-198
+* sphere        cases/flowAroundSphere/flowAroundSphereMain.cu:96-151, 183-197
+* channel       body-force driven periodic duct (serving mix; walls by moment BC)
+"""
+import numpy as np
+
+from . import ctypes_defs as T
+from . import geometry
+
+f32 = np.float32
+
+
+def _case(levels=1, res=1.0, dt=1.0, nu=1.0, rho=1.0):
+    c = T.Case()
+    c.gridLevelCount = levels
+    c.wallRefinementCount = 0
+    c.resGlobal = float(f32(res))
+    c.dtPhysGlobal = float(f32(dt))
+    c.nuPhys = float(f32(nu))
+    c.rhoPhys = float(f32(rho))
+    c.trackWallForce = 1
+    c.trackRotorForce = 0
+    c.trackOpenBoundaries = 0
+    c.openDefault = T.default_bc()
+    return c
+
+
+def add_open_rule(c, x=T.FACE_ANY, y=T.FACE_ANY, z=T.FACE_ANY, level_mask=0, **bc):
+    r = c.openRules[c.nOpenRules]
+    r.levelMask = level_mask
+    r.faceMask[0], r.faceMask[1], r.faceMask[2] = x, y, z
+    r.bc = T.default_bc(**bc)
+    c.nOpenRules += 1
+
+
+def add_local_rule(c, wall_id=-2, level_mask=0, **kw):
+    r = c.localRules[c.nLocalRules]
+    r.levelMask = level_mask
+    r.wallID = wall_id
+    r.set = 0
+    if "velocity" in kw:
+        r.set |= T.SET_VELOCITY
+        r.ux, r.uy, r.uz = kw["velocity"]
+    if "omega" in kw:
+        r.set |= T.SET_ROTATION
+        for a in range(3):
+            r.omega[a] = kw["omega"][a]
+            r.origin[a] = kw.get("origin", (0.0, 0.0, 0.0))[a]
+    if "force" in kw:
+        r.set |= T.SET_FORCE
+        r.gx, r.gy, r.gz = kw["force"]
+    if "limiter" in kw:
+        r.set |= T.SET_LIMITER
+        r.collisionLimiter = kw["limiter"]
+    if "nu_multiplier" in kw:
+        r.set |= T.SET_NU_MULTIPLIER
+        r.nuMultiplier = kw["nu_multiplier"]
+    if "use_ibb" in kw:
+        r.set |= T.SET_USE_IBB
+        r.useIBB = int(kw["use_ibb"])
+    c.nLocalRules += 1
+
+
+def add_refine_rule(c, level, box, mode=T.REFINE_SET):
+    r = c.refineRules[c.nRefineRules]
+    r.level, r.mode = level, mode
+    r.xMin, r.yMin, r.zMin, r.xMax, r.yMax, r.zMax = [float(f32(v)) for v in box]
+    c.nRefineRules += 1
+
+
+def bounds(xmin, ymin, zmin, xmax, ymax, zmax):
+    b = T.Bounds()
+    b.xMin, b.yMin, b.zMin, b.xMax, b.yMax, b.zMax = [float(f32(v)) for v in (xmin, ymin, zmin, xmax, ymax, zmax)]
+    return b
+
+
+def taylor_green(n=64, u0=0.05, reynolds=1600.0, pressure=False, limiter=None):
+    """Periodic Taylor-Green vortex on n^3 active cells (plus one ghost layer per face).
+
+    Lattice viscosity nu = u0 * (n / 2 pi) / Re, i.e. Re on the length n/(2 pi)."""
+    PI = f32(3.14159265358979323846)
+    L = f32(1000.0)                       # reference length [mm]
+    domain = f32(2.0) * PI * L
+    res = domain / f32(n)
+    nu_phys = f32(1.0) * (L / f32(1000.0)) / f32(reynolds)
+    dt = (f32(u0) / f32(1.0)) * (res / f32(1000.0))
+    c = _case(1, res, dt, nu_phys)
+    c.trackWallForce = 0
+    c.init.kind = T.INIT_TAYLOR_GREEN
+    c.init.u0 = float(f32(u0))
+    c.init.length = float(L)
+    c.init.pressure = int(bool(pressure))
+    add_open_rule(c, periodic=1)
+    if limiter is not None:
+        add_local_rule(c, limiter=limiter)
+    half = f32(0.5) * domain
+    eps = f32(1e-6) * domain
+    lo = -half - res - eps
+    hi = half + res + eps
+    return c, bounds(lo, lo, lo, hi, hi, hi), []
+
+
+def sphere(nx=256, ny=128, nz=128, diameter=32.0, center=(64.0, 64.0, 64.0), u_inlet=0.05, reynolds=100.0,
+           limiter=0.1, k15_on_level0=True, segments=(120, 240)):
+    """Flow around a sphere on a uniform nx*ny*nz grid of 1 mm cells.
+
+    Boundaries and collision follow the reference sphere case
+    (flowAroundSphereMain.cu:108-151): velocity inlet at i=0, non-reflective
+    pressure outlet at i=nx-1, far-field velocity on every other boundary cell,
+    K17 with limiter 0.1 except K15 on level 0 (where the open boundaries are);
+    the sphere (wall 0) is a no-slip interpolated-bounceback wall.  On a single
+    uniform level, k15_on_level0=False selects K17 everywhere."""
+    res = f32(1.0)
+    u_phys = f32(1.0)
+    nu_phys = u_phys * (f32(diameter) / f32(1000.0)) / f32(reynolds)
+    dt = (f32(u_inlet) / u_phys) * (res / f32(1000.0))
+    c = _case(1, res, dt, nu_phys)
+    c.init.kind = T.INIT_UNIFORM
+    c.init.ux = float(f32(u_inlet))
+    U = float(f32(u_inlet))
+    add_open_rule(c, x=T.FACE_HIGH, y=T.FACE_MID, z=T.FACE_MID, dirichletRho=1, dRho=0.0, openBCID=1, nonReflective=1)
+    add_open_rule(c, x=T.FACE_LOW, y=T.FACE_MID, z=T.FACE_MID, dirichletU=1, ux=U, openBCID=0, nonReflective=0)
+    add_open_rule(c, dirichletU=1, ux=U, openBCID=2, nonReflective=0)
+    add_local_rule(c, limiter=limiter)
+    add_local_rule(c, wall_id=0, velocity=(0.0, 0.0, 0.0))
+    if k15_on_level0:
+        add_local_rule(c, level_mask=1, limiter=0.0)
+    tri = geometry.uv_sphere(diameter / 2.0, center, segments[0], segments[1])
+    return c, bounds(0, 0, 0, nx, ny, nz), [tri]
+
+
+def channel(nx=64, ny=32, nz=32, force=1e-6, nu=0.1, limiter=0.0):
+    """Body-force driven duct: periodic in x, no-slip (moment-based u=0) on y and z faces.
+
+    Lattice viscosity `nu` (K17 is singular at nu = 1/6, where omega1 = omega2 = 1).
+    K15 by default: like the reference cases, K17 is not used next to moment-based
+    boundaries."""
+    res = f32(1.0)
+    c = _case(1, res, f32(1.0), f32(1.0))
+    # lattice viscosity: nu = dt * nu_phys / (res/1000)^2 with dt = 1
+    c.dtPhysGlobal = float(f32(1.0))
+    c.nuPhys = float(f32(nu) * (res / f32(1000.0)) * (res / f32(1000.0)))
+    c.trackWallForce = 0
+    add_open_rule(c, y=T.FACE_LOW | T.FACE_HIGH, dirichletU=1, openBCID=0, nonReflective=0)
+    add_open_rule(c, z=T.FACE_LOW | T.FACE_HIGH, dirichletU=1, openBCID=0, nonReflective=0)
+    add_open_rule(c, periodic=1)
+    add_local_rule(c, force=(force, 0.0, 0.0))
+    if limiter is not None:
+        add_local_rule(c, limiter=limiter)
+    # one ghost layer in x (periodic), walls are the first/last cell layers in y and z
+    return c, bounds(-1, 0, 0, nx + 1, ny, nz), []

@@ -1,0 +1,568 @@
+// C ABI of the hot path (include/lbm3mi.h): simulation objects, level import,
+// the per-step driver with 2:1 sub-cycling, observables and timing.
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include "sim.hpp"
+
+namespace lbm3 {
+
+static thread_local std::string g_error;
+
+int set_error(int code, const std::string &msg)
+{
+    g_error = msg;
+    return code;
+}
+
+int check_hip(hipError_t e, const char *what)
+{
+    if (e == hipSuccess) return 0;
+    return set_error(LBM3_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIPCHK(expr)                                  \
+    do {                                              \
+        const int rc_ = check_hip((expr), #expr);     \
+        if (rc_) return rc_;                          \
+    } while (0)
+
+size_t storage_bytes(int storage)
+{
+    switch (storage) {
+    case LBM3_STORE_F16: case LBM3_STORE_BF16: return 2;
+    case LBM3_STORE_F64: return 8;
+    default: return 4;
+    }
+}
+size_t compute_bytes(int storage) { return storage == LBM3_STORE_F64 ? 8 : 4; }
+
+static Topology topo_of(const Level &l)
+{
+    return Topology{l.N, l.shifter.p, l.ri.p, l.rj.p, l.rk.p, l.rjp.p, l.rkp.p, l.rjkp.p};
+}
+
+template <class T> static int upload(DBuf<T> &b, const T *host, long long n, hipStream_t s)
+{
+    HIPCHK(b.alloc(n));
+    if (n > 0 && host) HIPCHK(hipMemcpyAsync(b.p, host, sizeof(T) * (size_t)n, hipMemcpyHostToDevice, s));
+    return 0;
+}
+
+// derived per-level tables: local BC, open-BC records, force accumulators, populations
+int finish_level_setup(Sim &s, int level)
+{
+    Level &l = s.L[level];
+    l.geom = LevelGeom{l.info.res, l.info.ox, l.info.oy, l.info.oz, l.info.dtPhys, l.info.cellCountX, l.info.cellCountY,
+                       l.info.cellCountZ};
+    l.bulk = local_bc(s.C, level, -1);
+    LocalBC wb[32];
+    for (int w = 0; w < 32; ++w) wb[w] = local_bc(s.C, level, w);
+    if (upload(l.wallBC, wb, 32, s.stream)) return LBM3_E_HIP;
+    std::vector<OpenBCRecord> rec(s.C.nOpenRules + 1);
+    for (int r = 0; r <= s.C.nOpenRules; ++r) rec[r] = open_record(open_bc(s.C, r));
+    if (upload(l.openRecords, rec.data(), (long long)rec.size(), s.stream)) return LBM3_E_HIP;
+    HIPCHK(l.gWall.alloc(3LL * (l.W > 0 ? l.W : 1)));
+    HIPCHK(hipMemsetAsync(l.gWall.p, 0, sizeof(double) * (size_t)l.gWall.n, s.stream));
+    HIPCHK(l.openRefs.alloc(2LL * (l.nOpen > 0 ? l.nOpen : 1) * (long long)compute_bytes(s.storage)));
+    HIPCHK(hipMemsetAsync(l.openRefs.p, 0, (size_t)l.openRefs.n, s.stream));
+    HIPCHK(l.openCum.alloc(2LL * (l.nOpen > 0 ? l.nOpen : 1)));
+    HIPCHK(hipMemsetAsync(l.openCum.p, 0, sizeof(double) * (size_t)l.openCum.n, s.stream));
+    HIPCHK(l.f.alloc(27LL * l.N * (long long)storage_bytes(s.storage)));
+    l.built = true;
+    return 0;
+}
+
+template <class S> static StepArgs<S> step_args(Sim &s, Level &l)
+{
+    StepArgs<S> a;
+    a.f = reinterpret_cast<S *>(l.f.p);
+    a.topo = topo_of(l);
+    a.wallMap = l.wallMap.p;
+    a.wallData = l.wallData.p;
+    a.linkLength = l.linkLength.p;
+    a.W = l.W;
+    a.gWall = s.C.trackWallForce ? l.gWall.p : nullptr;
+    a.nu = l.info.nu;
+    a.bulk = l.bulk;
+    a.wallBC = l.wallBC.p;
+    a.geom = l.geom;
+    return a;
+}
+
+template <class S> static OpenArgs<S> open_args(Sim &s, Level &l)
+{
+    using T = typename StoreTraits<S>::compute;
+    OpenArgs<S> a;
+    a.f = reinterpret_cast<S *>(l.f.p);
+    a.topo = topo_of(l);
+    a.count = l.nOpen;
+    a.cell = l.openCell.p;
+    a.meta = l.openMeta.p;
+    a.records = l.openRecords.p;
+    T *refs = reinterpret_cast<T *>(l.openRefs.p);
+    a.dRhoRef = refs;
+    a.uNormalRef = refs + (l.nOpen > 0 ? l.nOpen : 1);
+    a.dRhoCum = s.C.trackOpenBoundaries ? l.openCum.p : nullptr;
+    a.uNormalCum = s.C.trackOpenBoundaries ? l.openCum.p + (l.nOpen > 0 ? l.nOpen : 1) : nullptr;
+    a.iterationsFinished = l.info.iterationsFinished;
+    return a;
+}
+
+template <class S> static void step_level(Sim &s, int level)
+{
+    Level &l = s.L[level];
+    const bool flip = l.info.esotwistFlipper != 0;
+    const bool timed = s.timing && level == 0;
+    if (timed) {
+        if (s.evUsed >= (int)s.evStart.size()) {
+            hipEvent_t a, b;
+            (void)hipEventCreate(&a);
+            (void)hipEventCreate(&b);
+            s.evStart.push_back(a);
+            s.evStop.push_back(b);
+        }
+        (void)hipEventRecord(s.evStart[s.evUsed], s.stream);
+    }
+    launch_collide_stream<S>(step_args<S>(s, l), flip, s.stream);
+    if (timed) {
+        (void)hipEventRecord(s.evStop[s.evUsed], s.stream);
+        s.evUsed++;
+    }
+    const bool nflip = !flip;
+    l.info.esotwistFlipper = nflip ? 1 : 0;
+    launch_open<S>(open_args<S>(s, l), nflip, s.stream);
+    launch_periodic<S>(reinterpret_cast<S *>(l.f.p), topo_of(l), l.nPeriodic, l.perIndex.p, l.perSource.p, nflip, s.stream);
+    l.info.iterationsSinceTrackerReset++;
+    l.info.iterationsFinished++;
+}
+
+static void step_level_any(Sim &s, int level)
+{
+    switch (s.storage) {
+    case LBM3_STORE_F16: step_level<__half>(s, level); break;
+    case LBM3_STORE_BF16: step_level<__hip_bfloat16>(s, level); break;
+    case LBM3_STORE_F64: step_level<double>(s, level); break;
+    default: step_level<float>(s, level); break;
+    }
+}
+
+void update_interface(Sim &s, int coarse);   // interface.hip
+
+static void update_all(Sim &s, int level)
+{
+    step_level_any(s, level);
+    if (level < s.levels - 1) {
+        for (int i = 0; i < 2; ++i) update_all(s, level + 1);
+        update_interface(s, level);
+    }
+}
+
+template <class S> static int init_level(Sim &s, int level, const float *fieldsDev)
+{
+    Level &l = s.L[level];
+    InitDesc d{};
+    d.kind = fieldsDev ? 2 : s.C.init.kind;
+    d.dRho = s.C.init.dRho; d.ux = s.C.init.ux; d.uy = s.C.init.uy; d.uz = s.C.init.uz;
+    d.u0 = s.C.init.u0; d.length = s.C.init.length; d.pressure = s.C.init.pressure;
+    d.fields = fieldsDev;
+    const bool flip = l.info.esotwistFlipper != 0;
+    launch_init<S>(reinterpret_cast<S *>(l.f.p), topo_of(l), d, l.geom, flip, s.stream);
+    launch_open_ref_init<S>(open_args<S>(s, l), l.info.cellCountX, l.info.cellCountY, l.info.cellCountZ, flip, s.stream);
+    return check_hip(hipGetLastError(), "init kernels");
+}
+
+static int init_level_any(Sim &s, int level, const float *fieldsDev)
+{
+    switch (s.storage) {
+    case LBM3_STORE_F16: return init_level<__half>(s, level, fieldsDev);
+    case LBM3_STORE_BF16: return init_level<__hip_bfloat16>(s, level, fieldsDev);
+    case LBM3_STORE_F64: return init_level<double>(s, level, fieldsDev);
+    default: return init_level<float>(s, level, fieldsDev);
+    }
+}
+
+static int check_level(Sim &s, int level)
+{
+    if (level < 0 || level >= s.levels || !s.L[level].built) return set_error(LBM3_E_STATE, "level not built");
+    return 0;
+}
+
+template <class T> int collide_cells(T *f, int n, const lbm3_bc *bc, T nu)
+{
+    if (!f || !bc || n < 0) return set_error(LBM3_E_ARG, "bad collide arguments");
+    DBuf<T> d;
+    HIPCHK(d.alloc(27LL * n));
+    HIPCHK(hipMemcpy(d.p, f, sizeof(T) * 27 * (size_t)n, hipMemcpyHostToDevice));
+    const CollisionParams P{bc->gx, bc->gy, bc->gz, bc->collisionLimiter, bc->nuMultiplier};
+    launch_collide_batch<T>(d.p, n, P, nu, nullptr);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpy(f, d.p, sizeof(T) * 27 * (size_t)n, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+} // namespace lbm3
+
+using namespace lbm3;
+
+extern "C" {
+
+const char *lbm3_last_error(void) { return g_error.c_str(); }
+
+int lbm3_device_info(char *name, int nameLen, long long *hbmBytes, int *cuCount)
+{
+    hipDeviceProp_t p;
+    HIPCHK(hipGetDeviceProperties(&p, 0));
+    if (name && nameLen > 0) snprintf(name, (size_t)nameLen, "%s %s", p.name, p.gcnArchName);
+    if (hbmBytes) *hbmBytes = (long long)p.totalGlobalMem;
+    if (cuCount) *cuCount = p.multiProcessorCount;
+    return 0;
+}
+
+int lbm3_sim_create(const lbm3_case *c, int storage, lbm3_sim **out)
+{
+    if (!c || !out) return set_error(LBM3_E_ARG, "null argument");
+    if (c->gridLevelCount < 1 || c->gridLevelCount > LBM3_MAX_LEVELS) return set_error(LBM3_E_ARG, "gridLevelCount out of range");
+    if (storage < LBM3_STORE_F32 || storage > LBM3_STORE_F64) return set_error(LBM3_E_ARG, "unknown storage");
+    lbm3_sim *h = new (std::nothrow) lbm3_sim();
+    if (!h) return set_error(LBM3_E_MEMORY, "host allocation");
+    h->s.C = *c;
+    h->s.storage = storage;
+    h->s.levels = c->gridLevelCount;
+    const int rc = check_hip(hipStreamCreateWithFlags(&h->s.stream, hipStreamNonBlocking), "stream");
+    if (rc) { delete h; return rc; }
+    *out = h;
+    return 0;
+}
+
+int lbm3_sim_destroy(lbm3_sim *h)
+{
+    if (!h) return 0;
+    (void)hipStreamSynchronize(h->s.stream);
+    for (auto e : h->s.evStart) (void)hipEventDestroy(e);
+    for (auto e : h->s.evStop) (void)hipEventDestroy(e);
+    (void)hipStreamDestroy(h->s.stream);
+    delete h;
+    return 0;
+}
+
+int lbm3_import_level(lbm3_sim *h, int level, const lbm3_level_arrays *a)
+{
+    if (!h || !a || level < 0 || level >= h->s.levels) return set_error(LBM3_E_ARG, "bad import arguments");
+    Sim &s = h->s;
+    Level &l = s.L[level];
+    hipStream_t st = s.stream;
+    l.info = a->info;
+    l.N = a->info.cellCount;
+    l.nRows = a->nRows;
+    l.W = a->W;
+    l.wallCount = a->wallCount;
+    if (upload(l.shifter, a->shifter, l.N, st) || upload(l.ri, a->rowI, l.nRows, st) || upload(l.rj, a->rowJ, l.nRows, st) ||
+        upload(l.rk, a->rowK, l.nRows, st) || upload(l.rjp, a->rowJPlus, l.nRows, st) || upload(l.rkp, a->rowKPlus, l.nRows, st) ||
+        upload(l.rjkp, a->rowJKPlus, l.nRows, st) || upload(l.wallMap, a->wallMap, l.N, st) ||
+        upload(l.wallIndex, a->wallIndex, l.W, st) || upload(l.wallData, a->wallData, l.W, st) ||
+        upload(l.linkLength, a->linkLength, 7LL * l.W, st))
+        return LBM3_E_HIP;
+    // open boundary: concatenate BCIDs, derive normal index and rule per cell on the host
+    l.nOpenIDs = a->nOpenIDs;
+    l.openCount.assign(a->openCount, a->openCount + a->nOpenIDs);
+    l.nOpen = 0;
+    for (int b = 0; b < a->nOpenIDs; ++b) l.nOpen += a->openCount[b];
+    std::vector<uint32_t> meta(l.nOpen);
+    for (int x = 0; x < l.nOpen; ++x) {
+        const int cell = a->openIndex[x];
+        const int shift = a->shifter[cell];
+        const int row = shift >= 0 ? shift : a->shifter[cell + shift];
+        const int back = shift >= 0 ? 0 : -shift;
+        const int i = a->rowI[row] + back, j = a->rowJ[row], k = a->rowK[row];
+        const lbm3_info &I = a->info;
+        const int nx = i == 0 ? -1 : (i == I.cellCountX - 1 ? 1 : 0);
+        const int ny = j == 0 ? -1 : (j == I.cellCountY - 1 ? 1 : 0);
+        const int nz = k == 0 ? -1 : (k == I.cellCountZ - 1 ? 1 : 0);
+        const int rule = open_rule(s.C, level, I.cellCountX, I.cellCountY, I.cellCountZ, i, j, k);
+        const int wm = a->wallMap[cell];
+        bool track = wm != -2;
+        if (wm >= 0 && (a->wallData[wm] & 1u)) track = false;
+        meta[x] = (uint32_t)normal_index(nx, ny, nz) | ((uint32_t)rule << 5) | ((track ? 1u : 0u) << 13);
+    }
+    if (upload(l.openCell, a->openIndex, l.nOpen, st) || upload(l.openSource, a->openSource, l.nOpen, st) ||
+        upload(l.openMeta, meta.data(), l.nOpen, st))
+        return LBM3_E_HIP;
+    l.nPeriodic = a->nPeriodic;
+    if (upload(l.perIndex, a->periodicIndex, l.nPeriodic, st) || upload(l.perSource, a->periodicSource, l.nPeriodic, st))
+        return LBM3_E_HIP;
+    const int rc = finish_level_setup(s, level);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(st));
+    return 0;
+}
+
+int lbm3_apply_initial_condition(lbm3_sim *h, int level)
+{
+    if (!h) return set_error(LBM3_E_ARG, "null sim");
+    if (const int rc = check_level(h->s, level)) return rc;
+    return init_level_any(h->s, level, nullptr);
+}
+
+int lbm3_apply_initial_fields(lbm3_sim *h, int level, const float *fields4)
+{
+    if (!h || !fields4) return set_error(LBM3_E_ARG, "null argument");
+    if (const int rc = check_level(h->s, level)) return rc;
+    DBuf<float> d;
+    if (upload(d, fields4, 4LL * h->s.L[level].N, h->s.stream)) return LBM3_E_HIP;
+    const int rc = init_level_any(h->s, level, d.p);
+    HIPCHK(hipStreamSynchronize(h->s.stream));
+    return rc;
+}
+
+int lbm3_update_single_grid(lbm3_sim *h, int level)
+{
+    if (!h) return set_error(LBM3_E_ARG, "null sim");
+    if (const int rc = check_level(h->s, level)) return rc;
+    step_level_any(h->s, level);
+    return check_hip(hipGetLastError(), "updateSingleGrid launch");
+}
+
+int lbm3_update_all_grids(lbm3_sim *h, int level)
+{
+    if (!h) return set_error(LBM3_E_ARG, "null sim");
+    for (int L = level; L < h->s.levels; ++L)
+        if (const int rc = check_level(h->s, L)) return rc;
+    update_all(h->s, level);
+    return check_hip(hipGetLastError(), "updateAllGrids launch");
+}
+
+int lbm3_run(lbm3_sim *h, int steps)
+{
+    if (!h || steps < 0) return set_error(LBM3_E_ARG, "bad run arguments");
+    for (int L = 0; L < h->s.levels; ++L)
+        if (const int rc = check_level(h->s, L)) return rc;
+    for (int t = 0; t < steps; ++t) update_all(h->s, 0);
+    return check_hip(hipGetLastError(), "run launch");
+}
+
+int lbm3_synchronize(lbm3_sim *h)
+{
+    if (!h) return set_error(LBM3_E_ARG, "null sim");
+    return check_hip(hipStreamSynchronize(h->s.stream), "synchronize");
+}
+
+int lbm3_run_timed(lbm3_sim *h, int steps, int kernelEvents, double *out)
+{
+    if (!h || !out || steps <= 0) return set_error(LBM3_E_ARG, "bad timed-run arguments");
+    Sim &s = h->s;
+    for (int L = 0; L < s.levels; ++L)
+        if (const int rc = check_level(s, L)) return rc;
+    hipEvent_t a, b;
+    HIPCHK(hipEventCreate(&a));
+    HIPCHK(hipEventCreate(&b));
+    s.timing = kernelEvents != 0;
+    s.evUsed = 0;
+    HIPCHK(hipEventRecord(a, s.stream));
+    for (int t = 0; t < steps; ++t) update_all(s, 0);
+    HIPCHK(hipEventRecord(b, s.stream));
+    s.timing = false;
+    HIPCHK(hipEventSynchronize(b));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, a, b));
+    double kernel = 0.0;
+    for (int e = 0; e < s.evUsed; ++e) {
+        float k = 0.f;
+        HIPCHK(hipEventElapsedTime(&k, s.evStart[e], s.evStop[e]));
+        kernel += k;
+    }
+    out[0] = ms;
+    out[1] = kernel;
+    out[2] = s.evUsed;
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return check_hip(hipGetLastError(), "timed run");
+}
+
+int lbm3_get_info(lbm3_sim *h, int level, lbm3_info *out)
+{
+    if (!h || !out || level < 0 || level >= h->s.levels) return set_error(LBM3_E_ARG, "bad info arguments");
+    *out = h->s.L[level].info;
+    return 0;
+}
+
+int lbm3_level_count(lbm3_sim *h) { return h ? h->s.levels : 0; }
+
+int lbm3_download_macros(lbm3_sim *h, int level, void *out)
+{
+    if (!h || !out) return set_error(LBM3_E_ARG, "null argument");
+    Sim &s = h->s;
+    if (const int rc = check_level(s, level)) return rc;
+    Level &l = s.L[level];
+    const size_t cb = compute_bytes(s.storage);
+    DBuf<unsigned char> d;
+    HIPCHK(d.alloc(4LL * l.N * (long long)cb));
+    const bool flip = l.info.esotwistFlipper != 0;
+    const Topology t = topo_of(l);
+    switch (s.storage) {
+    case LBM3_STORE_F16: launch_macros<__half>(reinterpret_cast<__half *>(l.f.p), t, reinterpret_cast<float *>(d.p), flip, s.stream); break;
+    case LBM3_STORE_BF16: launch_macros<__hip_bfloat16>(reinterpret_cast<__hip_bfloat16 *>(l.f.p), t, reinterpret_cast<float *>(d.p), flip, s.stream); break;
+    case LBM3_STORE_F64: launch_macros<double>(reinterpret_cast<double *>(l.f.p), t, reinterpret_cast<double *>(d.p), flip, s.stream); break;
+    default: launch_macros<float>(reinterpret_cast<float *>(l.f.p), t, reinterpret_cast<float *>(d.p), flip, s.stream); break;
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out, d.p, 4 * (size_t)l.N * cb, hipMemcpyDeviceToHost, s.stream));
+    HIPCHK(hipStreamSynchronize(s.stream));
+    return 0;
+}
+
+int lbm3_download_populations(lbm3_sim *h, int level, void *out, long long bytes)
+{
+    if (!h || !out) return set_error(LBM3_E_ARG, "null argument");
+    Sim &s = h->s;
+    if (const int rc = check_level(s, level)) return rc;
+    if (bytes < s.L[level].f.n) return set_error(LBM3_E_ARG, "buffer too small");
+    HIPCHK(hipMemcpyAsync(out, s.L[level].f.p, (size_t)s.L[level].f.n, hipMemcpyDeviceToHost, s.stream));
+    HIPCHK(hipStreamSynchronize(s.stream));
+    return 0;
+}
+
+int lbm3_upload_populations(lbm3_sim *h, int level, const void *in, long long bytes)
+{
+    if (!h || !in) return set_error(LBM3_E_ARG, "null argument");
+    Sim &s = h->s;
+    if (const int rc = check_level(s, level)) return rc;
+    if (bytes != s.L[level].f.n) return set_error(LBM3_E_ARG, "size mismatch");
+    HIPCHK(hipMemcpyAsync(s.L[level].f.p, in, (size_t)bytes, hipMemcpyHostToDevice, s.stream));
+    HIPCHK(hipStreamSynchronize(s.stream));
+    return 0;
+}
+
+int lbm3_download_array(lbm3_sim *h, int level, const char *name, void *out, long long capacity, long long *count)
+{
+    if (!h || !name || !count || level < 0 || level >= h->s.levels) return set_error(LBM3_E_ARG, "bad array arguments");
+    Sim &s = h->s;
+    Level &l = s.L[level];
+    const void *src = nullptr;
+    long long n = -1;
+    size_t es = 4;
+#define ARR(nm, buf, cnt) else if (!strcmp(name, nm)) { src = (buf).p; n = (cnt); es = sizeof(*(buf).p); }
+    if (false) {}
+    ARR("shifter", l.shifter, l.N) ARR("rowI", l.ri, l.nRows) ARR("rowJ", l.rj, l.nRows) ARR("rowK", l.rk, l.nRows)
+    ARR("rowJPlus", l.rjp, l.nRows) ARR("rowKPlus", l.rkp, l.nRows) ARR("rowJKPlus", l.rjkp, l.nRows)
+    ARR("wallMap", l.wallMap, l.N) ARR("wallIndex", l.wallIndex, l.W) ARR("wallData", l.wallData, l.W)
+    ARR("linkLength", l.linkLength, 7LL * l.W) ARR("periodicIndex", l.perIndex, l.nPeriodic)
+    ARR("periodicSource", l.perSource, l.nPeriodic) ARR("openIndex", l.openCell, l.nOpen)
+    ARR("openSource", l.openSource, l.nOpen) ARR("openMeta", l.openMeta, l.nOpen)
+    ARR("wallForce", l.gWall, 3LL * l.W) ARR("rayHitScan", l.rayHitScan, l.rayHitScan.n) ARR("rayMap", l.rayMap, l.rayMap.n)
+    ARR("f2cIndex", l.F2C.index, l.F2C.interfaceCount) ARR("f2cChild", l.F2C.childMap, l.F2C.interfaceCount)
+    ARR("c2fIndex", l.C2F.index, l.C2F.interfaceCount) ARR("c2fChild", l.C2F.childMap, l.C2F.interfaceCount)
+    ARR("leftoverIndex", l.C2F.leftoverIndex, l.C2F.leftoverCount)
+    ARR("leftoverParent", l.C2F.leftoverParent, l.C2F.leftoverCount)
+    ARR("leftoverNbrI", l.C2F.leftoverNbrI, l.C2F.leftoverCount)
+    ARR("leftoverNbrJ", l.C2F.leftoverNbrJ, l.C2F.leftoverCount)
+    ARR("leftoverNbrK", l.C2F.leftoverNbrK, l.C2F.leftoverCount)
+#undef ARR
+    else if (!strcmp(name, "openCount")) {
+        *count = l.nOpenIDs;
+        if (out) {
+            if (capacity < l.nOpenIDs) return set_error(LBM3_E_ARG, "buffer too small");
+            memcpy(out, l.openCount.data(), sizeof(int) * (size_t)l.nOpenIDs);
+        }
+        return 0;
+    } else if (!strcmp(name, "openRefs")) {
+        src = l.openRefs.p;
+        es = compute_bytes(s.storage);
+        n = 2LL * (l.nOpen > 0 ? l.nOpen : 1);
+    } else return set_error(LBM3_E_ARG, std::string("unknown array ") + name);
+    *count = n;
+    if (!out || n <= 0 || !src) return 0;
+    if (capacity < n) return set_error(LBM3_E_ARG, "buffer too small");
+    HIPCHK(hipStreamSynchronize(s.stream));
+    HIPCHK(hipMemcpy(out, src, es * (size_t)n, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int lbm3_kinetic_energy(lbm3_sim *h, double *out)
+{
+    if (!h || !out) return set_error(LBM3_E_ARG, "null argument");
+    Sim &s = h->s;
+    double total = 0.0;
+    for (int level = 0; level < s.levels; ++level) {
+        if (const int rc = check_level(s, level)) return rc;
+        Level &l = s.L[level];
+        const long long nb = (l.N + 255) / 256;
+        if (s.partial.n < nb) HIPCHK(s.partial.alloc(nb));
+        if (!s.scalar.p) HIPCHK(s.scalar.alloc(8));
+        const bool flip = l.info.esotwistFlipper != 0;
+        const Topology t = topo_of(l);
+        switch (s.storage) {
+        case LBM3_STORE_F16: launch_kinetic_energy<__half>(reinterpret_cast<__half *>(l.f.p), t, l.wallMap.p, l.wallData.p, s.partial.p, s.scalar.p, flip, s.stream); break;
+        case LBM3_STORE_BF16: launch_kinetic_energy<__hip_bfloat16>(reinterpret_cast<__hip_bfloat16 *>(l.f.p), t, l.wallMap.p, l.wallData.p, s.partial.p, s.scalar.p, flip, s.stream); break;
+        case LBM3_STORE_F64: launch_kinetic_energy<double>(reinterpret_cast<double *>(l.f.p), t, l.wallMap.p, l.wallData.p, s.partial.p, s.scalar.p, flip, s.stream); break;
+        default: launch_kinetic_energy<float>(reinterpret_cast<float *>(l.f.p), t, l.wallMap.p, l.wallData.p, s.partial.p, s.scalar.p, flip, s.stream); break;
+        }
+        HIPCHK(hipGetLastError());
+        double v = 0.0;
+        HIPCHK(hipMemcpyAsync(&v, s.scalar.p, sizeof(double), hipMemcpyDeviceToHost, s.stream));
+        HIPCHK(hipStreamSynchronize(s.stream));
+        total += v / (double)(1LL << (3 * level));
+    }
+    *out = total;
+    return 0;
+}
+
+int lbm3_wall_forces(lbm3_sim *h, int level, const double *origin, int nBodies, double *out)
+{
+    if (!h || !out || nBodies <= 0 || nBodies > LBM3_MAX_BODIES) return set_error(LBM3_E_ARG, "bad force arguments");
+    Sim &s = h->s;
+    if (const int rc = check_level(s, level)) return rc;
+    Level &l = s.L[level];
+    for (int i = 0; i < 6 * nBodies; ++i) out[i] = 0.0;
+    if (l.W == 0) return 0;
+    const long long nb = (l.W + 255) / 256;
+    DBuf<double> part;
+    HIPCHK(part.alloc(nb * 6 * nBodies));
+    const double o[3] = {origin ? origin[0] : 0.0, origin ? origin[1] : 0.0, origin ? origin[2] : 0.0};
+    launch_wall_force(l.gWall.p, l.W, l.wallData.p, l.wallIndex.p, topo_of(l), l.geom, o, nBodies, part.p, s.stream);
+    HIPCHK(hipGetLastError());
+    std::vector<double> host((size_t)(nb * 6 * nBodies));
+    HIPCHK(hipMemcpyAsync(host.data(), part.p, sizeof(double) * host.size(), hipMemcpyDeviceToHost, s.stream));
+    HIPCHK(hipStreamSynchronize(s.stream));
+    for (long long b = 0; b < nb; ++b)
+        for (int i = 0; i < 6 * nBodies; ++i) out[i] += host[(size_t)(b * 6 * nBodies + i)];
+    return 0;
+}
+
+int lbm3_reset_wall_forces(lbm3_sim *h)
+{
+    if (!h) return set_error(LBM3_E_ARG, "null sim");
+    for (int level = 0; level < h->s.levels; ++level)
+        if (h->s.L[level].gWall.p)
+            HIPCHK(hipMemsetAsync(h->s.L[level].gWall.p, 0, sizeof(double) * (size_t)h->s.L[level].gWall.n, h->s.stream));
+    return 0;
+}
+
+int lbm3_copy_probe(long long bytes, int reps, double *msPerCopy)
+{
+    if (bytes <= 0 || reps <= 0 || !msPerCopy) return set_error(LBM3_E_ARG, "bad probe arguments");
+    DBuf<unsigned char> a, b;
+    HIPCHK(a.alloc(bytes));
+    HIPCHK(b.alloc(bytes));
+    HIPCHK(hipMemset(a.p, 1, (size_t)bytes));
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    launch_copy(a.p, b.p, bytes, nullptr);
+    HIPCHK(hipEventRecord(e0, nullptr));
+    for (int r = 0; r < reps; ++r) launch_copy(a.p, b.p, bytes, nullptr);
+    HIPCHK(hipEventRecord(e1, nullptr));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    *msPerCopy = ms / reps;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return check_hip(hipGetLastError(), "copy probe");
+}
+
+int lbm3_collide_cells_f32(float *f, int n, const lbm3_bc *bc, float nu) { return collide_cells<float>(f, n, bc, nu); }
+int lbm3_collide_cells_f64(double *f, int n, const lbm3_bc *bc, double nu) { return collide_cells<double>(f, n, bc, nu); }
+
+} // extern "C"

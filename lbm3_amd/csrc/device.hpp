@@ -1,0 +1,88 @@
+// Device-side views of one grid level and the per-step argument blocks.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include "lattice.hpp"
+#include "cumulant.hpp"
+#include "boundary.hpp"
+#include "storage.hpp"
+
+namespace lbm3 {
+
+// Compressed IJK/NBR (reference types.h:162-164): one shifter per cell plus six
+// per-row ints.  shifter >= 0: the cell starts row `shifter`; < 0: offset back
+// to the row head.  i+ is cell+1 (wrapping to 0), j+/k+/jk+ come from the row.
+struct Topology {
+    int N;
+    const int *shifter;
+    const int *ri, *rj, *rk, *rjp, *rkp, *rjkp;
+};
+
+__device__ __forceinline__ void decode(const Topology &t, int cell, int (&nb)[8], int *ijk)
+{
+    const int shift = t.shifter[cell];
+    int row = shift, back = 0;
+    if (shift < 0) { row = t.shifter[cell + shift]; back = -shift; }
+    if (ijk) { ijk[0] = t.ri[row] + back; ijk[1] = t.rj[row]; ijk[2] = t.rk[row]; }
+    const int N = t.N;
+    nb[0] = cell;
+    nb[2] = t.rjp[row] + back;
+    nb[3] = t.rkp[row] + back;
+    nb[6] = t.rjkp[row] + back;
+    nb[1] = cell + 1 >= N ? 0 : cell + 1;
+    nb[4] = nb[2] + 1 >= N ? 0 : nb[2] + 1;
+    nb[5] = nb[3] + 1 >= N ? 0 : nb[3] + 1;
+    nb[7] = nb[6] + 1 >= N ? 0 : nb[6] + 1;
+}
+
+// getLocalBC resolved per (level, wallID): rules depend only on those two keys,
+// except the rotating-wall velocity which needs the cell position.
+struct LocalBC {
+    float gx, gy, gz, limiter, nuMult;
+    int useIBB;
+    int velMode;            // 0: wall moves with the fluid's own velocity, 1: fixed, 2: rotation
+    float ux, uy, uz;
+    float omega[3], origin[3];
+};
+
+struct LevelGeom {          // what the hooks need about the level
+    float res, ox, oy, oz, dtPhys;
+    int cellCountX, cellCountY, cellCountZ;
+};
+
+template <class S> struct StepArgs {
+    S *f;
+    Topology topo;
+    const int *wallMap;
+    const uint32_t *wallData;
+    const uint32_t *linkLength;   // [7][W]
+    int W;
+    double *gWall;                // [3][W] or null (no force tracking)
+    float nu;
+    LocalBC bulk;                 // cells that are not next to a wall
+    const LocalBC *wallBC;        // [32] by wallID
+    LevelGeom geom;
+};
+
+// Open boundary cells of one level, all BCIDs concatenated (BCID order, then cell order)
+template <class S> struct OpenArgs {
+    S *f;
+    Topology topo;
+    int count;
+    const int *cell;
+    const uint32_t *meta;         // bits 0-4: normal index, 5-12: record, 13: track flow
+    const OpenBCRecord *records;
+    typename StoreTraits<S>::compute *dRhoRef, *uNormalRef;  // reference state, compute precision
+    double *dRhoCum, *uNormalCum; // flow accumulators or null
+    int iterationsFinished;
+};
+
+__device__ __forceinline__ int xcd_block(int b, int n)
+{
+    // blocks are dispatched round-robin over the 8 XCDs; give each XCD a contiguous range
+    const int per = n / 8;
+    if (b >= per * 8) return b;
+    return (b % 8) * per + b / 8;
+}
+
+} // namespace lbm3

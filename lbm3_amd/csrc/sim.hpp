@@ -1,0 +1,98 @@
+// Device-resident state of a simulation (all grid levels) behind the C ABI.
+#pragma once
+#include <string>
+#include <vector>
+#include "../../include/lbm3mi.h"
+#include "step.hpp"
+#include "hooks.hpp"
+
+namespace lbm3 {
+
+// a device buffer that frees itself
+template <class T> struct DBuf {
+    T *p = nullptr;
+    long long n = 0;
+    DBuf() = default;
+    DBuf(const DBuf &) = delete;
+    DBuf &operator=(const DBuf &) = delete;
+    DBuf(DBuf &&o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    DBuf &operator=(DBuf &&o) noexcept
+    {
+        if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+        return *this;
+    }
+    ~DBuf() { release(); }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    hipError_t alloc(long long count)
+    {
+        release();
+        n = count;
+        if (count <= 0) return hipSuccess;
+        return hipMalloc((void **)&p, sizeof(T) * (size_t)count);
+    }
+};
+
+struct InterfaceLists {
+    int interfaceCount = 0, leftoverCount = 0;
+    DBuf<int> index, childMap, leftoverIndex, leftoverParent, leftoverNbrI, leftoverNbrJ, leftoverNbrK;
+};
+
+struct Level {
+    lbm3_info info{};
+    LevelGeom geom{};
+    int N = 0, nRows = 0, W = 0, wallCount = 0;
+    DBuf<int> shifter, ri, rj, rk, rjp, rkp, rjkp;
+    DBuf<int> wallMap, wallIndex;
+    DBuf<uint32_t> wallData, linkLength;
+    DBuf<double> gWall;
+    // open boundaries, all BCIDs concatenated in BCID order
+    int nOpenIDs = 0, nOpen = 0;
+    std::vector<int> openCount;
+    DBuf<int> openCell, openSource;
+    DBuf<uint32_t> openMeta;
+    DBuf<OpenBCRecord> openRecords;
+    DBuf<unsigned char> openRefs;    // 2 arrays of nOpen compute-type values
+    DBuf<double> openCum;            // 2 arrays of nOpen
+    int nPeriodic = 0;
+    DBuf<int> perIndex, perSource;
+    InterfaceLists C2F, F2C;
+    LocalBC bulk{};
+    DBuf<LocalBC> wallBC;            // [32]
+    DBuf<unsigned char> f;           // [27][N] storage type
+    // voxelizer output kept for flag checks
+    DBuf<long long> rayHitScan;
+    DBuf<int> rayMap;
+    bool built = false;
+};
+
+struct Sim {
+    lbm3_case C{};
+    int storage = LBM3_STORE_F32;
+    int levels = 0;
+    Level L[LBM3_MAX_LEVELS];
+    long long fluidUpdates = 0;
+    hipStream_t stream = nullptr;
+    DBuf<double> partial;            // reduction scratch
+    DBuf<double> scalar;
+    // timing of the level-0 collide-stream launches
+    bool timing = false;
+    std::vector<hipEvent_t> evStart, evStop;
+    int evUsed = 0;
+};
+
+size_t storage_bytes(int storage);
+size_t compute_bytes(int storage);
+int finish_level_setup(Sim &s, int level);  // derived tables after a level's arrays exist
+int set_error(int code, const std::string &msg);
+int check_hip(hipError_t e, const char *what);
+
+} // namespace lbm3
+
+struct lbm3_sim {
+    lbm3::Sim s;
+};

@@ -1,0 +1,487 @@
+// Per-step kernels of one grid level for gfx950.
+//
+//  k_collide_stream  fused read -> cumulant collision -> esotwist write, with
+//                    interpolated bounceback and FP64 momentum-exchange force on
+//                    wall-adjacent cells (reference updateGrid.h:51-178,
+//                    interpolatedBouncebackFunctions.h:43-157)
+//  k_open            moment-based open boundaries, all BCIDs in one launch
+//                    (updateGrid.h:183-355)
+//  k_periodic        periodic ghost copy (updateGrid.h:357-382)
+//  k_init, k_open_ref_init   initial condition (applyInitialCondition.h:8-84)
+//  k_macros, k_kinetic_energy, k_wall_force   observables, FP64 sums
+//
+// Populations are SoA planes f[q*N + cell] in the storage type S; each thread
+// owns one cell, keeps its 27 populations in registers and writes back into the
+// same 27 slots it read (esotwist), so the kernel is race free without any
+// synchronisation.  The esotwist parity is a template parameter, which makes
+// every plane offset a compile-time multiple of N.
+#include "step.hpp"
+
+namespace lbm3 {
+
+template <class S, bool FLIP>
+__device__ __forceinline__ void load_pre(const S *f, int N, const int (&nb)[8], typename StoreTraits<S>::compute (&v)[27])
+{
+#pragma unroll
+    for (int q = 0; q < 27; ++q) v[q] = StoreTraits<S>::load(f + (size_t)pre_plane(q, FLIP) * (size_t)N + nb[pre_nb(q)]);
+}
+
+template <class S, bool FLIP>
+__device__ __forceinline__ void store_pre(S *f, int N, const int (&nb)[8], const typename StoreTraits<S>::compute (&v)[27])
+{
+#pragma unroll
+    for (int q = 0; q < 27; ++q) StoreTraits<S>::store(f + (size_t)pre_plane(q, FLIP) * (size_t)N + nb[pre_nb(q)], v[q]);
+}
+
+// interpolated bounceback (Zhao & Yong 2017, single node, l = gamma) on the
+// post-collision populations of a wall-adjacent cell; opposite directions are
+// handled as pairs so both use the unmodified post-collision values.
+template <class S, bool FLIP>
+__device__ __forceinline__ void bounce_back(const StepArgs<S> &a, const int (&nb)[8], typename StoreTraits<S>::compute (&f)[27],
+                                            uint32_t wd, int wm, bool useIBB, const float (&wu)[3], double (&g3)[3])
+{
+    using T = typename StoreTraits<S>::compute;
+    const int N = a.topo.N;
+    T dRho, ux_, uy_, uz_;
+    moments(f, dRho, ux_, uy_, uz_);
+    int linkIndex = 0;
+    uint32_t code = 0u;
+    auto gamma_next = [&]() -> T {
+        if (!useIBB) return T(0.5);
+        if ((linkIndex & 3) == 0) code = a.linkLength[(size_t)(linkIndex >> 2) * (size_t)a.W + wm];
+        const uint32_t c8 = (code >> (8 * (linkIndex & 3))) & 255u;
+        ++linkIndex;
+        T g = T(c8) / T(254);
+        return g < T(0.01f) ? T(0.01f) : (g > T(1) ? T(1) : g);
+    };
+#pragma unroll
+    for (int p = 1; p < 27; p += 2) {
+        const int da = p, db = p + 1;   // opposite pair
+        const bool la = (wd >> da) & 1u, lb = (wd >> db) & 1u;
+        if (!la && !lb) continue;
+        const T fa = f[da], fb = f[db];
+        T ra = fa, rb = fb;
+        if (la) {
+            const T g = gamma_next();
+            const T inv1 = T(1) / (T(1) + g);
+            const T pre = StoreTraits<S>::load(a.f + (size_t)pre_plane(da, FLIP) * (size_t)N + nb[pre_nb(da)]);
+            const T eu = T(cx(db)) * T(wu[0]) + T(cy(db)) * T(wu[1]) + T(cz(db)) * T(wu[2]);
+            const T wall = T(6) * weight<T>(db) * (T(1) + dRho) * eu;
+            ra = ((T(1) - g) * inv1) * pre + (g * inv1) * fb + (g * inv1) * fa + inv1 * wall;
+            g3[0] += double(-fa * (T(cx(da)) - T(wu[0])) + ra * (T(cx(db)) - T(wu[0])));
+            g3[1] += double(-fa * (T(cy(da)) - T(wu[1])) + ra * (T(cy(db)) - T(wu[1])));
+            g3[2] += double(-fa * (T(cz(da)) - T(wu[2])) + ra * (T(cz(db)) - T(wu[2])));
+        }
+        if (lb) {
+            const T g = gamma_next();
+            const T inv1 = T(1) / (T(1) + g);
+            const T pre = StoreTraits<S>::load(a.f + (size_t)pre_plane(db, FLIP) * (size_t)N + nb[pre_nb(db)]);
+            const T eu = T(cx(da)) * T(wu[0]) + T(cy(da)) * T(wu[1]) + T(cz(da)) * T(wu[2]);
+            const T wall = T(6) * weight<T>(da) * (T(1) + dRho) * eu;
+            rb = ((T(1) - g) * inv1) * pre + (g * inv1) * fa + (g * inv1) * fb + inv1 * wall;
+            g3[0] += double(-fb * (T(cx(db)) - T(wu[0])) + rb * (T(cx(da)) - T(wu[0])));
+            g3[1] += double(-fb * (T(cy(db)) - T(wu[1])) + rb * (T(cy(da)) - T(wu[1])));
+            g3[2] += double(-fb * (T(cz(db)) - T(wu[2])) + rb * (T(cz(da)) - T(wu[2])));
+        }
+        f[da] = ra;
+        f[db] = rb;
+    }
+}
+
+template <class S, bool FLIP>
+__global__ __launch_bounds__(256) void k_collide_stream(StepArgs<S> a)
+{
+    using T = typename StoreTraits<S>::compute;
+    const int N = a.topo.N;
+    const int cell = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+    if (cell >= N) return;
+    const int wm = a.wallMap[cell];
+    if (wm == -3) return;
+    int nb[8];
+    decode(a.topo, cell, nb, nullptr);
+    T f[27];
+    load_pre<S, FLIP>(a.f, N, nb, f);
+
+    if (wm < 0) {
+        const LocalBC &b = a.bulk;
+        const CollisionParams P{b.gx, b.gy, b.gz, b.limiter, b.nuMult};
+        collide(f, P, T(a.nu));
+#pragma unroll
+        for (int q = 0; q < 27; ++q)
+            StoreTraits<S>::store(a.f + (size_t)post_plane(q, FLIP) * (size_t)N + nb[post_nb(q)], f[q]);
+        return;
+    }
+
+    // wall-adjacent cell
+    const uint32_t wd = a.wallData[wm];
+    const LocalBC b = a.wallBC[(wd >> 27) & 31u];
+    float wu[3] = {b.ux, b.uy, b.uz};
+    if (b.velMode == 0) {
+        T d, u0, u1, u2;
+        moments(f, d, u0, u1, u2);
+        wu[0] = float(u0); wu[1] = float(u1); wu[2] = float(u2);
+    } else if (b.velMode == 2) {
+        int nbx[8], ijk[3];
+        decode(a.topo, cell, nbx, ijk);
+        const float x = float(ijk[0]) * a.geom.res + a.geom.ox - b.origin[0];
+        const float y = float(ijk[1]) * a.geom.res + a.geom.oy - b.origin[1];
+        const float z = float(ijk[2]) * a.geom.res + a.geom.oz - b.origin[2];
+        const float s = a.geom.dtPhys / a.geom.res;
+        wu[0] = (b.omega[1] * z - b.omega[2] * y) * s;
+        wu[1] = (b.omega[2] * x - b.omega[0] * z) * s;
+        wu[2] = (b.omega[0] * y - b.omega[1] * x) * s;
+    }
+    const CollisionParams P{b.gx, b.gy, b.gz, b.limiter, b.nuMult};
+    collide(f, P, T(a.nu));
+    double g3[3] = {0.0, 0.0, 0.0};
+    bounce_back<S, FLIP>(a, nb, f, wd, wm, b.useIBB != 0, wu, g3);
+#pragma unroll
+    for (int d = 0; d < 27; ++d) {
+        S *dst;
+        if (d != 0 && ((wd >> d) & 1u)) dst = a.f + (size_t)pre_plane(inv(d), !FLIP) * (size_t)N + nb[pre_nb(inv(d))];
+        else dst = a.f + (size_t)post_plane(d, FLIP) * (size_t)N + nb[post_nb(d)];
+        StoreTraits<S>::store(dst, f[d]);
+    }
+    if (a.gWall && !(wd & 1u)) {
+        a.gWall[wm] += g3[0];
+        a.gWall[(size_t)a.W + wm] += g3[1];
+        a.gWall[2 * (size_t)a.W + wm] += g3[2];
+    }
+}
+
+template <int NI, class S, bool FLIP>
+__device__ __forceinline__ void open_one(const OpenArgs<S> &a, int x, const int (&nb)[8], uint32_t meta)
+{
+    using T = typename StoreTraits<S>::compute;
+    constexpr int nx = LBM3_BT_NORMAL[NI][0], ny = LBM3_BT_NORMAL[NI][1], nz = LBM3_BT_NORMAL[NI][2];
+    const int N = a.topo.N;
+    T f[27];
+#pragma unroll
+    for (int q = 0; q < 27; ++q) {
+        if (unknown_dir(nx, ny, nz, q)) { f[q] = T(0); continue; }
+        f[q] = StoreTraits<S>::load(a.f + (size_t)pre_plane(q, FLIP) * (size_t)N + nb[pre_nb(q)]);
+    }
+    const OpenBCRecord R = a.records[(meta >> 5) & 255u];
+    const T dRef = a.dRhoRef[x], uRef = a.uNormalRef[x];
+    T dRho, un;
+    open_cell<NI>(f, R, dRef, uRef, dRho, un);
+#pragma unroll
+    for (int q = 0; q < 27; ++q)
+        if (unknown_dir(nx, ny, nz, q))
+            StoreTraits<S>::store(a.f + (size_t)pre_plane(q, FLIP) * (size_t)N + nb[pre_nb(q)], f[q]);
+    // exponential moving average of the reference state
+    const T beta = T(R.beta);
+    T alpha = beta / (T(a.iterationsFinished) + beta + T(10));
+    int every = 1;
+    while (alpha < T(1.e-5f)) { alpha *= T(3); every *= 3; }
+    if (a.iterationsFinished % every == 0) {
+        a.dRhoRef[x] = fma(alpha, dRho - dRef, dRef);
+        a.uNormalRef[x] = fma(alpha, un - uRef, uRef);
+    }
+    if (a.dRhoCum && ((meta >> 13) & 1u)) {
+        a.dRhoCum[x] += double(dRho);
+        a.uNormalCum[x] += double(un);
+    }
+}
+
+template <class S, bool FLIP>
+__global__ __launch_bounds__(256) void k_open(OpenArgs<S> a)
+{
+    const int x = blockIdx.x * 256 + threadIdx.x;
+    if (x >= a.count) return;
+    int nb[8];
+    decode(a.topo, a.cell[x], nb, nullptr);
+    const uint32_t meta = a.meta[x];
+    switch (meta & 31u) {
+#define OPEN_CASE(n) case n: open_one<n, S, FLIP>(a, x, nb, meta); break;
+        OPEN_CASE(0) OPEN_CASE(1) OPEN_CASE(2) OPEN_CASE(3) OPEN_CASE(4) OPEN_CASE(5) OPEN_CASE(6)
+        OPEN_CASE(7) OPEN_CASE(8) OPEN_CASE(9) OPEN_CASE(10) OPEN_CASE(11) OPEN_CASE(12) OPEN_CASE(13)
+        OPEN_CASE(14) OPEN_CASE(15) OPEN_CASE(16) OPEN_CASE(17) OPEN_CASE(18) OPEN_CASE(19) OPEN_CASE(20)
+        OPEN_CASE(21) OPEN_CASE(22) OPEN_CASE(23) OPEN_CASE(24) OPEN_CASE(25)
+#undef OPEN_CASE
+    default: break;
+    }
+}
+
+template <class S, bool FLIP>
+__global__ __launch_bounds__(256) void k_periodic(S *f, Topology topo, int count, const int *index, const int *source)
+{
+    const int x = blockIdx.x * 256 + threadIdx.x;
+    if (x >= count) return;
+    using T = typename StoreTraits<S>::compute;
+    int nb[8];
+    T v[27];
+    decode(topo, source[x], nb, nullptr);
+    load_pre<S, FLIP>(f, topo.N, nb, v);
+    decode(topo, index[x], nb, nullptr);
+    store_pre<S, FLIP>(f, topo.N, nb, v);
+}
+
+template <class S, bool FLIP>
+__global__ __launch_bounds__(256) void k_init(S *f, Topology topo, InitDesc d, LevelGeom g)
+{
+    using T = typename StoreTraits<S>::compute;
+    const int cell = blockIdx.x * 256 + threadIdx.x;
+    if (cell >= topo.N) return;
+    int nb[8], ijk[3];
+    decode(topo, cell, nb, ijk);
+    float dRho = d.dRho, ux = d.ux, uy = d.uy, uz = d.uz;
+    if (d.kind == 1) {
+        const float X = (float(ijk[0]) * g.res + g.ox) / d.length;
+        const float Y = (float(ijk[1]) * g.res + g.oy) / d.length;
+        const float Z = (float(ijk[2]) * g.res + g.oz) / d.length;
+        ux = d.u0 * sinf(X) * cosf(Y) * cosf(Z);
+        uy = -d.u0 * cosf(X) * sinf(Y) * cosf(Z);
+        uz = 0.f;
+        dRho = d.pressure ? (3.f * d.u0 * d.u0 / 16.f) * (cosf(2.f * X) + cosf(2.f * Y)) * (2.f + cosf(2.f * Z)) : 0.f;
+    } else if (d.kind == 2) {
+        dRho = d.fields[4 * (size_t)cell]; ux = d.fields[4 * (size_t)cell + 1];
+        uy = d.fields[4 * (size_t)cell + 2]; uz = d.fields[4 * (size_t)cell + 3];
+    }
+    T v[27];
+#pragma unroll
+    for (int q = 0; q < 27; ++q) v[q] = feq_shifted<T>(q, T(dRho), T(ux), T(uy), T(uz));
+    store_pre<S, FLIP>(f, topo.N, nb, v);
+}
+
+template <class S, bool FLIP>
+__global__ __launch_bounds__(256) void k_open_ref_init(OpenArgs<S> a, int cx_, int cy_, int cz_)
+{
+    using T = typename StoreTraits<S>::compute;
+    const int x = blockIdx.x * 256 + threadIdx.x;
+    if (x >= a.count) return;
+    int nb[8], ijk[3];
+    decode(a.topo, a.cell[x], nb, ijk);
+    T v[27];
+    load_pre<S, FLIP>(a.f, a.topo.N, nb, v);
+    T d, ux, uy, uz;
+    moments(v, d, ux, uy, uz);
+    const int nx = ijk[0] == 0 ? -1 : (ijk[0] == cx_ - 1 ? 1 : 0);
+    const int ny = ijk[1] == 0 ? -1 : (ijk[1] == cy_ - 1 ? 1 : 0);
+    const int nz = ijk[2] == 0 ? -1 : (ijk[2] == cz_ - 1 ? 1 : 0);
+    a.dRhoRef[x] = d;
+    a.uNormalRef[x] = T(nx) * ux + T(ny) * uy + T(nz) * uz;
+}
+
+template <class S, bool FLIP>
+__global__ __launch_bounds__(256) void k_macros(const S *f, Topology topo, typename StoreTraits<S>::compute *out)
+{
+    using T = typename StoreTraits<S>::compute;
+    const int cell = blockIdx.x * 256 + threadIdx.x;
+    if (cell >= topo.N) return;
+    int nb[8];
+    decode(topo, cell, nb, nullptr);
+    T v[27];
+    load_pre<S, FLIP>(f, topo.N, nb, v);
+    T d, ux, uy, uz;
+    moments(v, d, ux, uy, uz);
+    out[4 * (size_t)cell] = d;
+    out[4 * (size_t)cell + 1] = ux;
+    out[4 * (size_t)cell + 2] = uy;
+    out[4 * (size_t)cell + 3] = uz;
+}
+
+__device__ __forceinline__ double block_sum(double v, double *lds)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) lds[wid] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (threadIdx.x == 0)
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += lds[w];
+    return s;
+}
+
+// per-block partial sums of the kinetic energy 0.5 (1 + dRho) |u|^2 over counted cells
+template <class S, bool FLIP>
+__global__ __launch_bounds__(256) void k_kinetic_energy(const S *f, Topology topo, const int *wallMap, const uint32_t *wallData,
+                                                        double *partial)
+{
+    __shared__ double lds[4];
+    using T = typename StoreTraits<S>::compute;
+    const int cell = blockIdx.x * 256 + threadIdx.x;
+    double e = 0.0;
+    if (cell < topo.N) {
+        const int wm = wallMap[cell];
+        const bool skip = wm == -3 || wm == -2 || (wm >= 0 && (wallData[wm] & 1u));
+        if (!skip) {
+            int nb[8];
+            decode(topo, cell, nb, nullptr);
+            T v[27];
+            load_pre<S, FLIP>(f, topo.N, nb, v);
+            T d, ux, uy, uz;
+            moments(v, d, ux, uy, uz);
+            e = 0.5 * (1.0 + double(d)) * (double(ux) * double(ux) + double(uy) * double(uy) + double(uz) * double(uz));
+        }
+    }
+    const double s = block_sum(e, lds);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(256) void k_sum_partials(const double *partial, int n, double *out)
+{
+    __shared__ double lds[4];
+    double v = 0.0;
+    for (int i = threadIdx.x; i < n; i += 256) v += partial[i];
+    const double s = block_sum(v, lds);
+    if (threadIdx.x == 0) *out = s;
+}
+
+// per-wall-body sums of the accumulated force and its torque about `origin`;
+// out[body*6 + {Fx,Fy,Fz,Tx,Ty,Tz}] per block
+__global__ __launch_bounds__(256) void k_wall_force(const double *gWall, int W, const uint32_t *wallData, const int *wallIndex,
+                                                    Topology topo, LevelGeom g, double ox, double oy, double oz, int nBodies,
+                                                    double *partial)
+{
+    __shared__ double acc[LBM3_MAX_BODIES * 6];
+    for (int i = threadIdx.x; i < nBodies * 6; i += 256) acc[i] = 0.0;
+    __syncthreads();
+    const int w = blockIdx.x * 256 + threadIdx.x;
+    if (w < W) {
+        const int body = (wallData[w] >> 27) & 31u;
+        if (body < nBodies) {
+            int nb[8], ijk[3];
+            decode(topo, wallIndex[w], nb, ijk);
+            const double x = double(float(ijk[0]) * g.res + g.ox) - ox;
+            const double y = double(float(ijk[1]) * g.res + g.oy) - oy;
+            const double z = double(float(ijk[2]) * g.res + g.oz) - oz;
+            const double fx = gWall[w], fy = gWall[(size_t)W + w], fz = gWall[2 * (size_t)W + w];
+            atomicAdd(&acc[body * 6 + 0], fx);
+            atomicAdd(&acc[body * 6 + 1], fy);
+            atomicAdd(&acc[body * 6 + 2], fz);
+            atomicAdd(&acc[body * 6 + 3], y * fz - z * fy);
+            atomicAdd(&acc[body * 6 + 4], z * fx - x * fz);
+            atomicAdd(&acc[body * 6 + 5], x * fy - y * fx);
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < nBodies * 6; i += 256) partial[(size_t)blockIdx.x * nBodies * 6 + i] = acc[i];
+}
+
+// unit-level entry: collide n cells stored cell-major (27 per cell)
+template <class T>
+__global__ __launch_bounds__(256) void k_collide_batch(T *f, int n, CollisionParams P, T nu)
+{
+    const int c = blockIdx.x * 256 + threadIdx.x;
+    if (c >= n) return;
+    T v[27];
+#pragma unroll
+    for (int q = 0; q < 27; ++q) v[q] = f[27 * (size_t)c + q];
+    collide(v, P, nu);
+#pragma unroll
+    for (int q = 0; q < 27; ++q) f[27 * (size_t)c + q] = v[q];
+}
+
+// ---------------------------------------------------------------------------
+// host launchers
+// ---------------------------------------------------------------------------
+static inline unsigned blocks_for(long long n) { return (unsigned)((n + 255) / 256); }
+
+template <class S>
+void launch_collide_stream(const StepArgs<S> &a, bool flip, hipStream_t s)
+{
+    if (a.topo.N <= 0) return;
+    if (flip) hipLaunchKernelGGL((k_collide_stream<S, true>), dim3(blocks_for(a.topo.N)), dim3(256), 0, s, a);
+    else hipLaunchKernelGGL((k_collide_stream<S, false>), dim3(blocks_for(a.topo.N)), dim3(256), 0, s, a);
+}
+
+template <class S>
+void launch_open(const OpenArgs<S> &a, bool flip, hipStream_t s)
+{
+    if (a.count <= 0) return;
+    if (flip) hipLaunchKernelGGL((k_open<S, true>), dim3(blocks_for(a.count)), dim3(256), 0, s, a);
+    else hipLaunchKernelGGL((k_open<S, false>), dim3(blocks_for(a.count)), dim3(256), 0, s, a);
+}
+
+template <class S>
+void launch_periodic(S *f, const Topology &t, int count, const int *index, const int *source, bool flip, hipStream_t s)
+{
+    if (count <= 0) return;
+    if (flip) hipLaunchKernelGGL((k_periodic<S, true>), dim3(blocks_for(count)), dim3(256), 0, s, f, t, count, index, source);
+    else hipLaunchKernelGGL((k_periodic<S, false>), dim3(blocks_for(count)), dim3(256), 0, s, f, t, count, index, source);
+}
+
+template <class S>
+void launch_init(S *f, const Topology &t, const InitDesc &d, const LevelGeom &g, bool flip, hipStream_t s)
+{
+    if (t.N <= 0) return;
+    if (flip) hipLaunchKernelGGL((k_init<S, true>), dim3(blocks_for(t.N)), dim3(256), 0, s, f, t, d, g);
+    else hipLaunchKernelGGL((k_init<S, false>), dim3(blocks_for(t.N)), dim3(256), 0, s, f, t, d, g);
+}
+
+template <class S>
+void launch_open_ref_init(const OpenArgs<S> &a, int X, int Y, int Z, bool flip, hipStream_t s)
+{
+    if (a.count <= 0) return;
+    if (flip) hipLaunchKernelGGL((k_open_ref_init<S, true>), dim3(blocks_for(a.count)), dim3(256), 0, s, a, X, Y, Z);
+    else hipLaunchKernelGGL((k_open_ref_init<S, false>), dim3(blocks_for(a.count)), dim3(256), 0, s, a, X, Y, Z);
+}
+
+template <class S>
+void launch_macros(const S *f, const Topology &t, typename StoreTraits<S>::compute *out, bool flip, hipStream_t s)
+{
+    if (t.N <= 0) return;
+    if (flip) hipLaunchKernelGGL((k_macros<S, true>), dim3(blocks_for(t.N)), dim3(256), 0, s, f, t, out);
+    else hipLaunchKernelGGL((k_macros<S, false>), dim3(blocks_for(t.N)), dim3(256), 0, s, f, t, out);
+}
+
+template <class S>
+int launch_kinetic_energy(const S *f, const Topology &t, const int *wallMap, const uint32_t *wallData, double *partial,
+                          double *out, bool flip, hipStream_t s)
+{
+    const unsigned nb = blocks_for(t.N);
+    if (flip) hipLaunchKernelGGL((k_kinetic_energy<S, true>), dim3(nb), dim3(256), 0, s, f, t, wallMap, wallData, partial);
+    else hipLaunchKernelGGL((k_kinetic_energy<S, false>), dim3(nb), dim3(256), 0, s, f, t, wallMap, wallData, partial);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, partial, (int)nb, out);
+    return (int)nb;
+}
+
+void launch_wall_force(const double *gWall, int W, const uint32_t *wallData, const int *wallIndex, const Topology &t,
+                       const LevelGeom &g, const double origin[3], int nBodies, double *partial, hipStream_t s)
+{
+    if (W <= 0) return;
+    hipLaunchKernelGGL(k_wall_force, dim3(blocks_for(W)), dim3(256), 0, s, gWall, W, wallData, wallIndex, t, g, origin[0],
+                       origin[1], origin[2], nBodies, partial);
+}
+
+template <class T>
+void launch_collide_batch(T *f, int n, const CollisionParams &P, T nu, hipStream_t s)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL((k_collide_batch<T>), dim3(blocks_for(n)), dim3(256), 0, s, f, n, P, nu);
+}
+
+#define LBM3_INSTANTIATE(S)                                                                                          \
+    template void launch_collide_stream<S>(const StepArgs<S> &, bool, hipStream_t);                                 \
+    template void launch_open<S>(const OpenArgs<S> &, bool, hipStream_t);                                           \
+    template void launch_periodic<S>(S *, const Topology &, int, const int *, const int *, bool, hipStream_t);        \
+    template void launch_init<S>(S *, const Topology &, const InitDesc &, const LevelGeom &, bool, hipStream_t);     \
+    template void launch_open_ref_init<S>(const OpenArgs<S> &, int, int, int, bool, hipStream_t);                   \
+    template void launch_macros<S>(const S *, const Topology &, typename StoreTraits<S>::compute *, bool, hipStream_t); \
+    template int launch_kinetic_energy<S>(const S *, const Topology &, const int *, const uint32_t *, double *,       \
+                                          double *, bool, hipStream_t);
+LBM3_INSTANTIATE(float)
+LBM3_INSTANTIATE(double)
+LBM3_INSTANTIATE(__half)
+LBM3_INSTANTIATE(__hip_bfloat16)
+template void launch_collide_batch<float>(float *, int, const CollisionParams &, float, hipStream_t);
+template void launch_collide_batch<double>(double *, int, const CollisionParams &, double, hipStream_t);
+
+} // namespace lbm3
+
+// ---------------------------------------------------------------------------
+// plain streaming copy: the bandwidth probe used to calibrate HBM counters
+// ---------------------------------------------------------------------------
+namespace lbm3 {
+__global__ __launch_bounds__(256) void k_copy(const float *__restrict__ a, float *__restrict__ b, long long n)
+{
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
+        __builtin_nontemporal_store(__builtin_nontemporal_load(a + i), b + i);
+}
+void launch_copy(const void *a, void *b, long long bytes, hipStream_t s)
+{
+    const long long n = bytes / 4;
+    hipLaunchKernelGGL(k_copy, dim3(16384), dim3(256), 0, s, (const float *)a, (float *)b, n);
+}
+} // namespace lbm3

@@ -1,0 +1,30 @@
+// Launch wrappers for the per-step kernels (step.hip).
+#pragma once
+#include "device.hpp"
+
+#define LBM3_MAX_BODIES 31
+
+namespace lbm3 {
+
+struct InitDesc {
+    int kind;                 // 0 uniform, 1 Taylor-Green, 2 per-cell fields
+    float dRho, ux, uy, uz;
+    float u0, length;
+    int pressure;
+    const float *fields;      // kind 2: [N][4] (dRho, ux, uy, uz), device
+};
+
+template <class S> void launch_collide_stream(const StepArgs<S> &a, bool flip, hipStream_t s);
+template <class S> void launch_open(const OpenArgs<S> &a, bool flip, hipStream_t s);
+template <class S> void launch_periodic(S *f, const Topology &t, int count, const int *index, const int *source, bool flip, hipStream_t s);
+template <class S> void launch_init(S *f, const Topology &t, const InitDesc &d, const LevelGeom &g, bool flip, hipStream_t s);
+template <class S> void launch_open_ref_init(const OpenArgs<S> &a, int X, int Y, int Z, bool flip, hipStream_t s);
+template <class S> void launch_macros(const S *f, const Topology &t, typename StoreTraits<S>::compute *out, bool flip, hipStream_t s);
+template <class S> int launch_kinetic_energy(const S *f, const Topology &t, const int *wallMap, const uint32_t *wallData,
+                                             double *partial, double *out, bool flip, hipStream_t s);
+void launch_wall_force(const double *gWall, int W, const uint32_t *wallData, const int *wallIndex, const Topology &t,
+                       const LevelGeom &g, const double origin[3], int nBodies, double *partial, hipStream_t s);
+template <class T> void launch_collide_batch(T *f, int n, const CollisionParams &P, T nu, hipStream_t s);
+void launch_copy(const void *a, void *b, long long bytes, hipStream_t s);
+
+} // namespace lbm3

@@ -1,0 +1,55 @@
+// Population storage in HBM: shifted populations f - w as float, half,
+// bfloat16 or double.  Arithmetic always runs in the compute type (float for
+// the 32/16-bit stores, double for the 64-bit store); conversion happens at the
+// load and the store only.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+#include <hip/hip_bf16.h>
+
+namespace lbm3 {
+
+template <class S> struct StoreTraits;
+
+template <> struct StoreTraits<float> {
+    using compute = float;
+    static __device__ __forceinline__ float load(const float *p) { return __builtin_nontemporal_load(p); }
+    static __device__ __forceinline__ void store(float *p, float v) { __builtin_nontemporal_store(v, p); }
+};
+
+template <> struct StoreTraits<double> {
+    using compute = double;
+    static __device__ __forceinline__ double load(const double *p) { return *p; }
+    static __device__ __forceinline__ void store(double *p, double v) { *p = v; }
+};
+
+template <> struct StoreTraits<__half> {
+    using compute = float;
+    static __device__ __forceinline__ float load(const __half *p)
+    {
+        const unsigned short bits = __builtin_nontemporal_load(reinterpret_cast<const unsigned short *>(p));
+        return __half2float(__ushort_as_half(bits));
+    }
+    static __device__ __forceinline__ void store(__half *p, float v)
+    {
+        __builtin_nontemporal_store(__half_as_ushort(__float2half_rn(v)), reinterpret_cast<unsigned short *>(p));
+    }
+};
+
+template <> struct StoreTraits<__hip_bfloat16> {
+    using compute = float;
+    static __device__ __forceinline__ float load(const __hip_bfloat16 *p)
+    {
+        const unsigned short bits = __builtin_nontemporal_load(reinterpret_cast<const unsigned short *>(p));
+        return __uint_as_float(static_cast<unsigned int>(bits) << 16);
+    }
+    static __device__ __forceinline__ void store(__hip_bfloat16 *p, float v)
+    {
+        // round to nearest even on the upper 16 bits (finite values)
+        unsigned int u = __float_as_uint(v);
+        u += 0x7FFFu + ((u >> 16) & 1u);
+        __builtin_nontemporal_store(static_cast<unsigned short>(u >> 16), reinterpret_cast<unsigned short *>(p));
+    }
+};
+
+} // namespace lbm3

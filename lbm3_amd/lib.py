@@ -1,0 +1,243 @@
+"""ctypes binding of liblbm3mi.so (include/lbm3mi.h).
+
+The library is loaded from the package directory; there is no fallback: if the
+HIP library is missing or no GPU is present the calls fail loudly."""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import ctypes_defs as T
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liblbm3mi.so")
+HEADER = os.path.join(HERE, "..", "include", "lbm3mi.h")
+
+_lib = None
+
+
+class LBM3Error(RuntimeError):
+    pass
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise LBM3Error("liblbm3mi.so is not built (run __graft_entry__.build())")
+    L = C.CDLL(LIB_PATH)
+    vp, i32, i64, f32, f64 = C.c_void_p, C.c_int, C.c_longlong, C.c_float, C.c_double
+    sig = {
+        "lbm3_last_error": (C.c_char_p, []),
+        "lbm3_device_info": (i32, [C.c_char_p, i32, C.POINTER(i64), C.POINTER(i32)]),
+        "lbm3_sim_create": (i32, [C.POINTER(T.Case), i32, C.POINTER(vp)]),
+        "lbm3_sim_destroy": (i32, [vp]),
+        "lbm3_build_grids": (i32, [vp, vp, vp, i32, T.Bounds, C.POINTER(i64)]),
+        "lbm3_import_level": (i32, [vp, i32, vp]),
+        "lbm3_apply_initial_condition": (i32, [vp, i32]),
+        "lbm3_apply_initial_fields": (i32, [vp, i32, vp]),
+        "lbm3_update_single_grid": (i32, [vp, i32]),
+        "lbm3_update_all_grids": (i32, [vp, i32]),
+        "lbm3_run": (i32, [vp, i32]),
+        "lbm3_synchronize": (i32, [vp]),
+        "lbm3_run_timed": (i32, [vp, i32, i32, C.POINTER(f64)]),
+        "lbm3_get_info": (i32, [vp, i32, C.POINTER(T.Info)]),
+        "lbm3_level_count": (i32, [vp]),
+        "lbm3_download_macros": (i32, [vp, i32, vp]),
+        "lbm3_download_populations": (i32, [vp, i32, vp, i64]),
+        "lbm3_upload_populations": (i32, [vp, i32, vp, i64]),
+        "lbm3_download_array": (i32, [vp, i32, C.c_char_p, vp, i64, C.POINTER(i64)]),
+        "lbm3_kinetic_energy": (i32, [vp, C.POINTER(f64)]),
+        "lbm3_wall_forces": (i32, [vp, i32, C.POINTER(f64), i32, C.POINTER(f64)]),
+        "lbm3_reset_wall_forces": (i32, [vp]),
+        "lbm3_copy_probe": (i32, [i64, i32, C.POINTER(f64)]),
+        "lbm3_collide_cells_f32": (i32, [vp, i32, C.POINTER(T.BC), f32]),
+        "lbm3_collide_cells_f64": (i32, [vp, i32, C.POINTER(T.BC), f64]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def check(rc):
+    if rc != 0:
+        raise LBM3Error("liblbm3mi error %d: %s" % (rc, load().lbm3_last_error().decode()))
+
+
+class LevelArrays(C.Structure):
+    _fields_ = [("info", T.Info), ("nRows", C.c_int)] + \
+               [(n, C.c_void_p) for n in ("shifter", "rowI", "rowJ", "rowK", "rowJPlus", "rowKPlus", "rowJKPlus")] + \
+               [("W", C.c_int)] + [(n, C.c_void_p) for n in ("wallMap", "wallIndex", "wallData", "linkLength")] + \
+               [("nOpenIDs", C.c_int)] + [(n, C.c_void_p) for n in ("openCount", "openIndex", "openSource")] + \
+               [("nPeriodic", C.c_int), ("periodicIndex", C.c_void_p), ("periodicSource", C.c_void_p),
+                ("wallCount", C.c_int)]
+
+
+STORE = {"f32": T.STORE_F32, "f16": T.STORE_F16, "bf16": T.STORE_BF16, "f64": T.STORE_F64}
+_UINT_ARRAYS = ("wallData", "linkLength", "openMeta")
+
+
+class Sim:
+    """A simulation (all grid levels) resident on the GPU."""
+
+    def __init__(self, case, storage="f32"):
+        L = load()
+        self.case = case
+        self.storage = storage
+        h = C.c_void_p()
+        check(L.lbm3_sim_create(C.byref(case), STORE[storage], C.byref(h)))
+        self.h = h
+        self.levels = case.gridLevelCount
+        self.fluid_updates = None
+        self._keep = []
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().lbm3_sim_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    # --- build ---
+    def build(self, domain, triangles=()):
+        tris = [np.ascontiguousarray(t, np.float32).reshape(-1, 9) for t in triangles]
+        allt = np.concatenate(tris) if tris else np.zeros((1, 9), np.float32)
+        counts = np.array([len(t) for t in tris] or [0], np.int32)
+        out = C.c_longlong()
+        check(load().lbm3_build_grids(self.h, allt.ctypes.data, counts.ctypes.data, len(tris), domain, C.byref(out)))
+        self.fluid_updates = out.value
+        return out.value
+
+    def import_level(self, level, arrays, info):
+        """Upload a level from named host arrays (the gridBuilderToGrid layout)."""
+        keep = {k: np.ascontiguousarray(v) for k, v in arrays.items()}
+        self._keep.append(keep)
+        a = LevelArrays()
+        a.info = info
+        a.nRows = len(keep["rowI"])
+        for n in ("shifter", "rowI", "rowJ", "rowK", "rowJPlus", "rowKPlus", "rowJKPlus", "wallMap", "wallIndex",
+                  "wallData", "linkLength", "openCount", "openIndex", "openSource", "periodicIndex", "periodicSource"):
+            setattr(a, n, keep[n].ctypes.data if keep[n].size else None)
+        a.W = len(keep["wallIndex"])
+        a.nOpenIDs = len(keep["openCount"])
+        a.nPeriodic = len(keep["periodicIndex"])
+        a.wallCount = int(np.asarray(keep.get("wallCount", 0)).reshape(-1)[0])
+        check(load().lbm3_import_level(self.h, level, C.byref(a)))
+
+    # --- run ---
+    def init(self, level=None):
+        for L in range(self.levels) if level is None else [level]:
+            check(load().lbm3_apply_initial_condition(self.h, L))
+        self.sync()
+
+    def init_fields(self, level, fields):
+        f = np.ascontiguousarray(fields, np.float32)
+        check(load().lbm3_apply_initial_fields(self.h, level, f.ctypes.data))
+
+    def run(self, steps):
+        check(load().lbm3_run(self.h, steps))
+
+    def step_level(self, level):
+        check(load().lbm3_update_single_grid(self.h, level))
+
+    def sync(self):
+        check(load().lbm3_synchronize(self.h))
+
+    def run_timed(self, steps, kernel_events=False):
+        """(total ms, summed level-0 collide-stream kernel ms, launches) over `steps` coarse steps."""
+        out = (C.c_double * 3)()
+        check(load().lbm3_run_timed(self.h, steps, int(kernel_events), out))
+        return out[0], out[1], int(out[2])
+
+    # --- state ---
+    def info(self, level=0):
+        i = T.Info()
+        check(load().lbm3_get_info(self.h, level, C.byref(i)))
+        return i
+
+    def macros(self, level=0):
+        n = self.info(level).cellCount
+        out = np.zeros((n, 4), np.float64 if self.storage == "f64" else np.float32)
+        check(load().lbm3_download_macros(self.h, level, out.ctypes.data))
+        return out
+
+    def populations(self, level=0):
+        n = self.info(level).cellCount
+        dt = {"f32": np.float32, "f64": np.float64, "f16": np.float16, "bf16": np.uint16}[self.storage]
+        out = np.zeros((27, n), dt)
+        check(load().lbm3_download_populations(self.h, level, out.ctypes.data, out.nbytes))
+        return out
+
+    def set_populations(self, level, f):
+        f = np.ascontiguousarray(f)
+        check(load().lbm3_upload_populations(self.h, level, f.ctypes.data, f.nbytes))
+
+    def array(self, name, level=0):
+        cnt = C.c_longlong()
+        check(load().lbm3_download_array(self.h, level, name.encode(), None, 0, C.byref(cnt)))
+        n = max(cnt.value, 0)
+        if name in ("rayHitScan",):
+            dt = np.int64
+        elif name in ("wallForce",):
+            dt = np.float64
+        elif name == "openRefs":
+            dt = np.float64 if self.storage == "f64" else np.float32
+        elif name in _UINT_ARRAYS:
+            dt = np.uint32
+        else:
+            dt = np.int32
+        out = np.zeros(n, dt)
+        if n:
+            check(load().lbm3_download_array(self.h, level, name.encode(), out.ctypes.data, n, C.byref(cnt)))
+        return out
+
+    def kinetic_energy(self):
+        v = C.c_double()
+        check(load().lbm3_kinetic_energy(self.h, C.byref(v)))
+        return v.value
+
+    def wall_forces(self, level=0, origin=(0.0, 0.0, 0.0), bodies=1):
+        o = (C.c_double * 3)(*origin)
+        out = (C.c_double * (6 * bodies))()
+        check(load().lbm3_wall_forces(self.h, level, o, bodies, out))
+        return np.array(out[:]).reshape(bodies, 6)
+
+    def reset_wall_forces(self):
+        check(load().lbm3_reset_wall_forces(self.h))
+
+
+def collide_cells(f, bc, nu):
+    """applyCollision on the GPU for cell-major populations f[n, 27] (float32 or float64)."""
+    f = np.ascontiguousarray(f)
+    n = f.shape[0]
+    if f.dtype == np.float32:
+        check(load().lbm3_collide_cells_f32(f.ctypes.data, n, C.byref(bc), nu))
+    else:
+        check(load().lbm3_collide_cells_f64(f.ctypes.data, n, C.byref(bc), nu))
+    return f
+
+
+def device_info():
+    name = C.create_string_buffer(256)
+    mem = C.c_longlong()
+    cus = C.c_int()
+    check(load().lbm3_device_info(name, 256, C.byref(mem), C.byref(cus)))
+    return name.value.decode(), mem.value, cus.value
+
+
+def exported_symbols_in_header():
+    import re
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"^(?:const char \*|int |void )\s*(lbm3_\w+)\s*\(", text, re.M)))
+
+
+def copy_probe(nbytes, reps=20):
+    """Milliseconds per device-to-device copy of nbytes (nbytes read + nbytes written)."""
+    ms = C.c_double()
+    check(load().lbm3_copy_probe(int(nbytes), int(reps), C.byref(ms)))
+    return ms.value

@@ -1,0 +1,109 @@
+/*
+ * ORACLE -- test infrastructure, not product code.
+ *
+ * A plain-C CPU restatement of the LBM3 hot path (reference stepanmuller/LBM3,
+ * headers under include/), used only by tests/, __graft_entry__.smoke() and the CPU
+ * baseline leg of bench.py, as the checker.  Every function cites the
+ * reference file:line it restates.  The product (lbm3_amd/, HIP for gfx950)
+ * never links or calls anything here.
+ */
+#ifndef LBM3_ORACLE_H
+#define LBM3_ORACLE_H
+
+#include <stdint.h>
+#include "../include/lbm3mi_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* D3Q27 direction ids, D3Q27Directions.h:8-34 */
+enum { OOO = 0, POO, MOO, OOM, OOP, OMO, OPO, POM, MOP, POP, MOM, MMO, PPO, OPM, OMP,
+       MPO, PMO, OPP, OMM, MPM, PMP, MMP, PPM, PMM, MPP, MMM, PPP };
+
+extern const int ORC_CX[27], ORC_CY[27], ORC_CZ[27], ORC_INV[27];
+
+/* ---- numerics (numerics.c / real_impl.h) ---- */
+void orc_feq_f32(float dRho, float ux, float uy, float uz, float *feq);
+void orc_feq_f64(double dRho, double ux, double uy, double uz, double *feq);
+void orc_moments_f32(const float *f, float *out4);
+void orc_moments_f64(const double *f, double *out4);
+
+/* ---- STL (builder.c) ---- */
+typedef struct orc_stl {
+    int triangleCount;
+    float *ax, *ay, *az, *bx, *by, *bz, *cx, *cy, *cz;
+    lbm3_bounds Bounds;
+    float binSize, oxBin, oyBin, ozBin;
+    int binCountX, binCountY, binCountZ;
+    int *binArray, *firstInBin;
+    int *edgeCount; /* [triangleCount][3] shared-edge counts */
+    int excluded;   /* triangles dropped for repeated vertices */
+} orc_stl;
+
+typedef struct orc_raymap {
+    int gridID;
+    long long totalHitCount;
+    int nRays;
+    long long *hitScan; /* nRays + 1 */
+    int *rayMap;        /* totalHitCount */
+} orc_raymap;
+
+typedef struct orc_interface {
+    int interfaceCount;
+    int *index, *childMap;
+    int leftoverCount;
+    int *leftoverIndex, *leftoverParent, *leftoverNbrI, *leftoverNbrJ, *leftoverNbrK;
+} orc_interface;
+
+typedef struct orc_openbc {
+    int openBCID, count, trackFlowCount;
+    int *index, *source;
+    void *dRhoRef, *uNormalRef, *dRhoCum, *uNormalCum; /* REAL[count] */
+} orc_openbc;
+
+typedef struct orc_grid {
+    lbm3_info Info;
+    /* compressed IJK/NBR, types.h:162-164 */
+    int *shifter; int nRows;
+    int *ri, *rj, *rk, *rjp, *rkp, *rjkp;
+    /* walls, types.h:202-204 */
+    int *wallMap; int W; int *wallIndex; uint32_t *wallData; uint32_t *linkLength; /* [7][W] */
+    void *gWall; /* REAL[3][W] force accumulators */
+    orc_interface C2F, F2C;
+    int nOpen; orc_openbc open[8];
+    int nPeriodic; int *perIndex, *perSource;
+    int wallCount; /* cells marked wall by the builder */
+    void *f; /* REAL[27][N], index q*N + cell */
+} orc_grid;
+
+typedef struct orc_sim {
+    lbm3_case C;
+    int isDouble;
+    int levels;
+    orc_grid g[LBM3_MAX_LEVELS];
+    long long fluidUpdatesPerIteration;
+    int nStatic; orc_stl *statics;
+    /* voxelizer results per level (rayMapTotal), kept for flag parity tests */
+    orc_raymap rayTotal[LBM3_MAX_LEVELS];
+    char error[256];
+} orc_sim;
+
+/* STL */
+int orc_stl_read(orc_stl *stl, const char *path, float resGlobal);
+int orc_stl_from_triangles(orc_stl *stl, const float *tri9, int n, float resGlobal);
+void orc_stl_free(orc_stl *stl);
+
+/* build + run */
+orc_sim *orc_sim_create(const lbm3_case *C, int isDouble);
+int orc_build_grids(orc_sim *S, const orc_stl *statics, int nStatic, lbm3_bounds Domain);
+void orc_apply_initial_condition(orc_sim *S, int level);
+void orc_update_all_grids(orc_sim *S, int level);
+void orc_update_single_grid(orc_sim *S, int level);
+void orc_update_interface(orc_sim *S, int coarse);
+void orc_sim_free(orc_sim *S);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,221 @@
+/*
+ * ORACLE (test infrastructure only): instantiates the per-step restatement for
+ * float and double and provides the driver entry points.
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include "oracle.h"
+#include "../lbm3_amd/csrc/generated/boundary_tables.h"
+
+lbm3_bc orc_hook_open_bc(const lbm3_case *C, const lbm3_info *I, int i, int j, int k);
+
+/* the per-cell numerics are compiled in numerics.c; re-include them here
+   (static inline helpers + the bcr type) under the same names */
+#define ORC_STEP_UNIT 1
+
+#define REAL float
+#define NAME(x) x##_f32
+#define ORC_ABS fabsf
+#define ORC_SQRT sqrtf
+#define ORC_FMA fmaf
+#define ORC_INVSQRT3 0.57735026918962576f
+#define orc_feq_f32 orc_feq_f32_s
+#define orc_moments_f32 orc_moments_f32_s
+#define orc_collide_f32 orc_collide_f32_s
+#define orc_restore_rho_f32 orc_restore_rho_f32_s
+#define orc_restore_u_f32 orc_restore_u_f32_s
+#define orc_mbbc_f32 orc_mbbc_f32_s
+#include "real_impl.h"
+#include "step_impl.h"
+#undef REAL
+#undef NAME
+#undef ORC_ABS
+#undef ORC_SQRT
+#undef ORC_FMA
+#undef ORC_INVSQRT3
+
+#define REAL double
+#define NAME(x) x##_f64
+#define ORC_ABS fabs
+#define ORC_SQRT sqrt
+#define ORC_FMA fma
+#define ORC_INVSQRT3 0.57735026918962576
+#define orc_feq_f64 orc_feq_f64_s
+#define orc_moments_f64 orc_moments_f64_s
+#define orc_collide_f64 orc_collide_f64_s
+#define orc_restore_rho_f64 orc_restore_rho_f64_s
+#define orc_restore_u_f64 orc_restore_u_f64_s
+#define orc_mbbc_f64 orc_mbbc_f64_s
+#include "real_impl.h"
+#include "step_impl.h"
+#undef REAL
+#undef NAME
+
+orc_sim *orc_sim_create(const lbm3_case *C, int isDouble)
+{
+    orc_sim *S = (orc_sim *)calloc(1, sizeof(orc_sim));
+    S->C = *C;
+    S->isDouble = isDouble;
+    return S;
+}
+
+void orc_apply_initial_condition(orc_sim *S, int L)
+{
+    if (S->isDouble) orc_init_level_f64(S, L);
+    else orc_init_level_f32(S, L);
+}
+
+void orc_update_single_grid(orc_sim *S, int L)
+{
+    if (S->isDouble) orc_step_level_f64(S, L);
+    else orc_step_level_f32(S, L);
+}
+
+void orc_update_interface(orc_sim *S, int coarse);
+
+/* updateAllGrids, updateGrid.h:388-396 */
+void orc_update_all_grids(orc_sim *S, int L)
+{
+    orc_update_single_grid(S, L);
+    if (L < S->levels - 1) {
+        for (int i = 0; i < 2; ++i) orc_update_all_grids(S, L + 1);
+        orc_update_interface(S, L);
+    }
+}
+
+void orc_update_interface(orc_sim *S, int coarse)
+{
+    (void)S; (void)coarse; /* multilevel transfer: interface.c */
+}
+
+/* ---- ctypes accessors ---- */
+int orc_levels(const orc_sim *S) { return S->levels; }
+long long orc_fluid_updates(const orc_sim *S) { return S->fluidUpdatesPerIteration; }
+const char *orc_error(const orc_sim *S) { return S->error; }
+void orc_info(const orc_sim *S, int L, lbm3_info *out) { *out = S->g[L].Info; }
+
+/* named arrays of a level: returns pointer and element count, elemSize in bytes */
+void *orc_array(orc_sim *S, int L, const char *name, long long *count, int *elemSize)
+{
+    orc_grid *g = &S->g[L];
+    const int N = g->Info.cellCount;
+    const int rs = S->isDouble ? 8 : 4;
+    *elemSize = 4;
+#define A(nm, p, n) if (!strcmp(name, nm)) { *count = (n); return (void *)(p); }
+    A("shifter", g->shifter, N) A("rowI", g->ri, g->nRows) A("rowJ", g->rj, g->nRows) A("rowK", g->rk, g->nRows)
+    A("rowJPlus", g->rjp, g->nRows) A("rowKPlus", g->rkp, g->nRows) A("rowJKPlus", g->rjkp, g->nRows)
+    A("wallMap", g->wallMap, N) A("wallIndex", g->wallIndex, g->W) A("wallData", g->wallData, g->W)
+    A("linkLength", g->linkLength, 7LL * g->W) A("periodicIndex", g->perIndex, g->nPeriodic)
+    A("periodicSource", g->perSource, g->nPeriodic)
+    A("f2cIndex", g->F2C.index, g->F2C.interfaceCount) A("f2cChild", g->F2C.childMap, g->F2C.interfaceCount)
+    A("c2fIndex", g->C2F.index, g->C2F.interfaceCount) A("c2fChild", g->C2F.childMap, g->C2F.interfaceCount)
+    A("leftoverIndex", g->C2F.leftoverIndex, g->C2F.leftoverCount)
+    A("leftoverParent", g->C2F.leftoverParent, g->C2F.leftoverCount)
+    A("leftoverNbrI", g->C2F.leftoverNbrI, g->C2F.leftoverCount)
+    A("leftoverNbrJ", g->C2F.leftoverNbrJ, g->C2F.leftoverCount)
+    A("leftoverNbrK", g->C2F.leftoverNbrK, g->C2F.leftoverCount)
+    *elemSize = rs;
+    A("f", g->f, 27LL * N) A("wallForce", g->gWall, 3LL * g->W)
+    *elemSize = 4;
+    if (!strncmp(name, "open", 4)) {
+        const int b = name[4] - '0';
+        if (b >= 0 && b < g->nOpen) {
+            orc_openbc *O = &g->open[b];
+            const char *f = name + 5;
+#define B(nm, p, n) if (!strcmp(f, nm)) { *count = (n); return (void *)(p); }
+            B("Index", O->index, O->count) B("Source", O->source, O->count)
+            *elemSize = rs;
+            B("DRhoRef", O->dRhoRef, O->count) B("UNormalRef", O->uNormalRef, O->count)
+            B("DRhoCum", O->dRhoCum, O->count) B("UNormalCum", O->uNormalCum, O->count)
+#undef B
+        }
+    }
+    if (!strcmp(name, "rayHitScan")) { *elemSize = 8; *count = S->rayTotal[L].nRays + 1; return S->rayTotal[L].hitScan; }
+    if (!strcmp(name, "rayMap")) { *count = S->rayTotal[L].totalHitCount; return S->rayTotal[L].rayMap; }
+#undef A
+    *count = -1;
+    return NULL;
+}
+int orc_open_count(const orc_sim *S, int L) { return S->g[L].nOpen; }
+
+void orc_macros(const orc_sim *S, int L, void *out4)
+{
+    if (S->isDouble) orc_macros_level_f64(S, L, (double *)out4);
+    else orc_macros_level_f32(S, L, (float *)out4);
+}
+
+/* kinetic energy sum in double, cases/taylorGreenVortex/taylorGreenVortexMain.cu:116-180
+   (returns the lattice sum 0.5 (1 + dRho) |u|^2 over counted cells of every level,
+   each weighted by res^3 relative to level 0) */
+double orc_kinetic_energy(const orc_sim *S)
+{
+    double total = 0.0;
+    for (int L = 0; L < S->levels; ++L) {
+        const orc_grid *g = &S->g[L];
+        const int N = g->Info.cellCount;
+        const int dbl = S->isDouble;
+        void *m = malloc((size_t)N * 4 * (dbl ? 8 : 4));
+        orc_macros(S, L, m);
+        double sum = 0.0;
+        for (int c = 0; c < N; ++c) {
+            const int wm = g->wallMap[c];
+            if (wm == -3 || wm == -2) continue;
+            if (wm >= 0 && (g->wallData[wm] & 1u)) continue;
+            double d, ux, uy, uz;
+            if (dbl) { const double *p = (const double *)m + 4 * (size_t)c; d = p[0]; ux = p[1]; uy = p[2]; uz = p[3]; }
+            else { const float *p = (const float *)m + 4 * (size_t)c; d = p[0]; ux = p[1]; uy = p[2]; uz = p[3]; }
+            sum += 0.5 * (1.0 + d) * (ux * ux + uy * uy + uz * uz);
+        }
+        free(m);
+        const double scale = 1.0 / (double)(1 << (3 * L));
+        total += sum * scale;
+    }
+    return total;
+}
+
+/* advance the whole hierarchy n coarse steps */
+void orc_run(orc_sim *S, int steps)
+{
+    for (int s = 0; s < steps; ++s) orc_update_all_grids(S, 0);
+}
+
+int orc_build_from_triangles(orc_sim *S, const float *tri, const int *triCounts, int nStatic, lbm3_bounds Domain)
+{
+    orc_stl *stl = (orc_stl *)calloc((size_t)(nStatic > 0 ? nStatic : 1), sizeof(orc_stl));
+    int off = 0;
+    for (int s = 0; s < nStatic; ++s) {
+        if (orc_stl_from_triangles(&stl[s], tri + 9 * (size_t)off, triCounts[s], S->C.resGlobal) != 0) return -20;
+        off += triCounts[s];
+    }
+    const int rc = orc_build_grids(S, stl, nStatic, Domain);
+    S->statics = stl;
+    return rc;
+}
+
+void orc_init_all(orc_sim *S)
+{
+    for (int L = 0; L < S->levels; ++L) orc_apply_initial_condition(S, L);
+}
+
+void orc_sim_free(orc_sim *S)
+{
+    if (!S) return;
+    for (int L = 0; L < S->levels; ++L) {
+        orc_grid *g = &S->g[L];
+        free(g->shifter); free(g->ri); free(g->rj); free(g->rk); free(g->rjp); free(g->rkp); free(g->rjkp);
+        free(g->wallMap); free(g->wallIndex); free(g->wallData); free(g->linkLength); free(g->gWall);
+        free(g->perIndex); free(g->perSource); free(g->f);
+        free(g->F2C.index); free(g->F2C.childMap); free(g->C2F.index); free(g->C2F.childMap);
+        free(g->C2F.leftoverIndex); free(g->C2F.leftoverParent); free(g->C2F.leftoverNbrI);
+        free(g->C2F.leftoverNbrJ); free(g->C2F.leftoverNbrK);
+        for (int b = 0; b < g->nOpen; ++b) {
+            free(g->open[b].index); free(g->open[b].source); free(g->open[b].dRhoRef);
+            free(g->open[b].uNormalRef); free(g->open[b].dRhoCum); free(g->open[b].uNormalCum);
+        }
+        free(S->rayTotal[L].hitScan); free(S->rayTotal[L].rayMap);
+    }
+    for (int s = 0; s < S->nStatic; ++s) orc_stl_free(&S->statics[s]);
+    free(S->statics);
+    free(S);
+}

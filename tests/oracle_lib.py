@@ -1,0 +1,121 @@
+"""ctypes access to the CPU oracle (oracle/_build/liboracle.so) -- the checker.
+
+Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg use this."""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from lbm3_amd import ctypes_defs as T
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB = os.path.join(ROOT, "oracle", "_build", "liboracle.so")
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle")], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = C.CDLL(LIB)
+        L.orc_sim_create.restype = C.c_void_p
+        L.orc_sim_create.argtypes = [C.POINTER(T.Case), C.c_int]
+        L.orc_build_from_triangles.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, T.Bounds]
+        L.orc_build_from_triangles.restype = C.c_int
+        L.orc_error.restype = C.c_char_p
+        L.orc_error.argtypes = [C.c_void_p]
+        L.orc_levels.argtypes = [C.c_void_p]
+        L.orc_fluid_updates.restype = C.c_longlong
+        L.orc_fluid_updates.argtypes = [C.c_void_p]
+        L.orc_info.argtypes = [C.c_void_p, C.c_int, C.POINTER(T.Info)]
+        L.orc_array.restype = C.c_void_p
+        L.orc_array.argtypes = [C.c_void_p, C.c_int, C.c_char_p, C.POINTER(C.c_longlong), C.POINTER(C.c_int)]
+        L.orc_open_count.argtypes = [C.c_void_p, C.c_int]
+        L.orc_macros.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
+        L.orc_kinetic_energy.restype = C.c_double
+        L.orc_kinetic_energy.argtypes = [C.c_void_p]
+        L.orc_run.argtypes = [C.c_void_p, C.c_int]
+        L.orc_init_all.argtypes = [C.c_void_p]
+        L.orc_sim_free.argtypes = [C.c_void_p]
+        L.orc_update_single_grid.argtypes = [C.c_void_p, C.c_int]
+        L.orc_collide_batch_f32.argtypes = [C.c_void_p, C.c_int, C.POINTER(T.BC), C.c_float]
+        L.orc_collide_batch_f64.argtypes = [C.c_void_p, C.c_int, C.POINTER(T.BC), C.c_double]
+        L.orc_feq_batch_f32.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
+        L.orc_feq_batch_f64.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
+        L.orc_moments_batch_f32.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
+        L.orc_moments_batch_f64.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
+        L.orc_openbc_cell_f32.argtypes = [C.c_int, C.c_int, C.POINTER(T.BC), C.c_void_p]
+        _lib = L
+    return _lib
+
+
+_INT_DTYPES = {4: np.int32, 8: np.int64}
+
+
+class OracleSim:
+    """One simulation (all levels) on the CPU oracle."""
+
+    def __init__(self, case, domain, triangles=(), double=False):
+        L = lib()
+        self.double = double
+        self.case = case
+        self.h = L.orc_sim_create(C.byref(case), int(double))
+        tris = [np.ascontiguousarray(t, np.float32).reshape(-1, 9) for t in triangles]
+        allt = np.concatenate(tris) if tris else np.zeros((1, 9), np.float32)
+        counts = np.array([len(t) for t in tris] or [0], np.int32)
+        rc = L.orc_build_from_triangles(self.h, allt.ctypes.data, counts.ctypes.data, len(tris), domain)
+        if rc != 0:
+            raise RuntimeError("oracle build failed (%d): %s" % (rc, L.orc_error(self.h).decode()))
+        self.levels = L.orc_levels(self.h)
+        self.fluid_updates = L.orc_fluid_updates(self.h)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_sim_free(self.h)
+            self.h = None
+
+    def info(self, level=0):
+        i = T.Info()
+        lib().orc_info(self.h, level, C.byref(i))
+        return i
+
+    def array(self, name, level=0):
+        n = C.c_longlong()
+        es = C.c_int()
+        p = lib().orc_array(self.h, level, name.encode(), C.byref(n), C.byref(es))
+        if n.value < 0:
+            raise KeyError(name)
+        if name in ("f", "wallForce") or name.startswith("open") and name[5:] in ("DRhoRef", "UNormalRef", "DRhoCum", "UNormalCum"):
+            dt = np.float64 if es.value == 8 else np.float32
+        elif name in ("wallData", "linkLength"):
+            dt = np.uint32
+        else:
+            dt = _INT_DTYPES[es.value]
+        if n.value == 0 or not p:
+            return np.zeros(0, dt)
+        buf = (C.c_char * (n.value * es.value)).from_address(p)
+        return np.frombuffer(buf, dtype=dt).copy()
+
+    def open_count(self, level=0):
+        return lib().orc_open_count(self.h, level)
+
+    def init(self):
+        lib().orc_init_all(self.h)
+
+    def run(self, steps):
+        lib().orc_run(self.h, steps)
+
+    def macros(self, level=0):
+        n = self.info(level).cellCount
+        out = np.zeros((n, 4), np.float64 if self.double else np.float32)
+        lib().orc_macros(self.h, level, out.ctypes.data)
+        return out
+
+    def kinetic_energy(self):
+        return lib().orc_kinetic_energy(self.h)

@@ -1,0 +1,165 @@
+"""GPU (HIP, gfx950) vs CPU oracle parity of the per-step hot path.
+
+Bars (per the hot-path contract): FP32 macroscopic fields rho, u within 1e-5
+relative max-norm after 100 steps; the FP64 build within 1e-10.  Grids are built
+by the oracle and imported, so these tests isolate the per-step kernels;
+builder parity is covered separately."""
+import numpy as np
+import pytest
+
+from lbm3_amd import cases
+from lbm3_amd import ctypes_defs as T
+from lbm3_amd import lib
+from tests.helpers import level_arrays, rel_max
+from tests.oracle_lib import OracleSim, lib as orc
+
+pytestmark = pytest.mark.gpu
+
+
+def _random_cells(n, seed, dtype):
+    rng = np.random.default_rng(seed)
+    m4 = np.zeros((n, 4), dtype)
+    m4[:, 0] = rng.uniform(-0.02, 0.02, n)
+    m4[:, 1:] = rng.uniform(-0.06, 0.06, (n, 3))
+    feq = np.zeros((n, 27), dtype)
+    fn = orc().orc_feq_batch_f32 if dtype == np.float32 else orc().orc_feq_batch_f64
+    fn(m4.ctypes.data, n, feq.ctypes.data)
+    w = np.array([8 / 27] + [2 / 27] * 6 + [1 / 54] * 12 + [1 / 216] * 8)
+    return np.ascontiguousarray(feq + (0.01 * w * rng.standard_normal((n, 27))).astype(dtype))
+
+
+BCS = [dict(), dict(collisionLimiter=0.0), dict(collisionLimiter=0.1, gx=1e-4, gy=-2e-5, gz=3e-5),
+       dict(collisionLimiter=0.05, nuMultiplier=2.0)]
+
+
+@pytest.mark.parametrize("bcargs", BCS)
+@pytest.mark.parametrize("nu", [3.2e-4, 0.02, 0.16])
+def test_collision_f32_matches_oracle(bcargs, nu):
+    f = _random_cells(4096, 7, np.float32)
+    bc = T.default_bc(**bcargs)
+    ref = f.copy()
+    orc().orc_collide_batch_f32(ref.ctypes.data, len(ref), bc, nu)
+    got = lib.collide_cells(f.copy(), bc, nu)
+    # one collision: a few float ulps of the population scale
+    assert rel_max(got, ref) < 2e-6
+
+
+@pytest.mark.parametrize("bcargs", BCS)
+def test_collision_f64_matches_oracle(bcargs):
+    f = _random_cells(2048, 11, np.float64)
+    bc = T.default_bc(**bcargs)
+    ref = f.copy()
+    orc().orc_collide_batch_f64(ref.ctypes.data, len(ref), bc, 0.01)
+    got = lib.collide_cells(f.copy(), bc, 0.01)
+    assert rel_max(got, ref) < 1e-13
+
+
+def _pair(case, domain, tris, storage):
+    o = OracleSim(case, domain, tris, double=(storage == "f64"))
+    g = lib.Sim(case, storage)
+    for L in range(o.levels):
+        g.import_level(L, level_arrays(o, L), o.info(L))
+    o.init()
+    g.init()
+    if storage == "f64":
+        # the initial-condition hooks evaluate sin/cos in float, whose last bit differs
+        # between libm and the device; start the FP64 comparison from identical populations
+        for L in range(o.levels):
+            g.set_populations(L, o.array("f", L).reshape(27, -1))
+    return o, g
+
+
+def _compare(o, g, level=0):
+    mo, mg = o.macros(level), g.macros(level)
+    rho_err = rel_max(1.0 + mg[:, 0], 1.0 + mo[:, 0])
+    u_err = rel_max(mg[:, 1:], mo[:, 1:])
+    return rho_err, u_err
+
+
+@pytest.mark.parametrize("n", [16, 32])
+def test_taylor_green_100_steps_f32(n):
+    case, dom, tris = cases.taylor_green(n, u0=0.05, reynolds=1600.0)
+    o, g = _pair(case, dom, tris, "f32")
+    r0, u0 = _compare(o, g)
+    assert r0 < 1e-6 and u0 < 1e-6, (r0, u0)
+    o.run(100)
+    g.run(100)
+    g.sync()
+    r, u = _compare(o, g)
+    print("TGV %d^3 after 100 steps: rel max rho %.2e, u %.2e" % (n, r, u))
+    assert r < 1e-5 and u < 1e-5, (r, u)
+
+
+def test_taylor_green_100_steps_f64():
+    case, dom, tris = cases.taylor_green(16, u0=0.05, reynolds=1600.0)
+    o, g = _pair(case, dom, tris, "f64")
+    o.run(100)
+    g.run(100)
+    g.sync()
+    r, u = _compare(o, g)
+    assert r < 1e-10 and u < 1e-10, (r, u)
+
+
+def _small_sphere(**kw):
+    return cases.sphere(64, 32, 32, diameter=8.0, center=(16.0, 16.0, 16.0), segments=(24, 48), **kw)
+
+
+def test_sphere_100_steps_f32():
+    """Reference sphere-case collision (K15 where the open boundaries are): 1e-5 bar."""
+    case, dom, tris = _small_sphere()
+    o, g = _pair(case, dom, tris, "f32")
+    o.run(100)
+    g.run(100)
+    g.sync()
+    r, u = _compare(o, g)
+    print("sphere 64x32x32 after 100 steps: rel max rho %.2e, u %.2e" % (r, u))
+    assert r < 1e-5 and u < 1e-5, (r, u)
+    # FP64 momentum-exchange force sum vs the oracle's per-cell float accumulators
+    fo = o.array("wallForce").reshape(3, -1).astype(np.float64).sum(axis=1)
+    fg = g.wall_forces(0, bodies=1)[0, :3]
+    assert np.allclose(fg, fo, rtol=1e-4, atol=1e-6 * np.abs(fo).max()), (fg, fo)
+    # open-boundary reference state
+    refs = g.array("openRefs")
+    n = len(refs) // 2
+    ro = np.concatenate([o.array("open%dDRhoRef" % b) for b in range(o.open_count())])
+    assert np.max(np.abs(refs[:n] - ro)) < 1e-6
+
+
+def test_sphere_k17_100_steps_f32_within_fp32_sensitivity():
+    """K17 next to the impulsively started sphere and the outlet corners is
+    ill-conditioned in FP32: the oracle's own FP32 and FP64 runs differ by
+    ~1e-4 after 100 steps.  The GPU FP32 run must stay within a small multiple
+    of that sensitivity; exact agreement is checked by the FP64 test below."""
+    case, dom, tris = _small_sphere(k15_on_level0=False)
+    o, g = _pair(case, dom, tris, "f32")
+    o64 = OracleSim(case, dom, tris, double=True)
+    o64.init()
+    for x in (o, g, o64):
+        x.run(100)
+    g.sync()
+    r, u = _compare(o, g)
+    m64, mo = o64.macros(), o.macros()
+    sens = rel_max(mo[:, 1:], m64[:, 1:])
+    print("K17 sphere: GPU vs oracle FP32 %.2e, oracle FP32 vs FP64 %.2e" % (u, sens))
+    assert r < 1e-5 and u < 3 * sens + 1e-6, (r, u, sens)
+
+
+@pytest.mark.parametrize("k15", [True, False])
+def test_sphere_100_steps_f64(k15):
+    case, dom, tris = _small_sphere(k15_on_level0=k15)
+    o, g = _pair(case, dom, tris, "f64")
+    o.run(100)
+    g.run(100)
+    g.sync()
+    r, u = _compare(o, g)
+    assert r < 1e-10 and u < 1e-10, (r, u)
+
+
+def test_channel_periodic_and_moment_walls_f32():
+    case, dom, tris = cases.channel(32, 16, 16, force=1e-5)
+    o, g = _pair(case, dom, tris, "f32")
+    o.run(100)
+    g.run(100)
+    g.sync()
+    r, u = _compare(o, g)
+    assert r < 1e-5 and u < 1e-5, (r, u)
