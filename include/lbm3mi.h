@@ -32,7 +32,7 @@ typedef struct lbm3_sim lbm3_sim;
 
 /* Host-side arrays of one built level, as produced by the reference builder
  * gridBuilderToGrid (gridBuilderFunctions.h:1032-1569).  Used by
- * lbm3_import_level and filled by lbm3_export_level. */
+ * lbm3_import_level; lbm3_download_array reads the same arrays back. */
 typedef struct lbm3_level_arrays {
     lbm3_info info;
     int nRows;
@@ -87,8 +87,10 @@ int lbm3_synchronize(lbm3_sim *s);
 
 /* Timed run: `steps` coarse steps between HIP events on the sim's stream.
  * out[0] = total ms; with kernelEvents != 0 also out[1] = summed ms of the
- * level-0 collide-stream kernel (events around each launch) and out[2] = the
- * number of those launches.  No counterpart in the reference (TNL::Timer). */
+ * level-0 bulk collide-stream kernel (events around each launch), out[2] = the
+ * number of those launches and out[3] = summed ms of the level-0 wall-adjacent
+ * collide kernel.  out holds 4 doubles.  No counterpart in the reference
+ * (TNL::Timer). */
 int lbm3_run_timed(lbm3_sim *s, int steps, int kernelEvents, double *out);
 
 /* State access. */

@@ -136,8 +136,9 @@ def sphere(nx=256, ny=128, nz=128, diameter=32.0, center=(64.0, 64.0, 64.0), u_i
     return c, bounds(0, 0, 0, nx, ny, nz), [tri]
 
 
-def channel(nx=64, ny=32, nz=32, force=1e-6, nu=0.1, limiter=0.0):
-    """Body-force driven duct: periodic in x, no-slip (moment-based u=0) on y and z faces.
+def channel(nx=64, ny=32, nz=32, force=1e-6, nu=0.1, limiter=0.0, periodic_z=False):
+    """Body-force driven duct: periodic in x, no-slip (moment-based u=0) on y and z faces
+    (with periodic_z: periodic in z too, i.e. plane Poiseuille flow between the y walls).
 
     Lattice viscosity `nu` (K17 is singular at nu = 1/6, where omega1 = omega2 = 1).
     K15 by default: like the reference cases, K17 is not used next to moment-based
@@ -149,10 +150,15 @@ def channel(nx=64, ny=32, nz=32, force=1e-6, nu=0.1, limiter=0.0):
     c.nuPhys = float(f32(nu) * (res / f32(1000.0)) * (res / f32(1000.0)))
     c.trackWallForce = 0
     add_open_rule(c, y=T.FACE_LOW | T.FACE_HIGH, dirichletU=1, openBCID=0, nonReflective=0)
-    add_open_rule(c, z=T.FACE_LOW | T.FACE_HIGH, dirichletU=1, openBCID=0, nonReflective=0)
+    if not periodic_z:
+        add_open_rule(c, z=T.FACE_LOW | T.FACE_HIGH, dirichletU=1, openBCID=0, nonReflective=0)
     add_open_rule(c, periodic=1)
     add_local_rule(c, force=(force, 0.0, 0.0))
     if limiter is not None:
         add_local_rule(c, limiter=limiter)
-    # one ghost layer in x (periodic), walls are the first/last cell layers in y and z
+    # one ghost layer per periodic direction, walls are the first/last cell layers; where a
+    # ghost layer meets a wall face the cell takes the wall rule (edge normal), as the
+    # reference's getOpenBC gives the first matching condition
+    if periodic_z:
+        return c, bounds(-1, 0, -1, nx + 1, ny, nz + 1), []
     return c, bounds(-1, 0, 0, nx + 1, ny, nz), []

@@ -56,10 +56,22 @@ int finish_level_setup(Sim &s, int level)
     Level &l = s.L[level];
     l.geom = LevelGeom{l.info.res, l.info.ox, l.info.oy, l.info.oz, l.info.dtPhys, l.info.cellCountX, l.info.cellCountY,
                        l.info.cellCountZ};
+    if ((unsigned long long)l.N * storage_bytes(s.storage) >= (1ULL << 32))
+        return set_error(LBM3_E_MEMORY, "a level's population plane exceeds 4 GiB; split the domain over more levels or GPUs");
     l.bulk = local_bc(s.C, level, -1);
+    set_relax(l.bulk, double(l.info.nu));
     LocalBC wb[32];
-    for (int w = 0; w < 32; ++w) wb[w] = local_bc(s.C, level, w);
+    for (int w = 0; w < 32; ++w) {
+        wb[w] = local_bc(s.C, level, w);
+        set_relax(wb[w], double(l.info.nu));
+    }
     if (upload(l.wallBC, wb, 32, s.stream)) return LBM3_E_HIP;
+    HIPCHK(l.cellRow.alloc(l.N > 0 ? l.N : 1));
+    HIPCHK(l.rowTab.alloc(l.nRows > 0 ? l.nRows : 1));
+    launch_fast_topo(topo_of(l), l.wallMap.p, l.cellRow.p, l.rowTab.p, s.stream);
+    HIPCHK(l.openNb.alloc(2LL * (l.nOpen > 0 ? l.nOpen : 1)));
+    launch_open_nb(topo_of(l), l.nOpen, l.openCell.p, l.openNb.p, s.stream);
+    HIPCHK(hipGetLastError());
     std::vector<OpenBCRecord> rec(s.C.nOpenRules + 1);
     for (int r = 0; r <= s.C.nOpenRules; ++r) rec[r] = open_record(open_bc(s.C, r));
     if (upload(l.openRecords, rec.data(), (long long)rec.size(), s.stream)) return LBM3_E_HIP;
@@ -79,7 +91,9 @@ template <class S> static StepArgs<S> step_args(Sim &s, Level &l)
     StepArgs<S> a;
     a.f = reinterpret_cast<S *>(l.f.p);
     a.topo = topo_of(l);
+    a.fast = FastTopo{l.N, l.cellRow.p, l.rowTab.p};
     a.wallMap = l.wallMap.p;
+    a.wallIndex = l.wallIndex.p;
     a.wallData = l.wallData.p;
     a.linkLength = l.linkLength.p;
     a.W = l.W;
@@ -99,6 +113,7 @@ template <class S> static OpenArgs<S> open_args(Sim &s, Level &l)
     a.topo = topo_of(l);
     a.count = l.nOpen;
     a.cell = l.openCell.p;
+    a.nb = l.openNb.p;
     a.meta = l.openMeta.p;
     a.records = l.openRecords.p;
     T *refs = reinterpret_cast<T *>(l.openRefs.p);
@@ -116,19 +131,32 @@ template <class S> static void step_level(Sim &s, int level)
     const bool flip = l.info.esotwistFlipper != 0;
     const bool timed = s.timing && level == 0;
     if (timed) {
-        if (s.evUsed >= (int)s.evStart.size()) {
-            hipEvent_t a, b;
-            (void)hipEventCreate(&a);
-            (void)hipEventCreate(&b);
-            s.evStart.push_back(a);
-            s.evStop.push_back(b);
+        if (s.evUsed >= (int)s.ev.size()) {
+            std::array<hipEvent_t, 3> e{};
+            for (auto &x : e) (void)hipEventCreate(&x);
+            s.ev.push_back(e);
         }
-        (void)hipEventRecord(s.evStart[s.evUsed], s.stream);
+        (void)hipEventRecord(s.ev[s.evUsed][0], s.stream);
     }
-    launch_collide_stream<S>(step_args<S>(s, l), flip, s.stream);
-    if (timed) {
-        (void)hipEventRecord(s.evStop[s.evUsed], s.stream);
-        s.evUsed++;
+    const StepArgs<S> a = step_args<S>(s, l);
+    if (timed || l.W == 0) {
+        // in order on one stream, so that each kernel's events bracket it alone
+        launch_collide_stream<S>(a, flip, s.stream);
+        if (timed) (void)hipEventRecord(s.ev[s.evUsed][1], s.stream);
+        launch_collide_wall<S>(a, flip, s.stream);
+        if (timed) {
+            (void)hipEventRecord(s.ev[s.evUsed][2], s.stream);
+            s.evUsed++;
+        }
+    } else {
+        // the bulk and wall-adjacent kernels touch disjoint cells: run the small,
+        // latency-bound wall kernel on the second stream beside the bulk kernel
+        (void)hipEventRecord(s.fork, s.stream);
+        (void)hipStreamWaitEvent(s.aux, s.fork, 0);
+        launch_collide_wall<S>(a, flip, s.aux);
+        (void)hipEventRecord(s.join, s.aux);
+        launch_collide_stream<S>(a, flip, s.stream);
+        (void)hipStreamWaitEvent(s.stream, s.join, 0);
     }
     const bool nflip = !flip;
     l.info.esotwistFlipper = nflip ? 1 : 0;
@@ -195,8 +223,8 @@ template <class T> int collide_cells(T *f, int n, const lbm3_bc *bc, T nu)
     DBuf<T> d;
     HIPCHK(d.alloc(27LL * n));
     HIPCHK(hipMemcpy(d.p, f, sizeof(T) * 27 * (size_t)n, hipMemcpyHostToDevice));
-    const CollisionParams P{bc->gx, bc->gy, bc->gz, bc->collisionLimiter, bc->nuMultiplier};
-    launch_collide_batch<T>(d.p, n, P, nu, nullptr);
+    const Relax<T> R = make_relax<T>(nu, T(bc->nuMultiplier), T(bc->collisionLimiter), T(bc->gx), T(bc->gy), T(bc->gz));
+    launch_collide_batch<T>(d.p, n, R, nullptr);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpy(f, d.p, sizeof(T) * 27 * (size_t)n, hipMemcpyDeviceToHost));
     return 0;
@@ -230,8 +258,18 @@ int lbm3_sim_create(const lbm3_case *c, int storage, lbm3_sim **out)
     h->s.C = *c;
     h->s.storage = storage;
     h->s.levels = c->gridLevelCount;
-    const int rc = check_hip(hipStreamCreateWithFlags(&h->s.stream, hipStreamNonBlocking), "stream");
-    if (rc) { delete h; return rc; }
+    int rc = check_hip(hipStreamCreateWithFlags(&h->s.stream, hipStreamNonBlocking), "stream");
+    if (!rc) rc = check_hip(hipStreamCreateWithFlags(&h->s.aux, hipStreamNonBlocking), "stream");
+    if (!rc) rc = check_hip(hipEventCreateWithFlags(&h->s.fork, hipEventDisableTiming), "event");
+    if (!rc) rc = check_hip(hipEventCreateWithFlags(&h->s.join, hipEventDisableTiming), "event");
+    if (rc) {
+        if (h->s.join) (void)hipEventDestroy(h->s.join);
+        if (h->s.fork) (void)hipEventDestroy(h->s.fork);
+        if (h->s.aux) (void)hipStreamDestroy(h->s.aux);
+        if (h->s.stream) (void)hipStreamDestroy(h->s.stream);
+        delete h;
+        return rc;
+    }
     *out = h;
     return 0;
 }
@@ -240,10 +278,53 @@ int lbm3_sim_destroy(lbm3_sim *h)
 {
     if (!h) return 0;
     (void)hipStreamSynchronize(h->s.stream);
-    for (auto e : h->s.evStart) (void)hipEventDestroy(e);
-    for (auto e : h->s.evStop) (void)hipEventDestroy(e);
+    (void)hipStreamSynchronize(h->s.aux);
+    for (auto &e3 : h->s.ev)
+        for (auto e : e3) (void)hipEventDestroy(e);
+    (void)hipEventDestroy(h->s.fork);
+    (void)hipEventDestroy(h->s.join);
+    (void)hipStreamDestroy(h->s.aux);
     (void)hipStreamDestroy(h->s.stream);
     delete h;
+    return 0;
+}
+
+// every index the kernels follow must stay inside its array: checked on the
+// host before anything reaches the device
+static int validate_level(const lbm3_level_arrays &a)
+{
+    const long long N = a.info.cellCount, R = a.nRows, W = a.W;
+    auto bad = [](const char *what) { return set_error(LBM3_E_ARG, std::string("import: ") + what); };
+    if (N <= 0 || R <= 0 || W < 0 || a.nOpenIDs < 0 || a.nPeriodic < 0) return bad("negative or empty sizes");
+    if (!a.shifter || !a.rowI || !a.rowJ || !a.rowK || !a.rowJPlus || !a.rowKPlus || !a.rowJKPlus || !a.wallMap)
+        return bad("missing topology array");
+    if (W > 0 && (!a.wallIndex || !a.wallData || !a.linkLength)) return bad("missing wall arrays");
+    for (long long c = 0; c < N; ++c) {
+        const int sh = a.shifter[c];
+        if (sh >= 0) {
+            if (sh >= R) return bad("shifter row out of range");
+        } else if (c + sh < 0 || a.shifter[c + sh] < 0 || a.shifter[c + sh] >= R) {
+            return bad("shifter offset does not reach a row head");
+        }
+        const int wm = a.wallMap[c];
+        if (wm < -3 || wm >= W) return bad("wallMap out of range");
+    }
+    for (long long r = 0; r < R; ++r)
+        if (a.rowJPlus[r] < 0 || a.rowJPlus[r] >= N || a.rowKPlus[r] < 0 || a.rowKPlus[r] >= N || a.rowJKPlus[r] < 0 ||
+            a.rowJKPlus[r] >= N)
+            return bad("row neighbour out of range");
+    for (long long w = 0; w < W; ++w)
+        if (a.wallIndex[w] < 0 || a.wallIndex[w] >= N || a.wallMap[a.wallIndex[w]] != w) return bad("wallIndex inconsistent with wallMap");
+    long long nOpen = 0;
+    for (int b = 0; b < a.nOpenIDs; ++b) {
+        if (a.openCount[b] < 0) return bad("negative open count");
+        nOpen += a.openCount[b];
+    }
+    for (long long x = 0; x < nOpen; ++x)
+        if (a.openIndex[x] < 0 || a.openIndex[x] >= N || a.openSource[x] < 0 || a.openSource[x] >= N) return bad("open cell out of range");
+    for (long long x = 0; x < a.nPeriodic; ++x)
+        if (a.periodicIndex[x] < 0 || a.periodicIndex[x] >= N || a.periodicSource[x] < 0 || a.periodicSource[x] >= N)
+            return bad("periodic cell out of range");
     return 0;
 }
 
@@ -253,6 +334,7 @@ int lbm3_import_level(lbm3_sim *h, int level, const lbm3_level_arrays *a)
     Sim &s = h->s;
     Level &l = s.L[level];
     hipStream_t st = s.stream;
+    if (const int rc = validate_level(*a)) return rc;
     l.info = a->info;
     l.N = a->info.cellCount;
     l.nRows = a->nRows;
@@ -366,15 +448,18 @@ int lbm3_run_timed(lbm3_sim *h, int steps, int kernelEvents, double *out)
     HIPCHK(hipEventSynchronize(b));
     float ms = 0.f;
     HIPCHK(hipEventElapsedTime(&ms, a, b));
-    double kernel = 0.0;
+    double bulk = 0.0, wall = 0.0;
     for (int e = 0; e < s.evUsed; ++e) {
-        float k = 0.f;
-        HIPCHK(hipEventElapsedTime(&k, s.evStart[e], s.evStop[e]));
-        kernel += k;
+        float k = 0.f, w = 0.f;
+        HIPCHK(hipEventElapsedTime(&k, s.ev[e][0], s.ev[e][1]));
+        HIPCHK(hipEventElapsedTime(&w, s.ev[e][1], s.ev[e][2]));
+        bulk += k;
+        wall += w;
     }
     out[0] = ms;
-    out[1] = kernel;
+    out[1] = bulk;
     out[2] = s.evUsed;
+    out[3] = wall;
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
     return check_hip(hipGetLastError(), "timed run");

@@ -16,11 +16,63 @@
 
 namespace lbm3 {
 
-struct CollisionParams {
-    float gx, gy, gz;          // body force (lattice units)
-    float collisionLimiter;    // lambda; <= 0 selects K15 AllOne
-    float nuMultiplier;
+// Everything of the relaxation that depends only on the viscosity, the
+// viscosity multiplier and the limiter, evaluated once per boundary-condition
+// entry instead of once per cell (same expressions as the reference, in the
+// compute precision).
+template <class T> struct Relax {
+    T gx, gy, gz;
+    T w1;                 // omega1 = 1 / (3 nu nuMult + 1/2)
+    T w3, w4, w5, A, B;   // K17 rates and fourth-order coefficients (omega2 = 1)
+    T lambda;
+    int k17;              // K17 if lambda > 0 and nuMult <= 1, else K15 AllOne
 };
+
+template <class T>
+__host__ __device__ inline Relax<T> make_relax(T nu, T nuMult, T lambda, T gx, T gy, T gz)
+{
+    Relax<T> r;
+    r.gx = gx; r.gy = gy; r.gz = gz;
+    r.lambda = lambda;
+    r.k17 = (lambda > T(0)) && (nuMult <= T(1));
+    const T w1 = T(1) / (T(3) * (nu * nuMult) + T(0.5)), w2 = T(1);
+    r.w1 = w1;
+    r.w3 = T(8) * (w1 - T(2)) * (w2 * (T(3) * w1 - T(1)) - T(5) * w1) /
+           (T(8) * (T(5) - T(2) * w1) * w1 + w2 * (T(8) + w1 * (T(9) * w1 - T(26))));
+    r.w4 = T(8) * (w1 - T(2)) * (w1 + w2 * (T(3) * w1 - T(7))) / (w2 * (T(56) - T(42) * w1 + T(9) * w1 * w1) - T(8) * w1);
+    r.w5 = T(24) * (w1 - T(2)) * (T(4) * w1 * w1 + w1 * w2 * (T(18) - T(13) * w1) + w2 * w2 * (T(2) + w1 * (T(6) * w1 - T(11)))) /
+           (T(16) * w1 * w1 * (w1 - T(6)) - T(2) * w1 * w2 * (T(216) + T(5) * w1 * (T(9) * w1 - T(46))) +
+            w2 * w2 * (w1 * (T(3) * w1 - T(10)) * (T(15) * w1 - T(28)) - T(48)));
+    r.A = (T(4) * w1 * w1 + T(2) * w1 * w2 * (w1 - T(6)) + w2 * w2 * (w1 * (T(10) - T(3) * w1) - T(4))) /
+          ((w1 - w2) * (w2 * (T(2) + T(3) * w1) - T(8) * w1));
+    r.B = (T(4) * w1 * w2 * (T(9) * w1 - T(16)) - T(4) * w1 * w1 - T(2) * w2 * w2 * (T(2) + T(9) * w1 * (w1 - T(2)))) /
+          (T(3) * (w1 - w2) * (w2 * (T(2) + T(3) * w1) - T(8) * w1));
+    if (!r.k17) { r.w3 = r.w4 = r.w5 = r.A = r.B = T(0); }
+    return r;
+}
+
+// reciprocal: the hardware estimate refined by one Newton step in FP32, exact division in FP64
+__host__ __device__ __forceinline__ float fast_rcp(float x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    const float r = __builtin_amdgcn_rcpf(x);
+    return r * (2.f - x * r);
+#else
+    return 1.f / x;
+#endif
+}
+__host__ __device__ __forceinline__ double fast_rcp(double x) { return 1.0 / x; }
+
+// hardware reciprocal estimate (about 1 ulp) for the limiter's rational function
+__host__ __device__ __forceinline__ float rough_rcp(float x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_rcpf(x);
+#else
+    return 1.f / x;
+#endif
+}
+__host__ __device__ __forceinline__ double rough_rcp(double x) { return 1.0 / x; }
 
 // forward: (f-, f0, f+) about velocity u -> (k0, k1, k2); K is the weight moment of the chain
 template <class T>
@@ -45,9 +97,10 @@ __host__ __device__ __forceinline__ void cm_backward(T k0, T k1, T k2, T u, T u2
     z = k0 - S;
 }
 
-// f[27] in/out (shifted populations), nu: lattice viscosity of the level
-template <class T>
-__host__ __device__ __forceinline__ void collide(T (&f)[27], const CollisionParams &P, T nu)
+// f[27] in/out (shifted populations).  MODE 0: K17 or K15 from R.k17 at run
+// time; 1: K17; 2: K15.
+template <class T, int MODE = 0>
+__host__ __device__ __forceinline__ void collide(T (&f)[27], const Relax<T> &R)
 {
     // weight moments of the well-conditioned chains: z-pass (per x,y pair), y-pass (per x)
     constexpr T W36 = T(1) / T(36), W9 = T(1) / T(9), W49 = T(4) / T(9);
@@ -61,14 +114,14 @@ __host__ __device__ __forceinline__ void collide(T (&f)[27], const CollisionPara
     const T xm = (s_aa + s_ab) + s_ac, x0 = (s_ba + s_bb) + s_bc, xp = (s_ca + s_cb) + s_cc;
     const T dRho = (xm + x0) + xp;
     const T rho = T(1) + dRho;
-    const T rhoInv = T(1) / rho;
+    const T rhoInv = fast_rcp(rho);
     const T jx = xp - xm;
     const T jy = ((s_ac + s_bc) + s_cc) - ((s_aa + s_ba) + s_ca);
     const T jz = (((f[MMP] - f[MMM]) + (f[MOP] - f[MOM])) + ((f[MPP] - f[MPM]) + (f[OMP] - f[OMM]))) +
                  (((f[OOP] - f[OOM]) + (f[OPP] - f[OPM])) + ((f[PMP] - f[PMM]) + (f[POP] - f[POM]))) + (f[PPP] - f[PPM]);
-    const T ux = (jx + T(0.5) * T(P.gx)) * rhoInv;
-    const T uy = (jy + T(0.5) * T(P.gy)) * rhoInv;
-    const T uz = (jz + T(0.5) * T(P.gz)) * rhoInv;
+    const T ux = (jx + T(0.5) * R.gx) * rhoInv;
+    const T uy = (jy + T(0.5) * R.gy) * rhoInv;
+    const T uz = (jz + T(0.5) * R.gz) * rhoInv;
     const T ux2 = ux * ux, uy2 = uy * uy, uz2 = uz * uz;
 
     // ---- z pass: chains over (x,y) = (a,b,c) x (a,b,c) ----
@@ -121,20 +174,15 @@ __host__ __device__ __forceinline__ void collide(T (&f)[27], const CollisionPara
     cm_forward(y_a22, y_b22, y_c22, ux, ux2, ninth, k022, k122, k222);
     (void)k212; (void)k221; (void)k122; (void)k222;
 
-    // ---- cumulants of order 4 (orders 2, 3 equal the central moments) ----
-    const T C211 = k211 - ((k200 + third) * k011 + T(2) * k101 * k110) * rhoInv;
-    const T C121 = k121 - ((k020 + third) * k101 + T(2) * k110 * k011) * rhoInv;
-    const T C112 = k112 - ((k002 + third) * k110 + T(2) * k011 * k101) * rhoInv;
+    // fourth-order cumulants relax to their equilibrium (rate 1): their
+    // pre-collision values, and the fourth-order moments, are not needed
+    (void)k211; (void)k121; (void)k112; (void)k220; (void)k022; (void)k202;
     const T dRhoNinth = dRho * rhoInv * ninth;
-    const T C220 = k220 - ((k020 * k200 + T(2) * k110 * k110 + (k200 + k020) * third) * rhoInv - dRhoNinth);
-    const T C022 = k022 - ((k002 * k020 + T(2) * k011 * k011 + (k002 + k020) * third) * rhoInv - dRhoNinth);
-    const T C202 = k202 - ((k200 * k002 + T(2) * k101 * k101 + (k200 + k002) * third) * rhoInv - dRhoNinth);
 
     // ---- relaxation ----
-    const T w1 = T(1) / (T(3) * (nu * T(P.nuMultiplier)) + T(0.5));
-    const T w2 = T(1);
+    const T w1 = R.w1;
     const T hr = T(0.5) * rhoInv;
-    const T Dxu = -w1 * hr * (T(2) * k200 - k020 - k002) - w2 * hr * (k200 + k020 + k002 - dRho);
+    const T Dxu = -w1 * hr * (T(2) * k200 - k020 - k002) - hr * (k200 + k020 + k002 - dRho);
     const T Dyv = Dxu + T(1.5) * w1 * rhoInv * (k200 - k020);
     const T Dzw = Dxu + T(1.5) * w1 * rhoInv * (k200 - k002);
     const T DxvDyu = -T(3) * w1 * rhoInv * k110;
@@ -144,57 +192,44 @@ __host__ __device__ __forceinline__ void collide(T (&f)[27], const CollisionPara
     const T om1 = T(1) - w1;
     const T s110 = om1 * k110, s101 = om1 * k101, s011 = om1 * k011;
     const T g1 = T(3) * rho * (T(1) - T(0.5) * w1);
-    const T g2 = T(3) * rho * (T(1) - T(0.5) * w2);
+    const T g2 = T(1.5) * rho;   // 3 rho (1 - omega2/2), omega2 = 1
     const T e33 = om1 * (k200 - k020) - g1 * (ux2 * Dxu - uy2 * Dyv);
     const T e34 = om1 * (k200 - k002) - g1 * (ux2 * Dxu - uz2 * Dzw);
-    const T e35 = dRho * w2 + (T(1) - w2) * (k200 + k020 + k002) - g2 * (ux2 * Dxu + uy2 * Dyv + uz2 * Dzw);
+    const T e35 = dRho - g2 * (ux2 * Dxu + uy2 * Dyv + uz2 * Dzw);
     const T s200 = (e33 + e34 + e35) * third;
     const T s020 = (e34 - T(2) * e33 + e35) * third;
     const T s002 = (e33 - T(2) * e34 + e35) * third;
 
     T s120 = 0, s102 = 0, s210 = 0, s012 = 0, s201 = 0, s021 = 0, s111 = 0;
     T c220 = 0, c202 = 0, c022 = 0, c211 = 0, c121 = 0, c112 = 0;
-    const T lambda = T(P.collisionLimiter);
-    if (lambda > T(0) && T(P.nuMultiplier) <= T(1)) {
-        const T w1s = w1 * w1;
-        const T w3 = T(8) * (w1 - T(2)) * (w2 * (T(3) * w1 - T(1)) - T(5) * w1) /
-                     (T(8) * (T(5) - T(2) * w1) * w1 + w2 * (T(8) + w1 * (T(9) * w1 - T(26))));
-        const T w4 = T(8) * (w1 - T(2)) * (w1 + w2 * (T(3) * w1 - T(7))) /
-                     (w2 * (T(56) - T(42) * w1 + T(9) * w1s) - T(8) * w1);
-        const T w5 = T(24) * (w1 - T(2)) *
-                     (T(4) * w1s + w1 * w2 * (T(18) - T(13) * w1) + w2 * w2 * (T(2) + w1 * (T(6) * w1 - T(11)))) /
-                     (T(16) * w1s * (w1 - T(6)) - T(2) * w1 * w2 * (T(216) + T(5) * w1 * (T(9) * w1 - T(46))) +
-                      w2 * w2 * (w1 * (T(3) * w1 - T(10)) * (T(15) * w1 - T(28)) - T(48)));
-        const T A = (T(4) * w1s + T(2) * w1 * w2 * (w1 - T(6)) + w2 * w2 * (w1 * (T(10) - T(3) * w1) - T(4))) /
-                    ((w1 - w2) * (w2 * (T(2) + T(3) * w1) - T(8) * w1));
-        const T B = (T(4) * w1 * w2 * (T(9) * w1 - T(16)) - T(4) * w1s - T(2) * w2 * w2 * (T(2) + T(9) * w1 * (w1 - T(2)))) /
-                    (T(3) * (w1 - w2) * (w2 * (T(2) + T(3) * w1) - T(8) * w1));
+    const bool k17 = MODE == 1 || (MODE == 0 && R.k17);
+    if (MODE != 2 && k17) {
+        const T lambda = R.lambda;
         // limited relaxation: the rate tends to 1 where the cumulant is large compared to lambda
         auto limited = [&](T c, T w) {
             const T a = c < T(0) ? -c : c;
-            const T wl = w + (T(1) - w) * a / (lambda + a);
+            const T wl = w + (T(1) - w) * a * rough_rcp(lambda + a);
             return (T(1) - wl) * c;
         };
-        const T p1 = limited(k120 + k102, w3), p2 = limited(k210 + k012, w3), p3 = limited(k201 + k021, w3);
-        const T m1 = limited(k120 - k102, w4), m2 = limited(k210 - k012, w4), m3 = limited(k201 - k021, w4);
+        const T p1 = limited(k120 + k102, R.w3), p2 = limited(k210 + k012, R.w3), p3 = limited(k201 + k021, R.w3);
+        const T m1 = limited(k120 - k102, R.w4), m2 = limited(k210 - k012, R.w4), m3 = limited(k201 - k021, R.w4);
         s120 = T(0.5) * (m1 + p1); s102 = T(0.5) * (p1 - m1);
         s210 = T(0.5) * (m2 + p2); s012 = T(0.5) * (p2 - m2);
         s201 = T(0.5) * (m3 + p3); s021 = T(0.5) * (p3 - m3);
-        s111 = limited(k111, w5);
+        s111 = limited(k111, R.w5);
         const T visc = (T(1) / w1 - T(0.5)) * rho;
-        const T a4 = T(2) / T(3) * visc * A;
+        const T a4 = T(2) / T(3) * visc * R.A;
         const T e43 = a4 * (Dxu - T(2) * Dyv + Dzw);
         const T e44 = a4 * (-T(2) * Dxu + Dyv + Dzw);
         const T e45 = -T(2) * a4 * (Dxu + Dyv + Dzw);
         c220 = (e43 + e44 + e45) * third;
         c202 = (e45 - e43) * third;
         c022 = (e45 - e44) * third;
-        const T b4 = -third * visc * B;
+        const T b4 = -third * visc * R.B;
         c211 = b4 * DywDzv;
         c121 = b4 * DxwDzu;
         c112 = b4 * DxvDyu;
     }
-    (void)C211; (void)C121; (void)C112; (void)C220; (void)C022; (void)C202;
 
     // ---- post-collision central moments ----
     const T q211 = c211 + ((s200 + third) * s011 + T(2) * s101 * s110) * rhoInv;
@@ -214,7 +249,7 @@ __host__ __device__ __forceinline__ void collide(T (&f)[27], const CollisionPara
                  + (T(3) * (q022 + q202 + q220) + (s200 + s020 + s002)) * rhoInv * ninth
                  - T(2) / T(3) * (T(2) * (s101 * s101 + s011 * s011 + s110 * s110) + (s002 * s020 + s002 * s200 + s020 * s200) +
                                   (s002 + s020 + s200) * third) * r2
-                 - (dRho * dRho - dRho) * r2 / T(27);
+                 - (dRho * dRho - dRho) * r2 * (T(1) / T(27));
 
     // ---- back transform: x pass (first moments flip sign: momentum gains the force) ----
     T b_a00, b_b00, b_c00, b_a01, b_b01, b_c01, b_a02, b_b02, b_c02;

@@ -35,6 +35,31 @@ __device__ __forceinline__ void decode(const Topology &t, int cell, int (&nb)[8]
     nb[7] = nb[6] + 1 >= N ? 0 : nb[6] + 1;
 }
 
+// Hot-loop form of the compressed topology, derived once per level: one 8-byte
+// record per cell {wallMap, row} and one 16-byte record per row {first cell of
+// the row, first cell of the j+ row, of the k+ row, of the jk+ row}.  The
+// neighbour cells of a cell are then two dependent loads away instead of four.
+struct FastTopo {
+    int N;
+    const int2 *cellRow;   // [N]
+    const int4 *rows;      // [nRows]
+};
+
+__device__ __forceinline__ void decode_fast(const FastTopo &t, int cell, int row, int (&nb)[8])
+{
+    const int4 r = t.rows[row];
+    const int back = cell - r.x;
+    const int N = t.N;
+    nb[0] = cell;
+    nb[2] = r.y + back;
+    nb[3] = r.z + back;
+    nb[6] = r.w + back;
+    nb[1] = cell + 1 >= N ? 0 : cell + 1;
+    nb[4] = nb[2] + 1 >= N ? 0 : nb[2] + 1;
+    nb[5] = nb[3] + 1 >= N ? 0 : nb[3] + 1;
+    nb[7] = nb[6] + 1 >= N ? 0 : nb[6] + 1;
+}
+
 // getLocalBC resolved per (level, wallID): rules depend only on those two keys,
 // except the rotating-wall velocity which needs the cell position.
 struct LocalBC {
@@ -43,7 +68,19 @@ struct LocalBC {
     int velMode;            // 0: wall moves with the fluid's own velocity, 1: fixed, 2: rotation
     float ux, uy, uz;
     float omega[3], origin[3];
+    Relax<float> rf;        // relaxation constants for the level's viscosity
+    Relax<double> rd;
 };
+
+template <class T> __host__ __device__ __forceinline__ const Relax<T> &relax_of(const LocalBC &b);
+template <> __host__ __device__ __forceinline__ const Relax<float> &relax_of<float>(const LocalBC &b) { return b.rf; }
+template <> __host__ __device__ __forceinline__ const Relax<double> &relax_of<double>(const LocalBC &b) { return b.rd; }
+
+inline void set_relax(LocalBC &b, double nu)
+{
+    b.rf = make_relax<float>(float(nu), b.nuMult, b.limiter, b.gx, b.gy, b.gz);
+    b.rd = make_relax<double>(nu, double(b.nuMult), double(b.limiter), double(b.gx), double(b.gy), double(b.gz));
+}
 
 struct LevelGeom {          // what the hooks need about the level
     float res, ox, oy, oz, dtPhys;
@@ -53,7 +90,9 @@ struct LevelGeom {          // what the hooks need about the level
 template <class S> struct StepArgs {
     S *f;
     Topology topo;
+    FastTopo fast;
     const int *wallMap;
+    const int *wallIndex;         // [W] cell of each wall-adjacent entry
     const uint32_t *wallData;
     const uint32_t *linkLength;   // [7][W]
     int W;
@@ -70,6 +109,7 @@ template <class S> struct OpenArgs {
     Topology topo;
     int count;
     const int *cell;
+    const int4 *nb;               // [2*count] the 8 neighbour cells of each open cell (setup-time decode)
     const uint32_t *meta;         // bits 0-4: normal index, 5-12: record, 13: track flow
     const OpenBCRecord *records;
     typename StoreTraits<S>::compute *dRhoRef, *uNormalRef;  // reference state, compute precision
@@ -77,9 +117,14 @@ template <class S> struct OpenArgs {
     int iterationsFinished;
 };
 
+#ifndef LBM3_XCD_REMAP
+#define LBM3_XCD_REMAP 1
+#endif
+
 __device__ __forceinline__ int xcd_block(int b, int n)
 {
     // blocks are dispatched round-robin over the 8 XCDs; give each XCD a contiguous range
+    if (!LBM3_XCD_REMAP) return b;
     const int per = n / 8;
     if (b >= per * 8) return b;
     return (b % 8) * per + b / 8;

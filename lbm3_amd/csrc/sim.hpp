@@ -1,6 +1,7 @@
 // Device-resident state of a simulation (all grid levels) behind the C ABI.
 #pragma once
 #include <string>
+#include <array>
 #include <vector>
 #include "../../include/lbm3mi.h"
 #include "step.hpp"
@@ -47,6 +48,8 @@ struct Level {
     LevelGeom geom{};
     int N = 0, nRows = 0, W = 0, wallCount = 0;
     DBuf<int> shifter, ri, rj, rk, rjp, rkp, rjkp;
+    DBuf<int2> cellRow;              // {wallMap, row} per cell
+    DBuf<int4> rowTab;               // {first cell, j+ row, k+ row, jk+ row} per row
     DBuf<int> wallMap, wallIndex;
     DBuf<uint32_t> wallData, linkLength;
     DBuf<double> gWall;
@@ -54,6 +57,7 @@ struct Level {
     int nOpenIDs = 0, nOpen = 0;
     std::vector<int> openCount;
     DBuf<int> openCell, openSource;
+    DBuf<int4> openNb;               // [2][nOpen] neighbour cells of each open cell
     DBuf<uint32_t> openMeta;
     DBuf<OpenBCRecord> openRecords;
     DBuf<unsigned char> openRefs;    // 2 arrays of nOpen compute-type values
@@ -76,12 +80,14 @@ struct Sim {
     int levels = 0;
     Level L[LBM3_MAX_LEVELS];
     long long fluidUpdates = 0;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;    // every call's work is ordered on this stream
+    hipStream_t aux = nullptr;       // wall-adjacent kernel beside the bulk kernel, joined back each step
+    hipEvent_t fork = nullptr, join = nullptr;
     DBuf<double> partial;            // reduction scratch
     DBuf<double> scalar;
     // timing of the level-0 collide-stream launches
     bool timing = false;
-    std::vector<hipEvent_t> evStart, evStop;
+    std::vector<std::array<hipEvent_t, 3>> ev;   // before bulk, between, after wall kernel
     int evUsed = 0;
 };
 

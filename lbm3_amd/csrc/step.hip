@@ -35,21 +35,19 @@ __device__ __forceinline__ void store_pre(S *f, int N, const int (&nb)[8], const
 
 // interpolated bounceback (Zhao & Yong 2017, single node, l = gamma) on the
 // post-collision populations of a wall-adjacent cell; opposite directions are
-// handled as pairs so both use the unmodified post-collision values.
-template <class S, bool FLIP>
-__device__ __forceinline__ void bounce_back(const StepArgs<S> &a, const int (&nb)[8], typename StoreTraits<S>::compute (&f)[27],
-                                            uint32_t wd, int wm, bool useIBB, const float (&wu)[3], double (&g3)[3])
+// handled as pairs so both use the unmodified post-collision values.  `pre`
+// holds the cell's pre-collision populations (the values its slots held before
+// this step), `codes` the cell's seven words of 8-bit link lengths in link order.
+template <class T>
+__device__ __forceinline__ void bounce_back(T (&f)[27], const T (&pre)[27], uint32_t wd, bool useIBB,
+                                            const uint32_t (&codes)[7], const float (&wu)[3], double (&g3)[3])
 {
-    using T = typename StoreTraits<S>::compute;
-    const int N = a.topo.N;
     T dRho, ux_, uy_, uz_;
     moments(f, dRho, ux_, uy_, uz_);
     int linkIndex = 0;
-    uint32_t code = 0u;
     auto gamma_next = [&]() -> T {
         if (!useIBB) return T(0.5);
-        if ((linkIndex & 3) == 0) code = a.linkLength[(size_t)(linkIndex >> 2) * (size_t)a.W + wm];
-        const uint32_t c8 = (code >> (8 * (linkIndex & 3))) & 255u;
+        const uint32_t c8 = (codes[linkIndex >> 2] >> (8 * (linkIndex & 3))) & 255u;
         ++linkIndex;
         T g = T(c8) / T(254);
         return g < T(0.01f) ? T(0.01f) : (g > T(1) ? T(1) : g);
@@ -64,10 +62,9 @@ __device__ __forceinline__ void bounce_back(const StepArgs<S> &a, const int (&nb
         if (la) {
             const T g = gamma_next();
             const T inv1 = T(1) / (T(1) + g);
-            const T pre = StoreTraits<S>::load(a.f + (size_t)pre_plane(da, FLIP) * (size_t)N + nb[pre_nb(da)]);
             const T eu = T(cx(db)) * T(wu[0]) + T(cy(db)) * T(wu[1]) + T(cz(db)) * T(wu[2]);
             const T wall = T(6) * weight<T>(db) * (T(1) + dRho) * eu;
-            ra = ((T(1) - g) * inv1) * pre + (g * inv1) * fb + (g * inv1) * fa + inv1 * wall;
+            ra = ((T(1) - g) * inv1) * pre[da] + (g * inv1) * fb + (g * inv1) * fa + inv1 * wall;
             g3[0] += double(-fa * (T(cx(da)) - T(wu[0])) + ra * (T(cx(db)) - T(wu[0])));
             g3[1] += double(-fa * (T(cy(da)) - T(wu[1])) + ra * (T(cy(db)) - T(wu[1])));
             g3[2] += double(-fa * (T(cz(da)) - T(wu[2])) + ra * (T(cz(db)) - T(wu[2])));
@@ -75,10 +72,9 @@ __device__ __forceinline__ void bounce_back(const StepArgs<S> &a, const int (&nb
         if (lb) {
             const T g = gamma_next();
             const T inv1 = T(1) / (T(1) + g);
-            const T pre = StoreTraits<S>::load(a.f + (size_t)pre_plane(db, FLIP) * (size_t)N + nb[pre_nb(db)]);
             const T eu = T(cx(da)) * T(wu[0]) + T(cy(da)) * T(wu[1]) + T(cz(da)) * T(wu[2]);
             const T wall = T(6) * weight<T>(da) * (T(1) + dRho) * eu;
-            rb = ((T(1) - g) * inv1) * pre + (g * inv1) * fa + (g * inv1) * fb + inv1 * wall;
+            rb = ((T(1) - g) * inv1) * pre[db] + (g * inv1) * fa + (g * inv1) * fb + inv1 * wall;
             g3[0] += double(-fb * (T(cx(db)) - T(wu[0])) + rb * (T(cx(da)) - T(wu[0])));
             g3[1] += double(-fb * (T(cy(db)) - T(wu[1])) + rb * (T(cy(da)) - T(wu[1])));
             g3[2] += double(-fb * (T(cz(db)) - T(wu[2])) + rb * (T(cz(da)) - T(wu[2])));
@@ -88,39 +84,71 @@ __device__ __forceinline__ void bounce_back(const StepArgs<S> &a, const int (&nb
     }
 }
 
-template <class S, bool FLIP>
+// neighbour addressing inside one population plane: a wave-uniform plane base
+// plus a 32-bit byte offset per lane (planes are limited to 4 GiB)
+template <class S> __device__ __forceinline__ S *at(S *plane, uint32_t off)
+{
+    return reinterpret_cast<S *>(reinterpret_cast<char *>(plane) + off);
+}
+
+// Cells are independent under esotwist streaming (each one reads and writes
+// only its own 27 slots), so the level is updated by two kernels: one over all
+// cells that skips the wall-adjacent ones, and one over the short list of
+// wall-adjacent cells.  The bulk kernel then carries no bounceback code, which
+// keeps its register budget and instruction stream to the collision itself.
+//
+// BULK: collision variant of the cells that are not next to a wall, fixed for
+// the level on the host (1: K17, 2: K15).
+template <class S, bool FLIP, int BULK>
 __global__ __launch_bounds__(256) void k_collide_stream(StepArgs<S> a)
 {
     using T = typename StoreTraits<S>::compute;
-    const int N = a.topo.N;
+    const int N = a.fast.N;
     const int cell = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
     if (cell >= N) return;
-    const int wm = a.wallMap[cell];
-    if (wm == -3) return;
+    const long long rec = __builtin_nontemporal_load(reinterpret_cast<const long long *>(a.fast.cellRow) + cell);
+    const int wm = int(rec), row = int(rec >> 32);
+    if (wm >= 0 || wm == -3) return;   // wall-adjacent (other kernel) or solid
     int nb[8];
-    decode(a.topo, cell, nb, nullptr);
+    decode_fast(a.fast, cell, row, nb);
+    uint32_t off[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) off[k] = uint32_t(nb[k]) * uint32_t(sizeof(S));
+    T f[27];
+#pragma unroll
+    for (int q = 0; q < 27; ++q) f[q] = StoreTraits<S>::load(at(a.f + (size_t)pre_plane(q, FLIP) * (size_t)N, off[pre_nb(q)]));
+    collide<T, BULK>(f, relax_of<T>(a.bulk));
+#pragma unroll
+    for (int q = 0; q < 27; ++q) StoreTraits<S>::store(at(a.f + (size_t)post_plane(q, FLIP) * (size_t)N, off[post_nb(q)]), f[q]);
+}
+
+// wall-adjacent cells: collision with the wall id's local BC, interpolated
+// bounceback of the links that cross the wall, momentum-exchange force
+template <class S, bool FLIP>
+__global__ __launch_bounds__(256) void k_collide_wall(StepArgs<S> a)
+{
+    using T = typename StoreTraits<S>::compute;
+    const int N = a.fast.N;
+    const int wm = blockIdx.x * 256 + threadIdx.x;
+    if (wm >= a.W) return;
+    const int cell = a.wallIndex[wm];
+    const int row = a.fast.cellRow[cell].y;
+    int nb[8];
+    decode_fast(a.fast, cell, row, nb);
     T f[27];
     load_pre<S, FLIP>(a.f, N, nb, f);
-
-    if (wm < 0) {
-        const LocalBC &b = a.bulk;
-        const CollisionParams P{b.gx, b.gy, b.gz, b.limiter, b.nuMult};
-        collide(f, P, T(a.nu));
-#pragma unroll
-        for (int q = 0; q < 27; ++q)
-            StoreTraits<S>::store(a.f + (size_t)post_plane(q, FLIP) * (size_t)N + nb[post_nb(q)], f[q]);
-        return;
-    }
-
-    // wall-adjacent cell
     const uint32_t wd = a.wallData[wm];
-    const LocalBC b = a.wallBC[(wd >> 27) & 31u];
+    uint32_t codes[7];
+#pragma unroll
+    for (int k = 0; k < 7; ++k) codes[k] = a.linkLength[(size_t)k * (size_t)a.W + wm];
+    const LocalBC &b = a.wallBC[(wd >> 27) & 31u];
     float wu[3] = {b.ux, b.uy, b.uz};
-    if (b.velMode == 0) {
+    const int velMode = b.velMode;
+    if (velMode == 0) {
         T d, u0, u1, u2;
         moments(f, d, u0, u1, u2);
         wu[0] = float(u0); wu[1] = float(u1); wu[2] = float(u2);
-    } else if (b.velMode == 2) {
+    } else if (velMode == 2) {
         int nbx[8], ijk[3];
         decode(a.topo, cell, nbx, ijk);
         const float x = float(ijk[0]) * a.geom.res + a.geom.ox - b.origin[0];
@@ -131,10 +159,13 @@ __global__ __launch_bounds__(256) void k_collide_stream(StepArgs<S> a)
         wu[1] = (b.omega[2] * x - b.omega[0] * z) * s;
         wu[2] = (b.omega[0] * y - b.omega[1] * x) * s;
     }
-    const CollisionParams P{b.gx, b.gy, b.gz, b.limiter, b.nuMult};
-    collide(f, P, T(a.nu));
+    const Relax<T> R = relax_of<T>(b);
+    T pre[27];
+#pragma unroll
+    for (int q = 0; q < 27; ++q) pre[q] = f[q];
+    collide<T, 0>(f, R);
     double g3[3] = {0.0, 0.0, 0.0};
-    bounce_back<S, FLIP>(a, nb, f, wd, wm, b.useIBB != 0, wu, g3);
+    bounce_back<T>(f, pre, wd, b.useIBB != 0, codes, wu, g3);
 #pragma unroll
     for (int d = 0; d < 27; ++d) {
         S *dst;
@@ -147,6 +178,28 @@ __global__ __launch_bounds__(256) void k_collide_stream(StepArgs<S> a)
         a.gWall[(size_t)a.W + wm] += g3[1];
         a.gWall[2 * (size_t)a.W + wm] += g3[2];
     }
+}
+
+// neighbour cells of every open-boundary cell, decoded once at setup
+__global__ __launch_bounds__(256) void k_open_nb(Topology t, int count, const int *cell, int4 *out)
+{
+    const int x = blockIdx.x * 256 + threadIdx.x;
+    if (x >= count) return;
+    int nb[8];
+    decode(t, cell[x], nb, nullptr);
+    out[2 * (size_t)x] = make_int4(nb[0], nb[1], nb[2], nb[3]);
+    out[2 * (size_t)x + 1] = make_int4(nb[4], nb[5], nb[6], nb[7]);
+}
+
+// derive the hot-loop topology records from the compressed IJK/NBR arrays
+__global__ __launch_bounds__(256) void k_fast_topo(Topology t, const int *wallMap, int2 *cellRow, int4 *rows)
+{
+    const int cell = blockIdx.x * 256 + threadIdx.x;
+    if (cell >= t.N) return;
+    const int shift = t.shifter[cell];
+    const int row = shift >= 0 ? shift : t.shifter[cell + shift];
+    cellRow[cell] = make_int2(wallMap[cell], row);
+    if (shift >= 0) rows[row] = make_int4(cell, t.rjp[row], t.rkp[row], t.rjkp[row]);
 }
 
 template <int NI, class S, bool FLIP>
@@ -189,8 +242,8 @@ __global__ __launch_bounds__(256) void k_open(OpenArgs<S> a)
 {
     const int x = blockIdx.x * 256 + threadIdx.x;
     if (x >= a.count) return;
-    int nb[8];
-    decode(a.topo, a.cell[x], nb, nullptr);
+    const int4 n0 = a.nb[2 * (size_t)x], n1 = a.nb[2 * (size_t)x + 1];
+    const int nb[8] = {n0.x, n0.y, n0.z, n0.w, n1.x, n1.y, n1.z, n1.w};
     const uint32_t meta = a.meta[x];
     switch (meta & 31u) {
 #define OPEN_CASE(n) case n: open_one<n, S, FLIP>(a, x, nb, meta); break;
@@ -361,14 +414,14 @@ __global__ __launch_bounds__(256) void k_wall_force(const double *gWall, int W, 
 
 // unit-level entry: collide n cells stored cell-major (27 per cell)
 template <class T>
-__global__ __launch_bounds__(256) void k_collide_batch(T *f, int n, CollisionParams P, T nu)
+__global__ __launch_bounds__(256) void k_collide_batch(T *f, int n, Relax<T> R)
 {
     const int c = blockIdx.x * 256 + threadIdx.x;
     if (c >= n) return;
     T v[27];
 #pragma unroll
     for (int q = 0; q < 27; ++q) v[q] = f[27 * (size_t)c + q];
-    collide(v, P, nu);
+    collide<T, 0>(v, R);
 #pragma unroll
     for (int q = 0; q < 27; ++q) f[27 * (size_t)c + q] = v[q];
 }
@@ -379,11 +432,44 @@ __global__ __launch_bounds__(256) void k_collide_batch(T *f, int n, CollisionPar
 static inline unsigned blocks_for(long long n) { return (unsigned)((n + 255) / 256); }
 
 template <class S>
+void launch_collide_wall(const StepArgs<S> &a, bool flip, hipStream_t s)
+{
+    if (a.W <= 0) return;
+    if (flip) hipLaunchKernelGGL((k_collide_wall<S, true>), dim3(blocks_for(a.W)), dim3(256), 0, s, a);
+    else hipLaunchKernelGGL((k_collide_wall<S, false>), dim3(blocks_for(a.W)), dim3(256), 0, s, a);
+}
+
+// dynamic LDS per bulk block: unused by the kernel, a cap on resident blocks per CU
+#ifndef LBM3_BULK_LDS
+#define LBM3_BULK_LDS 0
+#endif
+
+template <class S>
 void launch_collide_stream(const StepArgs<S> &a, bool flip, hipStream_t s)
 {
-    if (a.topo.N <= 0) return;
-    if (flip) hipLaunchKernelGGL((k_collide_stream<S, true>), dim3(blocks_for(a.topo.N)), dim3(256), 0, s, a);
-    else hipLaunchKernelGGL((k_collide_stream<S, false>), dim3(blocks_for(a.topo.N)), dim3(256), 0, s, a);
+    if (a.fast.N <= 0) return;
+    const dim3 g(blocks_for(a.fast.N)), b(256);
+    const unsigned lds = LBM3_BULK_LDS;
+    const bool k17 = relax_of<typename StoreTraits<S>::compute>(a.bulk).k17 != 0;
+    if (flip) {
+        if (k17) hipLaunchKernelGGL((k_collide_stream<S, true, 1>), g, b, lds, s, a);
+        else hipLaunchKernelGGL((k_collide_stream<S, true, 2>), g, b, lds, s, a);
+    } else {
+        if (k17) hipLaunchKernelGGL((k_collide_stream<S, false, 1>), g, b, lds, s, a);
+        else hipLaunchKernelGGL((k_collide_stream<S, false, 2>), g, b, lds, s, a);
+    }
+}
+
+void launch_open_nb(const Topology &t, int count, const int *cell, int4 *out, hipStream_t s)
+{
+    if (count <= 0) return;
+    hipLaunchKernelGGL(k_open_nb, dim3(blocks_for(count)), dim3(256), 0, s, t, count, cell, out);
+}
+
+void launch_fast_topo(const Topology &t, const int *wallMap, int2 *cellRow, int4 *rows, hipStream_t s)
+{
+    if (t.N <= 0) return;
+    hipLaunchKernelGGL(k_fast_topo, dim3(blocks_for(t.N)), dim3(256), 0, s, t, wallMap, cellRow, rows);
 }
 
 template <class S>
@@ -446,14 +532,15 @@ void launch_wall_force(const double *gWall, int W, const uint32_t *wallData, con
 }
 
 template <class T>
-void launch_collide_batch(T *f, int n, const CollisionParams &P, T nu, hipStream_t s)
+void launch_collide_batch(T *f, int n, const Relax<T> &R, hipStream_t s)
 {
     if (n <= 0) return;
-    hipLaunchKernelGGL((k_collide_batch<T>), dim3(blocks_for(n)), dim3(256), 0, s, f, n, P, nu);
+    hipLaunchKernelGGL((k_collide_batch<T>), dim3(blocks_for(n)), dim3(256), 0, s, f, n, R);
 }
 
 #define LBM3_INSTANTIATE(S)                                                                                          \
     template void launch_collide_stream<S>(const StepArgs<S> &, bool, hipStream_t);                                 \
+    template void launch_collide_wall<S>(const StepArgs<S> &, bool, hipStream_t);                                   \
     template void launch_open<S>(const OpenArgs<S> &, bool, hipStream_t);                                           \
     template void launch_periodic<S>(S *, const Topology &, int, const int *, const int *, bool, hipStream_t);        \
     template void launch_init<S>(S *, const Topology &, const InitDesc &, const LevelGeom &, bool, hipStream_t);     \
@@ -465,8 +552,8 @@ LBM3_INSTANTIATE(float)
 LBM3_INSTANTIATE(double)
 LBM3_INSTANTIATE(__half)
 LBM3_INSTANTIATE(__hip_bfloat16)
-template void launch_collide_batch<float>(float *, int, const CollisionParams &, float, hipStream_t);
-template void launch_collide_batch<double>(double *, int, const CollisionParams &, double, hipStream_t);
+template void launch_collide_batch<float>(float *, int, const Relax<float> &, hipStream_t);
+template void launch_collide_batch<double>(double *, int, const Relax<double> &, hipStream_t);
 
 } // namespace lbm3
 

@@ -14,7 +14,9 @@ struct InitDesc {
     const float *fields;      // kind 2: [N][4] (dRho, ux, uy, uz), device
 };
 
+// bulk cells (all but wall-adjacent and solid) and wall-adjacent cells; together one collide-stream step
 template <class S> void launch_collide_stream(const StepArgs<S> &a, bool flip, hipStream_t s);
+template <class S> void launch_collide_wall(const StepArgs<S> &a, bool flip, hipStream_t s);
 template <class S> void launch_open(const OpenArgs<S> &a, bool flip, hipStream_t s);
 template <class S> void launch_periodic(S *f, const Topology &t, int count, const int *index, const int *source, bool flip, hipStream_t s);
 template <class S> void launch_init(S *f, const Topology &t, const InitDesc &d, const LevelGeom &g, bool flip, hipStream_t s);
@@ -24,7 +26,9 @@ template <class S> int launch_kinetic_energy(const S *f, const Topology &t, cons
                                              double *partial, double *out, bool flip, hipStream_t s);
 void launch_wall_force(const double *gWall, int W, const uint32_t *wallData, const int *wallIndex, const Topology &t,
                        const LevelGeom &g, const double origin[3], int nBodies, double *partial, hipStream_t s);
-template <class T> void launch_collide_batch(T *f, int n, const CollisionParams &P, T nu, hipStream_t s);
+template <class T> void launch_collide_batch(T *f, int n, const Relax<T> &R, hipStream_t s);
+void launch_open_nb(const Topology &t, int count, const int *cell, int4 *out, hipStream_t s);
+void launch_fast_topo(const Topology &t, const int *wallMap, int2 *cellRow, int4 *rows, hipStream_t s);
 void launch_copy(const void *a, void *b, long long bytes, hipStream_t s);
 
 } // namespace lbm3

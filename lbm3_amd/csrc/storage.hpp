@@ -7,14 +7,42 @@
 #include <hip/hip_fp16.h>
 #include <hip/hip_bf16.h>
 
+// Population loads and stores of the 32/16-bit stores are non-temporal by
+// default: every slot is read once and written once per step, so caching them
+// only evicts the topology records.  Build with -DLBM3_LOAD_NT=0 /
+// -DLBM3_STORE_NT=0 for ordinary loads and stores.
+#ifndef LBM3_LOAD_NT
+#define LBM3_LOAD_NT 1
+#endif
+#ifndef LBM3_STORE_NT
+#define LBM3_STORE_NT 1
+#endif
+
 namespace lbm3 {
+
+template <class V> __device__ __forceinline__ V ld(const V *p)
+{
+#if LBM3_LOAD_NT
+    return __builtin_nontemporal_load(p);
+#else
+    return *p;
+#endif
+}
+template <class V> __device__ __forceinline__ void st(V *p, V v)
+{
+#if LBM3_STORE_NT
+    __builtin_nontemporal_store(v, p);
+#else
+    *p = v;
+#endif
+}
 
 template <class S> struct StoreTraits;
 
 template <> struct StoreTraits<float> {
     using compute = float;
-    static __device__ __forceinline__ float load(const float *p) { return __builtin_nontemporal_load(p); }
-    static __device__ __forceinline__ void store(float *p, float v) { __builtin_nontemporal_store(v, p); }
+    static __device__ __forceinline__ float load(const float *p) { return ld(p); }
+    static __device__ __forceinline__ void store(float *p, float v) { st(p, v); }
 };
 
 template <> struct StoreTraits<double> {
@@ -27,12 +55,12 @@ template <> struct StoreTraits<__half> {
     using compute = float;
     static __device__ __forceinline__ float load(const __half *p)
     {
-        const unsigned short bits = __builtin_nontemporal_load(reinterpret_cast<const unsigned short *>(p));
+        const unsigned short bits = ld(reinterpret_cast<const unsigned short *>(p));
         return __half2float(__ushort_as_half(bits));
     }
     static __device__ __forceinline__ void store(__half *p, float v)
     {
-        __builtin_nontemporal_store(__half_as_ushort(__float2half_rn(v)), reinterpret_cast<unsigned short *>(p));
+        st(reinterpret_cast<unsigned short *>(p), __half_as_ushort(__float2half_rn(v)));
     }
 };
 
@@ -40,7 +68,7 @@ template <> struct StoreTraits<__hip_bfloat16> {
     using compute = float;
     static __device__ __forceinline__ float load(const __hip_bfloat16 *p)
     {
-        const unsigned short bits = __builtin_nontemporal_load(reinterpret_cast<const unsigned short *>(p));
+        const unsigned short bits = ld(reinterpret_cast<const unsigned short *>(p));
         return __uint_as_float(static_cast<unsigned int>(bits) << 16);
     }
     static __device__ __forceinline__ void store(__hip_bfloat16 *p, float v)
@@ -48,7 +76,7 @@ template <> struct StoreTraits<__hip_bfloat16> {
         // round to nearest even on the upper 16 bits (finite values)
         unsigned int u = __float_as_uint(v);
         u += 0x7FFFu + ((u >> 16) & 1u);
-        __builtin_nontemporal_store(static_cast<unsigned short>(u >> 16), reinterpret_cast<unsigned short *>(p));
+        st(reinterpret_cast<unsigned short *>(p), static_cast<unsigned short>(u >> 16));
     }
 };
 

@@ -1,7 +1,9 @@
 """ctypes binding of liblbm3mi.so (include/lbm3mi.h).
 
-The library is loaded from the package directory; there is no fallback: if the
-HIP library is missing or no GPU is present the calls fail loudly."""
+The library is loaded from the package directory (or from the path in the
+LBM3MI_LIB environment variable, e.g. a build with other tuning defines);
+there is no fallback: if the HIP library is missing or no GPU is present the
+calls fail loudly."""
 import ctypes as C
 import os
 
@@ -10,7 +12,7 @@ import numpy as np
 from . import ctypes_defs as T
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "liblbm3mi.so")
+LIB_PATH = os.environ.get("LBM3MI_LIB") or os.path.join(HERE, "liblbm3mi.so")
 HEADER = os.path.join(HERE, "..", "include", "lbm3mi.h")
 
 _lib = None
@@ -25,7 +27,7 @@ def load():
     if _lib is not None:
         return _lib
     if not os.path.exists(LIB_PATH):
-        raise LBM3Error("liblbm3mi.so is not built (run __graft_entry__.build())")
+        raise LBM3Error("%s is not built (run __graft_entry__.build())" % LIB_PATH)
     L = C.CDLL(LIB_PATH)
     vp, i32, i64, f32, f64 = C.c_void_p, C.c_int, C.c_longlong, C.c_float, C.c_double
     sig = {
@@ -149,10 +151,11 @@ class Sim:
         check(load().lbm3_synchronize(self.h))
 
     def run_timed(self, steps, kernel_events=False):
-        """(total ms, summed level-0 collide-stream kernel ms, launches) over `steps` coarse steps."""
-        out = (C.c_double * 3)()
+        """(total ms, summed level-0 bulk collide-stream kernel ms, launches,
+        summed level-0 wall-adjacent collide kernel ms) over `steps` coarse steps."""
+        out = (C.c_double * 4)()
         check(load().lbm3_run_timed(self.h, steps, int(kernel_events), out))
-        return out[0], out[1], int(out[2])
+        return out[0], out[1], int(out[2]), out[3]
 
     # --- state ---
     def info(self, level=0):

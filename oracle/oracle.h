@@ -6,6 +6,25 @@
  * baseline leg of bench.py, as the checker.  Every function cites the
  * reference file:line it restates.  The product (lbm3_amd/, HIP for gfx950)
  * never links or calls anything here.
+ *
+ * The reference ships no tests or golden vectors for this path and cannot be
+ * built here (TNL, nvcc), so the restatement is pinned by the mathematics and
+ * the geometry (tests/test_oracle_pins.py, tests/test_boundary_tables.py):
+ *   lattice tables      names -> vectors, inverse pairs
+ *   feq / moments       closed-form zeroth, first and second moments
+ *   collision           mass, momentum + g, feq fixed point, omega1 = 1 at rest,
+ *                       cubic symmetry, shear decay (1 - omega1)
+ *   step + periodic     brute-force collide + pull streaming + ghost copy
+ *   viscosity           creeping Taylor-Green decay 6 nu k^2
+ *   force + moment BC   plane Poiseuille bulk curvature, no-slip nodes
+ *   MBBC / restore      tables equal the reference's generated code and
+ *                       reproduce the prescribed moments; uniform flow kept
+ *   voxelizer/builder   analytic box and sphere, link-code known answer
+ * Parity unpinned: the non-reflective characteristic update
+ * (updateGrid.h:267-307) beyond the uniform-flow fixed point, the near-wall
+ * layer of the moment-based walls and edge normals where ghost layers meet
+ * wall faces, and orc_update_interface (the multilevel transfer is not
+ * restated yet).
  */
 #ifndef LBM3_ORACLE_H
 #define LBM3_ORACLE_H

