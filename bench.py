@@ -102,6 +102,24 @@ def traffic_from_profiles(storage):
     return None if e is None else e.get("bytes_per_launch")
 
 
+def counters_workload():
+    """A short fixed sequence for rocprofv3 --pmc runs: the sphere workload for
+    10 steps in FP32 and in FP16, then three 1 GiB copies (the calibration
+    kernel whose bytes are known)."""
+    from lbm3_amd import lib
+    for st in ("f32", "f16"):
+        case, dom, tris = workload()
+        s = lib.Sim(case, st)
+        s.build(dom, tris)
+        s.init()
+        s.run(10)
+        s.sync()
+        s.close()
+        print("counters workload: %s sphere, 10 steps" % st, flush=True)
+    lib.copy_probe(1 << 30, 3)
+    print("counters workload: 3 copies of 1 GiB", flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -111,7 +129,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16")))
+    ap.add_argument("--counters-workload", action="store_true",
+                    help="run only the short sequence that the HBM byte counters are collected on")
     args = ap.parse_args()
+    if args.counters_workload:
+        counters_workload()
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (requested --gpus %d)" % args.gpus,
               file=sys.stderr)

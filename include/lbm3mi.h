@@ -49,6 +49,14 @@ typedef struct lbm3_level_arrays {
     int nPeriodic;
     const int *periodicIndex, *periodicSource;
     int wallCount;                /* cells marked as wall */
+    /* interface lists of this level towards the next finer one
+     * (types.h InterfaceStruct; gridBuilderFunctions.h:741-1030); zero counts on the finest level */
+    int f2cCount;
+    const int *f2cIndex, *f2cChild;       /* coarse cell, first fine cell of its 2x2x2 block */
+    int c2fCount;
+    const int *c2fIndex, *c2fChild;       /* first coarse cell of the 2x2x2 block, first fine target */
+    int leftoverCount;
+    const int *leftoverIndex, *leftoverParent, *leftoverNbrI, *leftoverNbrJ, *leftoverNbrK;
 } lbm3_level_arrays;
 
 /* Library. */

@@ -8,6 +8,7 @@ files in float32, so the builder sees the same numbers:
 * taylor_green  cases/taylorGreenVortex/taylorGreenVortexMain.cu:1-44, 82-102, 182-198
 * sphere        cases/flowAroundSphere/flowAroundSphereMain.cu:96-151, 183-197
 * channel       body-force driven periodic duct (serving mix; walls by moment BC)
+* multilevel_sphere  three-level sphere (coarse 128x64x64, nested boxes, finest patch 96^3)
 """
 import numpy as np
 
@@ -81,17 +82,19 @@ def bounds(xmin, ymin, zmin, xmax, ymax, zmax):
     return b
 
 
-def taylor_green(n=64, u0=0.05, reynolds=1600.0, pressure=False, limiter=None):
+def taylor_green(n=64, u0=0.05, reynolds=1600.0, pressure=False, limiter=None, levels=1):
     """Periodic Taylor-Green vortex on n^3 active cells (plus one ghost layer per face).
 
-    Lattice viscosity nu = u0 * (n / 2 pi) / Re, i.e. Re on the length n/(2 pi)."""
+    Lattice viscosity nu = u0 * (n / 2 pi) / Re, i.e. Re on the length n/(2 pi).
+    levels=2 refines the central cube [-pi L/2, pi L/2]^3 once, as the
+    reference's refined TGV run (taylorGreenVortexMain.cu:52-66)."""
     PI = f32(3.14159265358979323846)
     L = f32(1000.0)                       # reference length [mm]
     domain = f32(2.0) * PI * L
     res = domain / f32(n)
     nu_phys = f32(1.0) * (L / f32(1000.0)) / f32(reynolds)
     dt = (f32(u0) / f32(1.0)) * (res / f32(1000.0))
-    c = _case(1, res, dt, nu_phys)
+    c = _case(levels, res, dt, nu_phys)
     c.trackWallForce = 0
     c.init.kind = T.INIT_TAYLOR_GREEN
     c.init.u0 = float(f32(u0))
@@ -104,7 +107,29 @@ def taylor_green(n=64, u0=0.05, reynolds=1600.0, pressure=False, limiter=None):
     eps = f32(1e-6) * domain
     lo = -half - res - eps
     hi = half + res + eps
+    if levels > 1:
+        w = f32(0.25) * domain
+        add_refine_rule(c, 0, (-w, -w, -w, w, w, w))
     return c, bounds(lo, lo, lo, hi, hi, hi), []
+
+
+def multilevel_sphere(nx=128, ny=64, nz=64, diameter_fine=24.0, u_inlet=0.04, reynolds=100.0, seed=None,
+                      level1_box=(24, 16, 16, 72, 48, 48), level2_box=(28, 20, 20, 52, 44, 44),
+                      center=(40.0, 32.0, 32.0), segments=(60, 120), k15_on_level0=True):
+    """Flow around a sphere on three levels: a coarse nx*ny*nz grid of 1 mm cells
+    and two nested refinement boxes (factor 2 each), the finest around a sphere
+    of `diameter_fine` finest-level cells.  Boundaries as the sphere case.
+    With `seed`, the inlet velocity is drawn uniformly from [0.02, 0.06]."""
+    if seed is not None:
+        u_inlet = float(np.random.default_rng(seed).uniform(0.02, 0.06))
+    diameter = diameter_fine / 4.0                      # in coarse cells (1 mm)
+    c, dom, _ = sphere(nx, ny, nz, diameter=diameter, center=center, u_inlet=u_inlet, reynolds=reynolds,
+                       k15_on_level0=k15_on_level0, segments=segments)
+    c.gridLevelCount = 3
+    add_refine_rule(c, 0, level1_box)
+    add_refine_rule(c, 1, level2_box)
+    tri = geometry.uv_sphere(diameter / 2.0, center, segments[0], segments[1])
+    return c, dom, [tri]
 
 
 def sphere(nx=256, ny=128, nz=128, diameter=32.0, center=(64.0, 64.0, 64.0), u_inlet=0.05, reynolds=100.0,

@@ -231,8 +231,14 @@ int stl_upload(DevSTL &S, const float *tri, int n, float RES, hipStream_t s)
     const float *cx = S.v[6].p, *cy = S.v[7].p, *cz = S.v[8].p;
     const float ox = S.oxBin, oy = S.oyBin, oz = S.ozBin, tol = bs * 0.2f + RES * 3.f;
     const int BX = S.bx, BY = S.by, BZ = S.bz;
+    // total bin entries in 64 bits: a tolerance much larger than the body (tiny STL on a
+    // coarse grid) would overflow the int scan, so it is checked before anything is placed
+    DBuf<unsigned long long> entries;
+    BCHK(entries.alloc(1));
+    BCHK(hipMemsetAsync(entries.p, 0, sizeof(unsigned long long), s));
     for (int pass = 0; pass < 2; ++pass) {
         int *c = cnt.p, *cur = cursor.p, *first = S.firstInBin.p, *arr = S.binArray.p;
+        unsigned long long *ent = entries.p;
         for_n(S.T, s, [=] __device__(long long t) {
             const float x0 = fminf(ax[t] - ox, fminf(bx[t] - ox, cx[t] - ox)) - tol;
             const float y0 = fminf(ay[t] - oy, fminf(by[t] - oy, cy[t] - oy)) - tol;
@@ -242,6 +248,8 @@ int stl_upload(DevSTL &S, const float *tri, int n, float RES, hipStream_t s)
             const float z1 = fmaxf(az[t] - oz, fmaxf(bz[t] - oz, cz[t] - oz)) + tol;
             const int i0 = max(0, (int)(x0 / bs)), j0 = max(0, (int)(y0 / bs)), k0 = max(0, (int)(z0 / bs));
             const int i1 = min(BX, (int)(x1 / bs) + 1), j1 = min(BY, (int)(y1 / bs) + 1), k1 = min(BZ, (int)(z1 / bs) + 1);
+            if (pass == 0 && i1 > i0 && j1 > j0 && k1 > k0)
+                atomicAdd(ent, (unsigned long long)(i1 - i0) * (unsigned long long)(j1 - j0) * (unsigned long long)(k1 - k0));
             for (int kb = k0; kb < k1; ++kb)
                 for (int jb = j0; jb < j1; ++jb)
                     for (int ib = i0; ib < i1; ++ib) {
@@ -251,6 +259,12 @@ int stl_upload(DevSTL &S, const float *tri, int n, float RES, hipStream_t s)
                     }
         });
         if (pass == 0) {
+            unsigned long long total64 = 0;
+            BCHK(hipMemcpyAsync(&total64, entries.p, sizeof(total64), hipMemcpyDeviceToHost, s));
+            BCHK(hipStreamSynchronize(s));
+            if (total64 > 0x7fffffffULL)
+                return set_error(LBM3_E_BUILD, "sortSTLToBins: the STL is too small for the grid spacing (more than 2^31 "
+                                               "bin entries); use fewer, larger triangles or a finer grid");
             Scratch sc;
             if (exclusive_sum(sc, cnt.p, S.firstInBin.p, nb + 1, s)) return LBM3_E_HIP;
             const int total = read_one(S.firstInBin.p + nb, s);

@@ -217,6 +217,17 @@ static int check_level(Sim &s, int level)
     return 0;
 }
 
+// every level a sub-cycled update touches is built, and every interface it runs is checked
+static int check_run(Sim &s, int from)
+{
+    for (int L = from; L < s.levels; ++L) {
+        if (const int rc = check_level(s, L)) return rc;
+        if (L + 1 < s.levels && !s.L[L].interfaceChecked)
+            return set_error(LBM3_E_STATE, "interface lists of a level are not checked against the finer level");
+    }
+    return 0;
+}
+
 template <class T> int collide_cells(T *f, int n, const lbm3_bc *bc, T nu)
 {
     if (!f || !bc || n < 0) return set_error(LBM3_E_ARG, "bad collide arguments");
@@ -325,6 +336,28 @@ static int validate_level(const lbm3_level_arrays &a)
     for (long long x = 0; x < a.nPeriodic; ++x)
         if (a.periodicIndex[x] < 0 || a.periodicIndex[x] >= N || a.periodicSource[x] < 0 || a.periodicSource[x] >= N)
             return bad("periodic cell out of range");
+    if (a.f2cCount < 0 || a.c2fCount < 0 || a.leftoverCount < 0) return bad("negative interface count");
+    if ((a.f2cCount && (!a.f2cIndex || !a.f2cChild)) || (a.c2fCount && (!a.c2fIndex || !a.c2fChild)) ||
+        (a.leftoverCount && (!a.leftoverIndex || !a.leftoverParent || !a.leftoverNbrI || !a.leftoverNbrJ || !a.leftoverNbrK)))
+        return bad("missing interface arrays");
+    for (long long x = 0; x < a.f2cCount; ++x)
+        if (a.f2cIndex[x] < 0 || a.f2cIndex[x] >= N) return bad("F->C cell out of range");
+    for (long long x = 0; x < a.c2fCount; ++x)
+        if (a.c2fIndex[x] < 0 || a.c2fIndex[x] >= N) return bad("C->F cell out of range");
+    auto signedCell = [N](int c) { const long long v = c < 0 ? -(long long)c - 1 : c; return v >= 0 && v < N; };
+    for (long long x = 0; x < a.leftoverCount; ++x)
+        if (a.leftoverParent[x] < 0 || a.leftoverParent[x] >= N || !signedCell(a.leftoverNbrI[x]) ||
+            !signedCell(a.leftoverNbrJ[x]) || !signedCell(a.leftoverNbrK[x]))
+            return bad("leftover cell out of range");
+    return 0;
+}
+
+// fine-level cells named by a coarse level's interface lists must exist
+static int check_fine_refs(Level &coarse, long long nFine)
+{
+    for (const int c : coarse.fineRefs)
+        if (c < 0 || c >= nFine) return set_error(LBM3_E_ARG, "import: interface list names a fine cell out of range");
+    coarse.interfaceChecked = true;
     return 0;
 }
 
@@ -374,6 +407,26 @@ int lbm3_import_level(lbm3_sim *h, int level, const lbm3_level_arrays *a)
     l.nPeriodic = a->nPeriodic;
     if (upload(l.perIndex, a->periodicIndex, l.nPeriodic, st) || upload(l.perSource, a->periodicSource, l.nPeriodic, st))
         return LBM3_E_HIP;
+    // interface lists towards the finer level; their fine-level indices are checked
+    // when that level arrives (or now, if it is already there)
+    l.F2C.interfaceCount = a->f2cCount;
+    l.C2F.interfaceCount = a->c2fCount;
+    l.C2F.leftoverCount = a->leftoverCount;
+    if (upload(l.F2C.index, a->f2cIndex, a->f2cCount, st) || upload(l.F2C.childMap, a->f2cChild, a->f2cCount, st) ||
+        upload(l.C2F.index, a->c2fIndex, a->c2fCount, st) || upload(l.C2F.childMap, a->c2fChild, a->c2fCount, st) ||
+        upload(l.C2F.leftoverIndex, a->leftoverIndex, a->leftoverCount, st) ||
+        upload(l.C2F.leftoverParent, a->leftoverParent, a->leftoverCount, st) ||
+        upload(l.C2F.leftoverNbrI, a->leftoverNbrI, a->leftoverCount, st) ||
+        upload(l.C2F.leftoverNbrJ, a->leftoverNbrJ, a->leftoverCount, st) ||
+        upload(l.C2F.leftoverNbrK, a->leftoverNbrK, a->leftoverCount, st))
+        return LBM3_E_HIP;
+    l.fineRefs.clear();
+    l.fineRefs.insert(l.fineRefs.end(), a->f2cChild, a->f2cChild + a->f2cCount);
+    l.fineRefs.insert(l.fineRefs.end(), a->c2fChild, a->c2fChild + a->c2fCount);
+    l.fineRefs.insert(l.fineRefs.end(), a->leftoverIndex, a->leftoverIndex + a->leftoverCount);
+    l.interfaceChecked = l.fineRefs.empty();
+    if (level + 1 < s.levels && s.L[level + 1].built && check_fine_refs(l, s.L[level + 1].N)) return LBM3_E_ARG;
+    if (level > 0 && s.L[level - 1].built && check_fine_refs(s.L[level - 1], l.N)) return LBM3_E_ARG;
     const int rc = finish_level_setup(s, level);
     if (rc) return rc;
     HIPCHK(hipStreamSynchronize(st));
@@ -409,8 +462,7 @@ int lbm3_update_single_grid(lbm3_sim *h, int level)
 int lbm3_update_all_grids(lbm3_sim *h, int level)
 {
     if (!h) return set_error(LBM3_E_ARG, "null sim");
-    for (int L = level; L < h->s.levels; ++L)
-        if (const int rc = check_level(h->s, L)) return rc;
+    if (const int rc = check_run(h->s, level)) return rc;
     update_all(h->s, level);
     return check_hip(hipGetLastError(), "updateAllGrids launch");
 }
@@ -418,8 +470,7 @@ int lbm3_update_all_grids(lbm3_sim *h, int level)
 int lbm3_run(lbm3_sim *h, int steps)
 {
     if (!h || steps < 0) return set_error(LBM3_E_ARG, "bad run arguments");
-    for (int L = 0; L < h->s.levels; ++L)
-        if (const int rc = check_level(h->s, L)) return rc;
+    if (const int rc = check_run(h->s, 0)) return rc;
     for (int t = 0; t < steps; ++t) update_all(h->s, 0);
     return check_hip(hipGetLastError(), "run launch");
 }
@@ -434,8 +485,7 @@ int lbm3_run_timed(lbm3_sim *h, int steps, int kernelEvents, double *out)
 {
     if (!h || !out || steps <= 0) return set_error(LBM3_E_ARG, "bad timed-run arguments");
     Sim &s = h->s;
-    for (int L = 0; L < s.levels; ++L)
-        if (const int rc = check_level(s, L)) return rc;
+    if (const int rc = check_run(s, 0)) return rc;
     hipEvent_t a, b;
     HIPCHK(hipEventCreate(&a));
     HIPCHK(hipEventCreate(&b));

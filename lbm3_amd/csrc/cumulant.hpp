@@ -97,6 +97,60 @@ __host__ __device__ __forceinline__ void cm_backward(T k0, T k1, T k2, T u, T u2
     z = k0 - S;
 }
 
+// Well-conditioned central moments k_abc (x order a, y order b, z order c),
+// stored x-chain by x-chain: {k0jk, k1jk, k2jk} for (j,k) = 00, 01, 02, 10, 11, 12, 20, 21, 22.
+template <class T> struct CentralMoments {
+    T m000, m100, m200, m001, m101, m201, m002, m102, m202;
+    T m010, m110, m210, m011, m111, m211, m012, m112, m212;
+    T m020, m120, m220, m021, m121, m221, m022, m122, m222;
+};
+
+// central moments about u -> shifted populations: the inverse three-point
+// kernel along x, then y, then z, each chain carrying its weight moment
+template <class T>
+__host__ __device__ __forceinline__ void from_central_moments(T (&f)[27], const CentralMoments<T> &M, T ux, T uy, T uz)
+{
+    constexpr T W36 = T(1) / T(36), W9 = T(1) / T(9), W49 = T(4) / T(9);
+    constexpr T W6 = T(1) / T(6), W23 = T(2) / T(3), W18 = T(1) / T(18), W29 = T(2) / T(9);
+    constexpr T third = T(1) / T(3), ninth = T(1) / T(9);
+    const T ux2 = ux * ux, uy2 = uy * uy, uz2 = uz * uz;
+    T b_a00, b_b00, b_c00, b_a01, b_b01, b_c01, b_a02, b_b02, b_c02;
+    T b_a10, b_b10, b_c10, b_a11, b_b11, b_c11, b_a12, b_b12, b_c12;
+    T b_a20, b_b20, b_c20, b_a21, b_b21, b_c21, b_a22, b_b22, b_c22;
+    cm_backward(M.m000, M.m100, M.m200, ux, ux2, T(1), b_a00, b_b00, b_c00);
+    cm_backward(M.m001, M.m101, M.m201, ux, ux2, T(0), b_a01, b_b01, b_c01);
+    cm_backward(M.m002, M.m102, M.m202, ux, ux2, third, b_a02, b_b02, b_c02);
+    cm_backward(M.m010, M.m110, M.m210, ux, ux2, T(0), b_a10, b_b10, b_c10);
+    cm_backward(M.m011, M.m111, M.m211, ux, ux2, T(0), b_a11, b_b11, b_c11);
+    cm_backward(M.m012, M.m112, M.m212, ux, ux2, T(0), b_a12, b_b12, b_c12);
+    cm_backward(M.m020, M.m120, M.m220, ux, ux2, third, b_a20, b_b20, b_c20);
+    cm_backward(M.m021, M.m121, M.m221, ux, ux2, T(0), b_a21, b_b21, b_c21);
+    cm_backward(M.m022, M.m122, M.m222, ux, ux2, ninth, b_a22, b_b22, b_c22);
+
+    T a_aa0, a_ab0, a_ac0, a_ba0, a_bb0, a_bc0, a_ca0, a_cb0, a_cc0;
+    T a_aa1, a_ab1, a_ac1, a_ba1, a_bb1, a_bc1, a_ca1, a_cb1, a_cc1;
+    T a_aa2, a_ab2, a_ac2, a_ba2, a_bb2, a_bc2, a_ca2, a_cb2, a_cc2;
+    cm_backward(b_a00, b_a10, b_a20, uy, uy2, W6, a_aa0, a_ab0, a_ac0);
+    cm_backward(b_b00, b_b10, b_b20, uy, uy2, W23, a_ba0, a_bb0, a_bc0);
+    cm_backward(b_c00, b_c10, b_c20, uy, uy2, W6, a_ca0, a_cb0, a_cc0);
+    cm_backward(b_a01, b_a11, b_a21, uy, uy2, T(0), a_aa1, a_ab1, a_ac1);
+    cm_backward(b_b01, b_b11, b_b21, uy, uy2, T(0), a_ba1, a_bb1, a_bc1);
+    cm_backward(b_c01, b_c11, b_c21, uy, uy2, T(0), a_ca1, a_cb1, a_cc1);
+    cm_backward(b_a02, b_a12, b_a22, uy, uy2, W18, a_aa2, a_ab2, a_ac2);
+    cm_backward(b_b02, b_b12, b_b22, uy, uy2, W29, a_ba2, a_bb2, a_bc2);
+    cm_backward(b_c02, b_c12, b_c22, uy, uy2, W18, a_ca2, a_cb2, a_cc2);
+
+    cm_backward(a_aa0, a_aa1, a_aa2, uz, uz2, W36, f[MMM], f[MMO], f[MMP]);
+    cm_backward(a_ab0, a_ab1, a_ab2, uz, uz2, W9, f[MOM], f[MOO], f[MOP]);
+    cm_backward(a_ac0, a_ac1, a_ac2, uz, uz2, W36, f[MPM], f[MPO], f[MPP]);
+    cm_backward(a_ba0, a_ba1, a_ba2, uz, uz2, W9, f[OMM], f[OMO], f[OMP]);
+    cm_backward(a_bb0, a_bb1, a_bb2, uz, uz2, W49, f[OOM], f[OOO], f[OOP]);
+    cm_backward(a_bc0, a_bc1, a_bc2, uz, uz2, W9, f[OPM], f[OPO], f[OPP]);
+    cm_backward(a_ca0, a_ca1, a_ca2, uz, uz2, W36, f[PMM], f[PMO], f[PMP]);
+    cm_backward(a_cb0, a_cb1, a_cb2, uz, uz2, W9, f[POM], f[POO], f[POP]);
+    cm_backward(a_cc0, a_cc1, a_cc2, uz, uz2, W36, f[PPM], f[PPO], f[PPP]);
+}
+
 // f[27] in/out (shifted populations).  MODE 0: K17 or K15 from R.k17 at run
 // time; 1: K17; 2: K15.
 template <class T, int MODE = 0>
@@ -251,44 +305,11 @@ __host__ __device__ __forceinline__ void collide(T (&f)[27], const Relax<T> &R)
                                   (s002 + s020 + s200) * third) * r2
                  - (dRho * dRho - dRho) * r2 * (T(1) / T(27));
 
-    // ---- back transform: x pass (first moments flip sign: momentum gains the force) ----
-    T b_a00, b_b00, b_c00, b_a01, b_b01, b_c01, b_a02, b_b02, b_c02;
-    T b_a10, b_b10, b_c10, b_a11, b_b11, b_c11, b_a12, b_b12, b_c12;
-    T b_a20, b_b20, b_c20, b_a21, b_b21, b_c21, b_a22, b_b22, b_c22;
-    cm_backward(dRho, -k100, s200, ux, ux2, T(1), b_a00, b_b00, b_c00);
-    cm_backward(-k001, s101, s201, ux, ux2, T(0), b_a01, b_b01, b_c01);
-    cm_backward(s002, s102, q202, ux, ux2, third, b_a02, b_b02, b_c02);
-    cm_backward(-k010, s110, s210, ux, ux2, T(0), b_a10, b_b10, b_c10);
-    cm_backward(s011, s111, q211, ux, ux2, T(0), b_a11, b_b11, b_c11);
-    cm_backward(s012, q112, q212, ux, ux2, T(0), b_a12, b_b12, b_c12);
-    cm_backward(s020, s120, q220, ux, ux2, third, b_a20, b_b20, b_c20);
-    cm_backward(s021, q121, q221, ux, ux2, T(0), b_a21, b_b21, b_c21);
-    cm_backward(q022, q122, q222, ux, ux2, ninth, b_a22, b_b22, b_c22);
-
-    // ---- y pass ----
-    T a_aa0, a_ab0, a_ac0, a_ba0, a_bb0, a_bc0, a_ca0, a_cb0, a_cc0;
-    T a_aa1, a_ab1, a_ac1, a_ba1, a_bb1, a_bc1, a_ca1, a_cb1, a_cc1;
-    T a_aa2, a_ab2, a_ac2, a_ba2, a_bb2, a_bc2, a_ca2, a_cb2, a_cc2;
-    cm_backward(b_a00, b_a10, b_a20, uy, uy2, W6, a_aa0, a_ab0, a_ac0);
-    cm_backward(b_b00, b_b10, b_b20, uy, uy2, W23, a_ba0, a_bb0, a_bc0);
-    cm_backward(b_c00, b_c10, b_c20, uy, uy2, W6, a_ca0, a_cb0, a_cc0);
-    cm_backward(b_a01, b_a11, b_a21, uy, uy2, T(0), a_aa1, a_ab1, a_ac1);
-    cm_backward(b_b01, b_b11, b_b21, uy, uy2, T(0), a_ba1, a_bb1, a_bc1);
-    cm_backward(b_c01, b_c11, b_c21, uy, uy2, T(0), a_ca1, a_cb1, a_cc1);
-    cm_backward(b_a02, b_a12, b_a22, uy, uy2, W18, a_aa2, a_ab2, a_ac2);
-    cm_backward(b_b02, b_b12, b_b22, uy, uy2, W29, a_ba2, a_bb2, a_bc2);
-    cm_backward(b_c02, b_c12, b_c22, uy, uy2, W18, a_ca2, a_cb2, a_cc2);
-
-    // ---- z pass -> populations ----
-    cm_backward(a_aa0, a_aa1, a_aa2, uz, uz2, W36, f[MMM], f[MMO], f[MMP]);
-    cm_backward(a_ab0, a_ab1, a_ab2, uz, uz2, W9, f[MOM], f[MOO], f[MOP]);
-    cm_backward(a_ac0, a_ac1, a_ac2, uz, uz2, W36, f[MPM], f[MPO], f[MPP]);
-    cm_backward(a_ba0, a_ba1, a_ba2, uz, uz2, W9, f[OMM], f[OMO], f[OMP]);
-    cm_backward(a_bb0, a_bb1, a_bb2, uz, uz2, W49, f[OOM], f[OOO], f[OOP]);
-    cm_backward(a_bc0, a_bc1, a_bc2, uz, uz2, W9, f[OPM], f[OPO], f[OPP]);
-    cm_backward(a_ca0, a_ca1, a_ca2, uz, uz2, W36, f[PMM], f[PMO], f[PMP]);
-    cm_backward(a_cb0, a_cb1, a_cb2, uz, uz2, W9, f[POM], f[POO], f[POP]);
-    cm_backward(a_cc0, a_cc1, a_cc2, uz, uz2, W36, f[PPM], f[PPO], f[PPP]);
+    // ---- back transform (first moments flip sign: momentum gains the force) ----
+    const CentralMoments<T> M{dRho, -k100, s200, -k001, s101, s201, s002, s102, q202,
+                              -k010, s110, s210, s011, s111, q211, s012, q112, q212,
+                              s020, s120, q220, s021, q121, q221, q022, q122, q222};
+    from_central_moments(f, M, ux, uy, uz);
 }
 
 // getDRhoUxUyUz (cellFunctions.h:257-279): density deviation and velocity

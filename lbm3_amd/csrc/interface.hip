@@ -1,12 +1,262 @@
-// Multilevel coarse<->fine interface transfer (reference include/updateInterface.h).
+// Multilevel coarse <-> fine interface transfer for gfx950 (reference
+// include/updateInterface.h:155-890), run after every pair of fine steps.
+//
+//  k_fine_to_coarse  one coarse cell per group of 8 lanes: lane n loads fine
+//                    stencil corner n, the group shares the corner values in
+//                    LDS, every lane builds the interpolant and writes 1/8 of
+//                    the coarse cell's 27 populations
+//  k_coarse_to_fine  one 2x2x2 coarse block per group of 8 lanes: lane n loads
+//                    coarse corner n, shares it, and writes fine target n
+//  k_leftovers       fine cells outside every full block: centre coarse cell
+//                    plus three signed face neighbours (one lane each)
+//
+// Populations are read and written in the pre-collision slots of the current
+// esotwist parity of their level.
+#include <hip/hip_runtime.h>
+#include "interface.hpp"
 #include "sim.hpp"
 
 namespace lbm3 {
 
+namespace {
+
+template <class S, bool FLIP>
+__device__ __forceinline__ void load_cell(const S *f, const FastTopo &t, int cell, typename StoreTraits<S>::compute (&v)[27],
+                                          int (&nb)[8])
+{
+    decode_fast(t, cell, t.cellRow[cell].y, nb);
+#pragma unroll
+    for (int q = 0; q < 27; ++q) v[q] = StoreTraits<S>::load(f + (size_t)pre_plane(q, FLIP) * (size_t)t.N + nb[pre_nb(q)]);
+}
+
+// the stencil whose lowest corner is `c0`: self, i+, j+, ij+, k+, ik+, jk+, ijk+
+__device__ __forceinline__ int stencil_corner(const FastTopo &t, int c0, int n)
+{
+    int nb[8];
+    decode_fast(t, c0, t.cellRow[c0].y, nb);
+    // decode order: self, i+, j+, k+, ij+, ik+, jk+, ijk+
+    const int map[8] = {0, 1, 2, 4, 3, 5, 6, 7};
+    return nb[map[n]];
+}
+
+template <class T> struct IfArgs {
+    FastTopo coarse, fine;
+    void *fc, *ff;                 // populations of the two levels (storage type)
+    T w1c, w1f;                    // omega1 of the two levels
+    int count;
+    const int *index, *child;      // interface lists of the coarse level
+    const int *loIndex, *loParent, *loI, *loJ, *loK;
+    const int *wallMapC;
+    const uint32_t *wallDataC;
+    const LocalBC *wallBCC;        // [32] local BC of the coarse level by wall id
+    LevelGeom geomC;
+    Topology topoC;
+};
+
+// F->C: the coarse cell at the centre of a 2x2x2 fine stencil
+// (updateInterface.h:155-360)
+template <class S, bool FLIPC, bool FLIPF>
+__global__ __launch_bounds__(256) void k_fine_to_coarse(IfArgs<typename StoreTraits<S>::compute> a)
+{
+    using T = typename StoreTraits<S>::compute;
+    __shared__ CellValues<T> lds[256];
+    const int gid = blockIdx.x * 256 + threadIdx.x;
+    const int item = gid >> 3, n = gid & 7;
+    const bool active = item < a.count;
+    if (active) {
+        const int cell = stencil_corner(a.fine, a.child[item], n);
+        T v[27];
+        int nb[8];
+        load_cell<S, FLIPF>(reinterpret_cast<const S *>(a.ff), a.fine, cell, v, nb);
+        lds[threadIdx.x] = cell_values(v, a.w1f);
+    }
+    __syncthreads();
+    if (!active) return;
+    const int base = threadIdx.x & ~7;
+    CellValues<T> cv[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) cv[k] = lds[base + k];
+    const Interpolant<T> P = interpolant(cv);
+    T d, u[3], g[3][3];
+    evaluate(P, T(0), T(0), T(0), d, u, g);
+    // density: the plain mean of the eight fine cells (updateInterface.h:235-237)
+    T mean = 0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) mean += cv[k].d;
+    d = mean * T(0.125);
+    T f[27];
+    reconstruct(f, d, u, g[0][1] + g[1][0], g[1][2] + g[2][1], g[0][2] + g[2][0], g[0][0] - g[1][1], g[0][0] - g[2][2],
+                T(2) / a.w1c);
+    const int cell = a.index[item];
+    int nb[8];
+    decode_fast(a.coarse, cell, a.coarse.cellRow[cell].y, nb);
+    S *fc = reinterpret_cast<S *>(a.fc);
+    // lane n writes populations n, n+8, n+16, n+24
+#pragma unroll
+    for (int q = 0; q < 27; ++q)
+        if ((q & 7) == n) StoreTraits<S>::store(fc + (size_t)pre_plane(q, FLIPC) * (size_t)a.coarse.N + nb[pre_nb(q)], f[q]);
+}
+
+// C->F: the eight fine cells inside a 2x2x2 coarse block, at +-1/4
+// (updateInterface.h:362-687)
+template <class S, bool FLIPC, bool FLIPF>
+__global__ __launch_bounds__(256) void k_coarse_to_fine(IfArgs<typename StoreTraits<S>::compute> a)
+{
+    using T = typename StoreTraits<S>::compute;
+    __shared__ CellValues<T> lds[256];
+    const int gid = blockIdx.x * 256 + threadIdx.x;
+    const int item = gid >> 3, n = gid & 7;
+    const bool active = item < a.count;
+    if (active) {
+        const int cell = stencil_corner(a.coarse, a.index[item], n);
+        T v[27];
+        int nb[8];
+        load_cell<S, FLIPC>(reinterpret_cast<const S *>(a.fc), a.coarse, cell, v, nb);
+        lds[threadIdx.x] = cell_values(v, a.w1c);
+    }
+    __syncthreads();
+    if (!active) return;
+    const int base = threadIdx.x & ~7;
+    CellValues<T> cv[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) cv[k] = lds[base + k];
+    const Interpolant<T> P = interpolant(cv);
+    const T x = (n & 1) ? T(0.25) : T(-0.25), y = (n & 2) ? T(0.25) : T(-0.25), z = (n & 4) ? T(0.25) : T(-0.25);
+    T d, u[3], g[3][3];
+    evaluate(P, x, y, z, d, u, g);
+    T f[27];
+    reconstruct(f, d, u, g[0][1] + g[1][0], g[1][2] + g[2][1], g[0][2] + g[2][0], g[0][0] - g[1][1], g[0][0] - g[2][2],
+                T(0.5) / a.w1f);
+    const int cell = stencil_corner(a.fine, a.child[item], n);
+    int nb[8];
+    decode_fast(a.fine, cell, a.fine.cellRow[cell].y, nb);
+    S *ff = reinterpret_cast<S *>(a.ff);
+#pragma unroll
+    for (int q = 0; q < 27; ++q) StoreTraits<S>::store(ff + (size_t)pre_plane(q, FLIPF) * (size_t)a.fine.N + nb[pre_nb(q)], f[q]);
+}
+
+// fine cells without a full coarse block (updateInterface.h:689-884)
+template <class S, bool FLIPC, bool FLIPF>
+__global__ __launch_bounds__(256) void k_leftovers(IfArgs<typename StoreTraits<S>::compute> a)
+{
+    using T = typename StoreTraits<S>::compute;
+    const int x = blockIdx.x * 256 + threadIdx.x;
+    if (x >= a.count) return;
+    const S *fc = reinterpret_cast<const S *>(a.fc);
+    const int centre = a.loParent[x];
+    T v[27];
+    int nb[8];
+    load_cell<S, FLIPC>(fc, a.coarse, centre, v, nb);
+    const CellValues<T> c0 = cell_values(v, a.w1c);
+    // wall velocity of the centre cell: the local BC of its wall id (the cell's own velocity if none is set)
+    const int wm = a.wallMapC[centre];
+    const uint32_t wd = wm >= 0 ? a.wallDataC[wm] : 0u;
+    T uw[3] = {c0.u[0], c0.u[1], c0.u[2]};
+    if (wm >= 0) {
+        const LocalBC &b = a.wallBCC[(wd >> 27) & 31u];
+        if (b.velMode == 1) { uw[0] = T(b.ux); uw[1] = T(b.uy); uw[2] = T(b.uz); }
+        else if (b.velMode == 2) {
+            int nbx[8], ijk[3];
+            decode(a.topoC, centre, nbx, ijk);
+            const float px = float(ijk[0]) * a.geomC.res + a.geomC.ox - b.origin[0];
+            const float py = float(ijk[1]) * a.geomC.res + a.geomC.oy - b.origin[1];
+            const float pz = float(ijk[2]) * a.geomC.res + a.geomC.oz - b.origin[2];
+            const float s = a.geomC.dtPhys / a.geomC.res;
+            uw[0] = T((b.omega[1] * pz - b.omega[2] * py) * s);
+            uw[1] = T((b.omega[2] * px - b.omega[0] * pz) * s);
+            uw[2] = T((b.omega[0] * py - b.omega[1] * px) * s);
+        }
+    }
+    CellValues<T> r = c0;
+    const int nbr[3] = {a.loI[x], a.loJ[x], a.loK[x]};
+    const T q = T(0.25);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const bool opposite = nbr[k] < 0;
+        const int cell = opposite ? -nbr[k] - 1 : nbr[k];
+        load_cell<S, FLIPC>(fc, a.coarse, cell, v, nb);
+        const CellValues<T> c = cell_values(v, a.w1c);
+        // a neighbour on the + side adds the linear slope, one on the - side removes it
+        const T sgn = opposite ? T(-1) : T(1);
+        r.d += sgn * q * (c.d - c0.d);
+        r.sxy += sgn * q * (c.sxy - c0.sxy); r.syz += sgn * q * (c.syz - c0.syz); r.sxz += sgn * q * (c.sxz - c0.sxz);
+        r.nxy += sgn * q * (c.nxy - c0.nxy); r.nxz += sgn * q * (c.nxz - c0.nxz);
+        const int dir = k == 0 ? POO : (k == 1 ? OPO : OOP);
+        const bool wall = opposite && (((wd >> dir) & 1u) || ((wd >> inv(dir)) & 1u));
+        if (!wall) {
+#pragma unroll
+            for (int m = 0; m < 3; ++m) r.u[m] += sgn * q * (c.u[m] - c0.u[m]);
+        } else {
+            // quadratic through the wall (1/2 away), the centre and the opposite cell
+#pragma unroll
+            for (int m = 0; m < 3; ++m) r.u[m] += (T(1) / T(24)) * (T(10) * uw[m] - T(9) * c0.u[m] - c.u[m]);
+        }
+    }
+    T f[27];
+    reconstruct(f, r.d, r.u, r.sxy, r.syz, r.sxz, r.nxy, r.nxz, T(0.5) / a.w1f);
+    const int cell = a.loIndex[x];
+    decode_fast(a.fine, cell, a.fine.cellRow[cell].y, nb);
+    S *ff = reinterpret_cast<S *>(a.ff);
+#pragma unroll
+    for (int m = 0; m < 27; ++m) StoreTraits<S>::store(ff + (size_t)pre_plane(m, FLIPF) * (size_t)a.fine.N + nb[pre_nb(m)], f[m]);
+}
+
+inline unsigned groups_of_8(long long items) { return (unsigned)((8 * items + 255) / 256); }
+
+template <class S, bool FC, bool FF> void launch_all(const IfArgs<typename StoreTraits<S>::compute> &base, const Level &c,
+                                                      hipStream_t s)
+{
+    IfArgs<typename StoreTraits<S>::compute> a = base;
+    if (c.F2C.interfaceCount > 0) {
+        a.count = c.F2C.interfaceCount; a.index = c.F2C.index.p; a.child = c.F2C.childMap.p;
+        hipLaunchKernelGGL((k_fine_to_coarse<S, FC, FF>), dim3(groups_of_8(a.count)), dim3(256), 0, s, a);
+    }
+    if (c.C2F.interfaceCount > 0) {
+        a.count = c.C2F.interfaceCount; a.index = c.C2F.index.p; a.child = c.C2F.childMap.p;
+        hipLaunchKernelGGL((k_coarse_to_fine<S, FC, FF>), dim3(groups_of_8(a.count)), dim3(256), 0, s, a);
+    }
+    if (c.C2F.leftoverCount > 0) {
+        a.count = c.C2F.leftoverCount;
+        hipLaunchKernelGGL((k_leftovers<S, FC, FF>), dim3((a.count + 255) / 256), dim3(256), 0, s, a);
+    }
+}
+
+template <class S> void interface_typed(Sim &s, int coarse)
+{
+    using T = typename StoreTraits<S>::compute;
+    Level &c = s.L[coarse], &f = s.L[coarse + 1];
+    IfArgs<T> a{};
+    a.coarse = FastTopo{c.N, c.cellRow.p, c.rowTab.p};
+    a.fine = FastTopo{f.N, f.cellRow.p, f.rowTab.p};
+    a.fc = c.f.p;
+    a.ff = f.f.p;
+    // omega1 = 1 / (3 nu + 1/2) of each level, in the compute precision
+    a.w1c = T(1) / (T(3) * T(c.info.nu) + T(0.5));
+    a.w1f = T(1) / (T(3) * T(f.info.nu) + T(0.5));
+    a.loIndex = c.C2F.leftoverIndex.p; a.loParent = c.C2F.leftoverParent.p;
+    a.loI = c.C2F.leftoverNbrI.p; a.loJ = c.C2F.leftoverNbrJ.p; a.loK = c.C2F.leftoverNbrK.p;
+    a.wallMapC = c.wallMap.p;
+    a.wallDataC = c.wallData.p;
+    a.wallBCC = c.wallBC.p;
+    a.geomC = c.geom;
+    a.topoC = Topology{c.N, c.shifter.p, c.ri.p, c.rj.p, c.rk.p, c.rjp.p, c.rkp.p, c.rjkp.p};
+    const bool fc = c.info.esotwistFlipper != 0, ff = f.info.esotwistFlipper != 0;
+    if (fc) { if (ff) launch_all<S, true, true>(a, c, s.stream); else launch_all<S, true, false>(a, c, s.stream); }
+    else { if (ff) launch_all<S, false, true>(a, c, s.stream); else launch_all<S, false, false>(a, c, s.stream); }
+}
+
+} // namespace
+
+// updateInterface (updateInterface.h:886-890): F->C, then C->F on the coarse
+// values it has just written, then the leftovers
 void update_interface(Sim &s, int coarse)
 {
-    (void)s;
-    (void)coarse;
+    switch (s.storage) {
+    case LBM3_STORE_F64: interface_typed<double>(s, coarse); break;
+    case LBM3_STORE_F16: interface_typed<__half>(s, coarse); break;
+    case LBM3_STORE_BF16: interface_typed<__hip_bfloat16>(s, coarse); break;
+    default: interface_typed<float>(s, coarse); break;
+    }
 }
 
 } // namespace lbm3

@@ -65,6 +65,8 @@ struct Level {
     int nPeriodic = 0;
     DBuf<int> perIndex, perSource;
     InterfaceLists C2F, F2C;
+    std::vector<int> fineRefs;       // imported lists: fine cells they name, checked against the finer level
+    bool interfaceChecked = true;
     LocalBC bulk{};
     DBuf<LocalBC> wallBC;            // [32]
     DBuf<unsigned char> f;           // [27][N] storage type

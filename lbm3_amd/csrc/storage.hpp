@@ -7,15 +7,17 @@
 #include <hip/hip_fp16.h>
 #include <hip/hip_bf16.h>
 
-// Population loads and stores of the 32/16-bit stores are non-temporal by
-// default: every slot is read once and written once per step, so caching them
-// only evicts the topology records.  Build with -DLBM3_LOAD_NT=0 /
-// -DLBM3_STORE_NT=0 for ordinary loads and stores.
+// Population loads and stores of the 32/16-bit stores are ordinary cached
+// accesses by default.  Non-temporal hints (-DLBM3_LOAD_NT=1 / -DLBM3_STORE_NT=1)
+// were measured 1.47x slower on the 256x128x128 sphere bulk kernel: half of
+// the planes are read and written one cell off the 128-byte line grid (the i+
+// neighbour), and the neighbouring wave's half of each such line must still
+// be in L2 when it arrives.
 #ifndef LBM3_LOAD_NT
-#define LBM3_LOAD_NT 1
+#define LBM3_LOAD_NT 0
 #endif
 #ifndef LBM3_STORE_NT
-#define LBM3_STORE_NT 1
+#define LBM3_STORE_NT 0
 #endif
 
 namespace lbm3 {

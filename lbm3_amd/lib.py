@@ -76,7 +76,15 @@ class LevelArrays(C.Structure):
                [("W", C.c_int)] + [(n, C.c_void_p) for n in ("wallMap", "wallIndex", "wallData", "linkLength")] + \
                [("nOpenIDs", C.c_int)] + [(n, C.c_void_p) for n in ("openCount", "openIndex", "openSource")] + \
                [("nPeriodic", C.c_int), ("periodicIndex", C.c_void_p), ("periodicSource", C.c_void_p),
-                ("wallCount", C.c_int)]
+                ("wallCount", C.c_int)] + \
+               [("f2cCount", C.c_int), ("f2cIndex", C.c_void_p), ("f2cChild", C.c_void_p),
+                ("c2fCount", C.c_int), ("c2fIndex", C.c_void_p), ("c2fChild", C.c_void_p),
+                ("leftoverCount", C.c_int)] + \
+               [(n, C.c_void_p) for n in ("leftoverIndex", "leftoverParent", "leftoverNbrI", "leftoverNbrJ",
+                                          "leftoverNbrK")]
+
+INTERFACE_ARRAYS = ("f2cIndex", "f2cChild", "c2fIndex", "c2fChild", "leftoverIndex", "leftoverParent",
+                    "leftoverNbrI", "leftoverNbrJ", "leftoverNbrK")
 
 
 STORE = {"f32": T.STORE_F32, "f16": T.STORE_F16, "bf16": T.STORE_BF16, "f64": T.STORE_F64}
@@ -129,6 +137,13 @@ class Sim:
         a.nOpenIDs = len(keep["openCount"])
         a.nPeriodic = len(keep["periodicIndex"])
         a.wallCount = int(np.asarray(keep.get("wallCount", 0)).reshape(-1)[0])
+        for n in INTERFACE_ARRAYS:
+            v = keep.get(n)
+            if v is not None and v.size:
+                setattr(a, n, v.ctypes.data)
+        a.f2cCount = len(keep.get("f2cIndex", ()))
+        a.c2fCount = len(keep.get("c2fIndex", ()))
+        a.leftoverCount = len(keep.get("leftoverIndex", ()))
         check(load().lbm3_import_level(self.h, level, C.byref(a)))
 
     # --- run ---

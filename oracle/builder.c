@@ -80,7 +80,7 @@ static void orc_hook_refine(const lbm3_case *C, const lbm3_info *I, int i, int j
  * STL: readSTL STLFunctions.h:317-462, sortSTLToBins :5-128, checkSTLEdges :130-315
  * ---------------------------------------------------------------------- */
 
-static void stl_bins(orc_stl *S, float RES)
+static int stl_bins(orc_stl *S, float RES)
 {
     const lbm3_bounds *B = &S->Bounds;
     const int T = S->triangleCount;
@@ -121,11 +121,12 @@ static void stl_bins(orc_stl *S, float RES)
                     }
         }
         if (!pass) {
-            int run = 0;
-            for (int b = 0; b <= nb; ++b) { const int c = S->firstInBin[b]; S->firstInBin[b] = run; run += c; }
+            long long run = 0;
+            for (int b = 0; b <= nb; ++b) { const int c = S->firstInBin[b]; S->firstInBin[b] = (int)run; run += c; if (run > 0x7fffffffLL) return -1; }
             S->binArray = NEWI(run);
         } else free(cursor);
     }
+    return 0;
 }
 
 static void stl_edges(orc_stl *S)
@@ -187,7 +188,7 @@ int orc_stl_from_triangles(orc_stl *S, const float *tri, int n, float RES)
     S->cx = a[6]; S->cy = a[7]; S->cz = a[8];
     S->Bounds = B;
     if (w == 0) return -1;
-    stl_bins(S, RES);
+    if (stl_bins(S, RES)) return -1;   /* more than 2^31 bin entries */
     stl_edges(S);
     return 0;
 }

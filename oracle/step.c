@@ -28,6 +28,7 @@ lbm3_bc orc_hook_open_bc(const lbm3_case *C, const lbm3_info *I, int i, int j, i
 #define orc_mbbc_f32 orc_mbbc_f32_s
 #include "real_impl.h"
 #include "step_impl.h"
+#include "interface_impl.h"
 #undef REAL
 #undef NAME
 #undef ORC_ABS
@@ -49,6 +50,7 @@ lbm3_bc orc_hook_open_bc(const lbm3_case *C, const lbm3_info *I, int i, int j, i
 #define orc_mbbc_f64 orc_mbbc_f64_s
 #include "real_impl.h"
 #include "step_impl.h"
+#include "interface_impl.h"
 #undef REAL
 #undef NAME
 
@@ -86,7 +88,8 @@ void orc_update_all_grids(orc_sim *S, int L)
 
 void orc_update_interface(orc_sim *S, int coarse)
 {
-    (void)S; (void)coarse; /* multilevel transfer: interface.c */
+    if (S->isDouble) orc_update_interface_f64(S, coarse);
+    else orc_update_interface_f32(S, coarse);
 }
 
 /* ---- ctypes accessors ---- */

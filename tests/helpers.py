@@ -14,6 +14,10 @@ def level_arrays(osim, level=0):
     a["openIndex"] = np.concatenate(idx).astype(np.int32) if idx else np.zeros(0, np.int32)
     a["openSource"] = np.concatenate(src).astype(np.int32) if src else np.zeros(0, np.int32)
     a["wallCount"] = np.array(int((a["wallMap"] == -3).sum()))
+    if level < osim.levels - 1:
+        for n in ("f2cIndex", "f2cChild", "c2fIndex", "c2fChild", "leftoverIndex", "leftoverParent", "leftoverNbrI",
+                  "leftoverNbrJ", "leftoverNbrK"):
+            a[n] = osim.array(n, level).astype(np.int32)
     return a
 
 

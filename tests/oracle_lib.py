@@ -51,6 +51,10 @@ def lib():
         L.orc_moments_batch_f32.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
         L.orc_moments_batch_f64.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
         L.orc_openbc_cell_f32.argtypes = [C.c_int, C.c_int, C.POINTER(T.BC), C.c_void_p]
+        for t, real in (("f32", C.c_float), ("f64", C.c_double)):
+            getattr(L, "orc_if_values_batch_" + t).argtypes = [C.c_void_p, C.c_int, real, C.c_void_p]
+            getattr(L, "orc_if_coefficients_" + t).argtypes = [C.c_void_p, C.c_void_p]
+            getattr(L, "orc_if_reconstruct_" + t).argtypes = [C.c_void_p, C.c_void_p]
         _lib = L
     return _lib
 

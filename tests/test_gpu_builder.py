@@ -56,3 +56,31 @@ def test_builder_small_sphere():
 
 def test_builder_offcentre_sphere_fine_mesh():
     _check(*cases.sphere(48, 40, 36, diameter=13.3, center=(17.3, 19.1, 18.2), segments=(60, 120)))
+
+
+def _cut_sphere():
+    """Two levels whose refinement box ends inside the sphere: leftover cells with
+    opposite-side neighbours and wall-adjacent coarse parents."""
+    case, dom, tris = cases.sphere(48, 24, 24, diameter=7.0, center=(16.0, 12.0, 12.0), segments=(16, 32))
+    case.gridLevelCount = 2
+    cases.add_refine_rule(case, 0, (4, 4, 4, 16, 20, 20))
+    return case, dom, tris
+
+
+def _small_sphere3():
+    return cases.multilevel_sphere(64, 32, 32, level1_box=(12, 8, 8, 36, 24, 24), level2_box=(14, 10, 10, 26, 22, 22),
+                                   center=(20.0, 16.0, 16.0), diameter_fine=12.0, segments=(16, 32))
+
+
+def test_builder_two_level_taylor_green():
+    _check(*cases.taylor_green(16, levels=2))
+
+
+def test_builder_three_level_sphere():
+    o, g = _check(*_small_sphere3())
+    assert o.levels == 3 and len(o.array("f2cIndex", 1)) > 0
+
+
+def test_builder_refinement_cutting_the_sphere():
+    o, g = _check(*_cut_sphere())
+    assert len(o.array("leftoverIndex", 0)) > 0
