@@ -131,6 +131,11 @@ typedef struct lbm3_case {
     lbm3_local_rule localRules[LBM3_MAX_RULES];
     int nRefineRules;
     lbm3_refine_rule refineRules[LBM3_MAX_RULES];
+    /* Reference velocity of the 16-bit population encoding (no counterpart in the
+     * reference): FP16/BF16 stores hold (f - feq(rho = 1, storeRef)) / w, so a flow
+     * near that state keeps the full 16-bit precision for its deviation.  Zero gives
+     * the plain shifted populations (f - w) / w.  Ignored by the FP32/FP64 stores. */
+    float storeRef[3];
 } lbm3_case;
 
 /* Storage precisions of the population array (shifted f - w). */

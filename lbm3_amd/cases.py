@@ -157,6 +157,7 @@ def sphere(nx=256, ny=128, nz=128, diameter=32.0, center=(64.0, 64.0, 64.0), u_i
     add_local_rule(c, wall_id=0, velocity=(0.0, 0.0, 0.0))
     if k15_on_level0:
         add_local_rule(c, level_mask=1, limiter=0.0)
+    c.storeRef[0] = U                     # 16-bit stores: encode the deviation from the free stream
     tri = geometry.uv_sphere(diameter / 2.0, center, segments[0], segments[1])
     return c, bounds(0, 0, 0, nx, ny, nz), [tri]
 

@@ -38,6 +38,19 @@ size_t storage_bytes(int storage)
 }
 size_t compute_bytes(int storage) { return storage == LBM3_STORE_F64 ? 8 : 4; }
 
+// 16-bit encoding: deviation from the shifted equilibrium at rho = 1 and the
+// case's reference velocity, in units of the weight
+Codec make_codec(const lbm3_case &c)
+{
+    Codec k;
+    for (int q = 0; q < 27; ++q) {
+        k.off[q] = feq_shifted<float>(q, 0.f, c.storeRef[0], c.storeRef[1], c.storeRef[2]);
+        k.w[q] = weight<float>(q);
+        k.iw[q] = 1.f / k.w[q];
+    }
+    return k;
+}
+
 static Topology topo_of(const Level &l)
 {
     return Topology{l.N, l.shifter.p, l.ri.p, l.rj.p, l.rk.p, l.rjp.p, l.rkp.p, l.rjkp.p};
@@ -102,6 +115,7 @@ template <class S> static StepArgs<S> step_args(Sim &s, Level &l)
     a.bulk = l.bulk;
     a.wallBC = l.wallBC.p;
     a.geom = l.geom;
+    a.cd = s.codec;
     return a;
 }
 
@@ -122,6 +136,7 @@ template <class S> static OpenArgs<S> open_args(Sim &s, Level &l)
     a.dRhoCum = s.C.trackOpenBoundaries ? l.openCum.p : nullptr;
     a.uNormalCum = s.C.trackOpenBoundaries ? l.openCum.p + (l.nOpen > 0 ? l.nOpen : 1) : nullptr;
     a.iterationsFinished = l.info.iterationsFinished;
+    a.cd = s.codec;
     return a;
 }
 
@@ -196,7 +211,7 @@ template <class S> static int init_level(Sim &s, int level, const float *fieldsD
     d.u0 = s.C.init.u0; d.length = s.C.init.length; d.pressure = s.C.init.pressure;
     d.fields = fieldsDev;
     const bool flip = l.info.esotwistFlipper != 0;
-    launch_init<S>(reinterpret_cast<S *>(l.f.p), topo_of(l), d, l.geom, flip, s.stream);
+    launch_init<S>(reinterpret_cast<S *>(l.f.p), topo_of(l), d, l.geom, s.codec, flip, s.stream);
     launch_open_ref_init<S>(open_args<S>(s, l), l.info.cellCountX, l.info.cellCountY, l.info.cellCountZ, flip, s.stream);
     return check_hip(hipGetLastError(), "init kernels");
 }
@@ -269,6 +284,7 @@ int lbm3_sim_create(const lbm3_case *c, int storage, lbm3_sim **out)
     h->s.C = *c;
     h->s.storage = storage;
     h->s.levels = c->gridLevelCount;
+    h->s.codec = make_codec(*c);
     int rc = check_hip(hipStreamCreateWithFlags(&h->s.stream, hipStreamNonBlocking), "stream");
     if (!rc) rc = check_hip(hipStreamCreateWithFlags(&h->s.aux, hipStreamNonBlocking), "stream");
     if (!rc) rc = check_hip(hipEventCreateWithFlags(&h->s.fork, hipEventDisableTiming), "event");
@@ -536,10 +552,10 @@ int lbm3_download_macros(lbm3_sim *h, int level, void *out)
     const bool flip = l.info.esotwistFlipper != 0;
     const Topology t = topo_of(l);
     switch (s.storage) {
-    case LBM3_STORE_F16: launch_macros<__half>(reinterpret_cast<__half *>(l.f.p), t, reinterpret_cast<float *>(d.p), flip, s.stream); break;
-    case LBM3_STORE_BF16: launch_macros<__hip_bfloat16>(reinterpret_cast<__hip_bfloat16 *>(l.f.p), t, reinterpret_cast<float *>(d.p), flip, s.stream); break;
-    case LBM3_STORE_F64: launch_macros<double>(reinterpret_cast<double *>(l.f.p), t, reinterpret_cast<double *>(d.p), flip, s.stream); break;
-    default: launch_macros<float>(reinterpret_cast<float *>(l.f.p), t, reinterpret_cast<float *>(d.p), flip, s.stream); break;
+    case LBM3_STORE_F16: launch_macros<__half>(reinterpret_cast<__half *>(l.f.p), t, reinterpret_cast<float *>(d.p), s.codec, flip, s.stream); break;
+    case LBM3_STORE_BF16: launch_macros<__hip_bfloat16>(reinterpret_cast<__hip_bfloat16 *>(l.f.p), t, reinterpret_cast<float *>(d.p), s.codec, flip, s.stream); break;
+    case LBM3_STORE_F64: launch_macros<double>(reinterpret_cast<double *>(l.f.p), t, reinterpret_cast<double *>(d.p), s.codec, flip, s.stream); break;
+    default: launch_macros<float>(reinterpret_cast<float *>(l.f.p), t, reinterpret_cast<float *>(d.p), s.codec, flip, s.stream); break;
     }
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(out, d.p, 4 * (size_t)l.N * cb, hipMemcpyDeviceToHost, s.stream));
@@ -628,10 +644,10 @@ int lbm3_kinetic_energy(lbm3_sim *h, double *out)
         const bool flip = l.info.esotwistFlipper != 0;
         const Topology t = topo_of(l);
         switch (s.storage) {
-        case LBM3_STORE_F16: launch_kinetic_energy<__half>(reinterpret_cast<__half *>(l.f.p), t, l.wallMap.p, l.wallData.p, s.partial.p, s.scalar.p, flip, s.stream); break;
-        case LBM3_STORE_BF16: launch_kinetic_energy<__hip_bfloat16>(reinterpret_cast<__hip_bfloat16 *>(l.f.p), t, l.wallMap.p, l.wallData.p, s.partial.p, s.scalar.p, flip, s.stream); break;
-        case LBM3_STORE_F64: launch_kinetic_energy<double>(reinterpret_cast<double *>(l.f.p), t, l.wallMap.p, l.wallData.p, s.partial.p, s.scalar.p, flip, s.stream); break;
-        default: launch_kinetic_energy<float>(reinterpret_cast<float *>(l.f.p), t, l.wallMap.p, l.wallData.p, s.partial.p, s.scalar.p, flip, s.stream); break;
+        case LBM3_STORE_F16: launch_kinetic_energy<__half>(reinterpret_cast<__half *>(l.f.p), t, l.wallMap.p, l.wallData.p, s.partial.p, s.scalar.p, s.codec, flip, s.stream); break;
+        case LBM3_STORE_BF16: launch_kinetic_energy<__hip_bfloat16>(reinterpret_cast<__hip_bfloat16 *>(l.f.p), t, l.wallMap.p, l.wallData.p, s.partial.p, s.scalar.p, s.codec, flip, s.stream); break;
+        case LBM3_STORE_F64: launch_kinetic_energy<double>(reinterpret_cast<double *>(l.f.p), t, l.wallMap.p, l.wallData.p, s.partial.p, s.scalar.p, s.codec, flip, s.stream); break;
+        default: launch_kinetic_energy<float>(reinterpret_cast<float *>(l.f.p), t, l.wallMap.p, l.wallData.p, s.partial.p, s.scalar.p, s.codec, flip, s.stream); break;
         }
         HIPCHK(hipGetLastError());
         double v = 0.0;

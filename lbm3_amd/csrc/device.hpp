@@ -101,6 +101,7 @@ template <class S> struct StepArgs {
     LocalBC bulk;                 // cells that are not next to a wall
     const LocalBC *wallBC;        // [32] by wallID
     LevelGeom geom;
+    Codec cd;                     // 16-bit population encoding
 };
 
 // Open boundary cells of one level, all BCIDs concatenated (BCID order, then cell order)
@@ -115,6 +116,7 @@ template <class S> struct OpenArgs {
     typename StoreTraits<S>::compute *dRhoRef, *uNormalRef;  // reference state, compute precision
     double *dRhoCum, *uNormalCum; // flow accumulators or null
     int iterationsFinished;
+    Codec cd;
 };
 
 #ifndef LBM3_XCD_REMAP

@@ -22,11 +22,11 @@ namespace {
 
 template <class S, bool FLIP>
 __device__ __forceinline__ void load_cell(const S *f, const FastTopo &t, int cell, typename StoreTraits<S>::compute (&v)[27],
-                                          int (&nb)[8])
+                                          int (&nb)[8], const Codec &cd)
 {
     decode_fast(t, cell, t.cellRow[cell].y, nb);
 #pragma unroll
-    for (int q = 0; q < 27; ++q) v[q] = StoreTraits<S>::load(f + (size_t)pre_plane(q, FLIP) * (size_t)t.N + nb[pre_nb(q)]);
+    for (int q = 0; q < 27; ++q) v[q] = StoreTraits<S>::load(f + (size_t)pre_plane(q, FLIP) * (size_t)t.N + nb[pre_nb(q)], q, cd);
 }
 
 // the stencil whose lowest corner is `c0`: self, i+, j+, ij+, k+, ik+, jk+, ijk+
@@ -51,6 +51,7 @@ template <class T> struct IfArgs {
     const LocalBC *wallBCC;        // [32] local BC of the coarse level by wall id
     LevelGeom geomC;
     Topology topoC;
+    Codec cd;
 };
 
 // F->C: the coarse cell at the centre of a 2x2x2 fine stencil
@@ -67,7 +68,7 @@ __global__ __launch_bounds__(256) void k_fine_to_coarse(IfArgs<typename StoreTra
         const int cell = stencil_corner(a.fine, a.child[item], n);
         T v[27];
         int nb[8];
-        load_cell<S, FLIPF>(reinterpret_cast<const S *>(a.ff), a.fine, cell, v, nb);
+        load_cell<S, FLIPF>(reinterpret_cast<const S *>(a.ff), a.fine, cell, v, nb, a.cd);
         lds[threadIdx.x] = cell_values(v, a.w1f);
     }
     __syncthreads();
@@ -94,7 +95,7 @@ __global__ __launch_bounds__(256) void k_fine_to_coarse(IfArgs<typename StoreTra
     // lane n writes populations n, n+8, n+16, n+24
 #pragma unroll
     for (int q = 0; q < 27; ++q)
-        if ((q & 7) == n) StoreTraits<S>::store(fc + (size_t)pre_plane(q, FLIPC) * (size_t)a.coarse.N + nb[pre_nb(q)], f[q]);
+        if ((q & 7) == n) StoreTraits<S>::store(fc + (size_t)pre_plane(q, FLIPC) * (size_t)a.coarse.N + nb[pre_nb(q)], f[q], q, a.cd);
 }
 
 // C->F: the eight fine cells inside a 2x2x2 coarse block, at +-1/4
@@ -111,7 +112,7 @@ __global__ __launch_bounds__(256) void k_coarse_to_fine(IfArgs<typename StoreTra
         const int cell = stencil_corner(a.coarse, a.index[item], n);
         T v[27];
         int nb[8];
-        load_cell<S, FLIPC>(reinterpret_cast<const S *>(a.fc), a.coarse, cell, v, nb);
+        load_cell<S, FLIPC>(reinterpret_cast<const S *>(a.fc), a.coarse, cell, v, nb, a.cd);
         lds[threadIdx.x] = cell_values(v, a.w1c);
     }
     __syncthreads();
@@ -132,7 +133,7 @@ __global__ __launch_bounds__(256) void k_coarse_to_fine(IfArgs<typename StoreTra
     decode_fast(a.fine, cell, a.fine.cellRow[cell].y, nb);
     S *ff = reinterpret_cast<S *>(a.ff);
 #pragma unroll
-    for (int q = 0; q < 27; ++q) StoreTraits<S>::store(ff + (size_t)pre_plane(q, FLIPF) * (size_t)a.fine.N + nb[pre_nb(q)], f[q]);
+    for (int q = 0; q < 27; ++q) StoreTraits<S>::store(ff + (size_t)pre_plane(q, FLIPF) * (size_t)a.fine.N + nb[pre_nb(q)], f[q], q, a.cd);
 }
 
 // fine cells without a full coarse block (updateInterface.h:689-884)
@@ -146,7 +147,7 @@ __global__ __launch_bounds__(256) void k_leftovers(IfArgs<typename StoreTraits<S
     const int centre = a.loParent[x];
     T v[27];
     int nb[8];
-    load_cell<S, FLIPC>(fc, a.coarse, centre, v, nb);
+    load_cell<S, FLIPC>(fc, a.coarse, centre, v, nb, a.cd);
     const CellValues<T> c0 = cell_values(v, a.w1c);
     // wall velocity of the centre cell: the local BC of its wall id (the cell's own velocity if none is set)
     const int wm = a.wallMapC[centre];
@@ -174,7 +175,7 @@ __global__ __launch_bounds__(256) void k_leftovers(IfArgs<typename StoreTraits<S
     for (int k = 0; k < 3; ++k) {
         const bool opposite = nbr[k] < 0;
         const int cell = opposite ? -nbr[k] - 1 : nbr[k];
-        load_cell<S, FLIPC>(fc, a.coarse, cell, v, nb);
+        load_cell<S, FLIPC>(fc, a.coarse, cell, v, nb, a.cd);
         const CellValues<T> c = cell_values(v, a.w1c);
         // a neighbour on the + side adds the linear slope, one on the - side removes it
         const T sgn = opposite ? T(-1) : T(1);
@@ -198,7 +199,7 @@ __global__ __launch_bounds__(256) void k_leftovers(IfArgs<typename StoreTraits<S
     decode_fast(a.fine, cell, a.fine.cellRow[cell].y, nb);
     S *ff = reinterpret_cast<S *>(a.ff);
 #pragma unroll
-    for (int m = 0; m < 27; ++m) StoreTraits<S>::store(ff + (size_t)pre_plane(m, FLIPF) * (size_t)a.fine.N + nb[pre_nb(m)], f[m]);
+    for (int m = 0; m < 27; ++m) StoreTraits<S>::store(ff + (size_t)pre_plane(m, FLIPF) * (size_t)a.fine.N + nb[pre_nb(m)], f[m], m, a.cd);
 }
 
 inline unsigned groups_of_8(long long items) { return (unsigned)((8 * items + 255) / 256); }
@@ -240,6 +241,7 @@ template <class S> void interface_typed(Sim &s, int coarse)
     a.wallBCC = c.wallBC.p;
     a.geomC = c.geom;
     a.topoC = Topology{c.N, c.shifter.p, c.ri.p, c.rj.p, c.rk.p, c.rjp.p, c.rkp.p, c.rjkp.p};
+    a.cd = s.codec;
     const bool fc = c.info.esotwistFlipper != 0, ff = f.info.esotwistFlipper != 0;
     if (fc) { if (ff) launch_all<S, true, true>(a, c, s.stream); else launch_all<S, true, false>(a, c, s.stream); }
     else { if (ff) launch_all<S, false, true>(a, c, s.stream); else launch_all<S, false, false>(a, c, s.stream); }

@@ -82,6 +82,7 @@ struct Sim {
     int levels = 0;
     Level L[LBM3_MAX_LEVELS];
     long long fluidUpdates = 0;
+    Codec codec{};                   // 16-bit population encoding (case storeRef)
     hipStream_t stream = nullptr;    // every call's work is ordered on this stream
     hipStream_t aux = nullptr;       // wall-adjacent kernel beside the bulk kernel, joined back each step
     hipEvent_t fork = nullptr, join = nullptr;

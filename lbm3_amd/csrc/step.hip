@@ -20,17 +20,19 @@
 namespace lbm3 {
 
 template <class S, bool FLIP>
-__device__ __forceinline__ void load_pre(const S *f, int N, const int (&nb)[8], typename StoreTraits<S>::compute (&v)[27])
+__device__ __forceinline__ void load_pre(const S *f, int N, const int (&nb)[8], typename StoreTraits<S>::compute (&v)[27],
+                                         const Codec &cd)
 {
 #pragma unroll
-    for (int q = 0; q < 27; ++q) v[q] = StoreTraits<S>::load(f + (size_t)pre_plane(q, FLIP) * (size_t)N + nb[pre_nb(q)]);
+    for (int q = 0; q < 27; ++q) v[q] = StoreTraits<S>::load(f + (size_t)pre_plane(q, FLIP) * (size_t)N + nb[pre_nb(q)], q, cd);
 }
 
 template <class S, bool FLIP>
-__device__ __forceinline__ void store_pre(S *f, int N, const int (&nb)[8], const typename StoreTraits<S>::compute (&v)[27])
+__device__ __forceinline__ void store_pre(S *f, int N, const int (&nb)[8], const typename StoreTraits<S>::compute (&v)[27],
+                                          const Codec &cd)
 {
 #pragma unroll
-    for (int q = 0; q < 27; ++q) StoreTraits<S>::store(f + (size_t)pre_plane(q, FLIP) * (size_t)N + nb[pre_nb(q)], v[q]);
+    for (int q = 0; q < 27; ++q) StoreTraits<S>::store(f + (size_t)pre_plane(q, FLIP) * (size_t)N + nb[pre_nb(q)], v[q], q, cd);
 }
 
 // interpolated bounceback (Zhao & Yong 2017, single node, l = gamma) on the
@@ -116,10 +118,11 @@ __global__ __launch_bounds__(256) void k_collide_stream(StepArgs<S> a)
     for (int k = 0; k < 8; ++k) off[k] = uint32_t(nb[k]) * uint32_t(sizeof(S));
     T f[27];
 #pragma unroll
-    for (int q = 0; q < 27; ++q) f[q] = StoreTraits<S>::load(at(a.f + (size_t)pre_plane(q, FLIP) * (size_t)N, off[pre_nb(q)]));
+    for (int q = 0; q < 27; ++q) f[q] = StoreTraits<S>::load(at(a.f + (size_t)pre_plane(q, FLIP) * (size_t)N, off[pre_nb(q)]), q, a.cd);
     collide<T, BULK>(f, relax_of<T>(a.bulk));
 #pragma unroll
-    for (int q = 0; q < 27; ++q) StoreTraits<S>::store(at(a.f + (size_t)post_plane(q, FLIP) * (size_t)N, off[post_nb(q)]), f[q]);
+    for (int q = 0; q < 27; ++q)
+        StoreTraits<S>::store(at(a.f + (size_t)post_plane(q, FLIP) * (size_t)N, off[post_nb(q)]), f[q], q, a.cd);
 }
 
 // wall-adjacent cells: collision with the wall id's local BC, interpolated
@@ -136,7 +139,7 @@ __global__ __launch_bounds__(256) void k_collide_wall(StepArgs<S> a)
     int nb[8];
     decode_fast(a.fast, cell, row, nb);
     T f[27];
-    load_pre<S, FLIP>(a.f, N, nb, f);
+    load_pre<S, FLIP>(a.f, N, nb, f, a.cd);
     const uint32_t wd = a.wallData[wm];
     uint32_t codes[7];
 #pragma unroll
@@ -166,12 +169,13 @@ __global__ __launch_bounds__(256) void k_collide_wall(StepArgs<S> a)
     collide<T, 0>(f, R);
     double g3[3] = {0.0, 0.0, 0.0};
     bounce_back<T>(f, pre, wd, b.useIBB != 0, codes, wu, g3);
+    // a reflected population is read next step as the opposite direction
 #pragma unroll
     for (int d = 0; d < 27; ++d) {
-        S *dst;
-        if (d != 0 && ((wd >> d) & 1u)) dst = a.f + (size_t)pre_plane(inv(d), !FLIP) * (size_t)N + nb[pre_nb(inv(d))];
-        else dst = a.f + (size_t)post_plane(d, FLIP) * (size_t)N + nb[post_nb(d)];
-        StoreTraits<S>::store(dst, f[d]);
+        if (d != 0 && ((wd >> d) & 1u))
+            StoreTraits<S>::store(a.f + (size_t)pre_plane(inv(d), !FLIP) * (size_t)N + nb[pre_nb(inv(d))], f[d], inv(d), a.cd);
+        else
+            StoreTraits<S>::store(a.f + (size_t)post_plane(d, FLIP) * (size_t)N + nb[post_nb(d)], f[d], d, a.cd);
     }
     if (a.gWall && !(wd & 1u)) {
         a.gWall[wm] += g3[0];
@@ -212,7 +216,7 @@ __device__ __forceinline__ void open_one(const OpenArgs<S> &a, int x, const int 
 #pragma unroll
     for (int q = 0; q < 27; ++q) {
         if (unknown_dir(nx, ny, nz, q)) { f[q] = T(0); continue; }
-        f[q] = StoreTraits<S>::load(a.f + (size_t)pre_plane(q, FLIP) * (size_t)N + nb[pre_nb(q)]);
+        f[q] = StoreTraits<S>::load(a.f + (size_t)pre_plane(q, FLIP) * (size_t)N + nb[pre_nb(q)], q, a.cd);
     }
     const OpenBCRecord R = a.records[(meta >> 5) & 255u];
     const T dRef = a.dRhoRef[x], uRef = a.uNormalRef[x];
@@ -221,7 +225,7 @@ __device__ __forceinline__ void open_one(const OpenArgs<S> &a, int x, const int 
 #pragma unroll
     for (int q = 0; q < 27; ++q)
         if (unknown_dir(nx, ny, nz, q))
-            StoreTraits<S>::store(a.f + (size_t)pre_plane(q, FLIP) * (size_t)N + nb[pre_nb(q)], f[q]);
+            StoreTraits<S>::store(a.f + (size_t)pre_plane(q, FLIP) * (size_t)N + nb[pre_nb(q)], f[q], q, a.cd);
     // exponential moving average of the reference state
     const T beta = T(R.beta);
     T alpha = beta / (T(a.iterationsFinished) + beta + T(10));
@@ -256,22 +260,24 @@ __global__ __launch_bounds__(256) void k_open(OpenArgs<S> a)
     }
 }
 
+// the stored values are copied as they are: source and ghost read them as the same directions
 template <class S, bool FLIP>
 __global__ __launch_bounds__(256) void k_periodic(S *f, Topology topo, int count, const int *index, const int *source)
 {
     const int x = blockIdx.x * 256 + threadIdx.x;
     if (x >= count) return;
-    using T = typename StoreTraits<S>::compute;
-    int nb[8];
-    T v[27];
-    decode(topo, source[x], nb, nullptr);
-    load_pre<S, FLIP>(f, topo.N, nb, v);
-    decode(topo, index[x], nb, nullptr);
-    store_pre<S, FLIP>(f, topo.N, nb, v);
+    int ns[8], nd[8];
+    decode(topo, source[x], ns, nullptr);
+    decode(topo, index[x], nd, nullptr);
+    S v[27];
+#pragma unroll
+    for (int q = 0; q < 27; ++q) v[q] = f[(size_t)pre_plane(q, FLIP) * (size_t)topo.N + ns[pre_nb(q)]];
+#pragma unroll
+    for (int q = 0; q < 27; ++q) f[(size_t)pre_plane(q, FLIP) * (size_t)topo.N + nd[pre_nb(q)]] = v[q];
 }
 
 template <class S, bool FLIP>
-__global__ __launch_bounds__(256) void k_init(S *f, Topology topo, InitDesc d, LevelGeom g)
+__global__ __launch_bounds__(256) void k_init(S *f, Topology topo, InitDesc d, LevelGeom g, Codec cd)
 {
     using T = typename StoreTraits<S>::compute;
     const int cell = blockIdx.x * 256 + threadIdx.x;
@@ -294,7 +300,7 @@ __global__ __launch_bounds__(256) void k_init(S *f, Topology topo, InitDesc d, L
     T v[27];
 #pragma unroll
     for (int q = 0; q < 27; ++q) v[q] = feq_shifted<T>(q, T(dRho), T(ux), T(uy), T(uz));
-    store_pre<S, FLIP>(f, topo.N, nb, v);
+    store_pre<S, FLIP>(f, topo.N, nb, v, cd);
 }
 
 template <class S, bool FLIP>
@@ -306,7 +312,7 @@ __global__ __launch_bounds__(256) void k_open_ref_init(OpenArgs<S> a, int cx_, i
     int nb[8], ijk[3];
     decode(a.topo, a.cell[x], nb, ijk);
     T v[27];
-    load_pre<S, FLIP>(a.f, a.topo.N, nb, v);
+    load_pre<S, FLIP>(a.f, a.topo.N, nb, v, a.cd);
     T d, ux, uy, uz;
     moments(v, d, ux, uy, uz);
     const int nx = ijk[0] == 0 ? -1 : (ijk[0] == cx_ - 1 ? 1 : 0);
@@ -317,7 +323,7 @@ __global__ __launch_bounds__(256) void k_open_ref_init(OpenArgs<S> a, int cx_, i
 }
 
 template <class S, bool FLIP>
-__global__ __launch_bounds__(256) void k_macros(const S *f, Topology topo, typename StoreTraits<S>::compute *out)
+__global__ __launch_bounds__(256) void k_macros(const S *f, Topology topo, typename StoreTraits<S>::compute *out, Codec cd)
 {
     using T = typename StoreTraits<S>::compute;
     const int cell = blockIdx.x * 256 + threadIdx.x;
@@ -325,7 +331,7 @@ __global__ __launch_bounds__(256) void k_macros(const S *f, Topology topo, typen
     int nb[8];
     decode(topo, cell, nb, nullptr);
     T v[27];
-    load_pre<S, FLIP>(f, topo.N, nb, v);
+    load_pre<S, FLIP>(f, topo.N, nb, v, cd);
     T d, ux, uy, uz;
     moments(v, d, ux, uy, uz);
     out[4 * (size_t)cell] = d;
@@ -349,7 +355,7 @@ __device__ __forceinline__ double block_sum(double v, double *lds)
 // per-block partial sums of the kinetic energy 0.5 (1 + dRho) |u|^2 over counted cells
 template <class S, bool FLIP>
 __global__ __launch_bounds__(256) void k_kinetic_energy(const S *f, Topology topo, const int *wallMap, const uint32_t *wallData,
-                                                        double *partial)
+                                                        double *partial, Codec cd)
 {
     __shared__ double lds[4];
     using T = typename StoreTraits<S>::compute;
@@ -362,7 +368,7 @@ __global__ __launch_bounds__(256) void k_kinetic_energy(const S *f, Topology top
             int nb[8];
             decode(topo, cell, nb, nullptr);
             T v[27];
-            load_pre<S, FLIP>(f, topo.N, nb, v);
+            load_pre<S, FLIP>(f, topo.N, nb, v, cd);
             T d, ux, uy, uz;
             moments(v, d, ux, uy, uz);
             e = 0.5 * (1.0 + double(d)) * (double(ux) * double(ux) + double(uy) * double(uy) + double(uz) * double(uz));
@@ -494,11 +500,11 @@ void launch_periodic(S *f, const Topology &t, int count, const int *index, const
 }
 
 template <class S>
-void launch_init(S *f, const Topology &t, const InitDesc &d, const LevelGeom &g, bool flip, hipStream_t s)
+void launch_init(S *f, const Topology &t, const InitDesc &d, const LevelGeom &g, const Codec &cd, bool flip, hipStream_t s)
 {
     if (t.N <= 0) return;
-    if (flip) hipLaunchKernelGGL((k_init<S, true>), dim3(blocks_for(t.N)), dim3(256), 0, s, f, t, d, g);
-    else hipLaunchKernelGGL((k_init<S, false>), dim3(blocks_for(t.N)), dim3(256), 0, s, f, t, d, g);
+    if (flip) hipLaunchKernelGGL((k_init<S, true>), dim3(blocks_for(t.N)), dim3(256), 0, s, f, t, d, g, cd);
+    else hipLaunchKernelGGL((k_init<S, false>), dim3(blocks_for(t.N)), dim3(256), 0, s, f, t, d, g, cd);
 }
 
 template <class S>
@@ -510,20 +516,21 @@ void launch_open_ref_init(const OpenArgs<S> &a, int X, int Y, int Z, bool flip, 
 }
 
 template <class S>
-void launch_macros(const S *f, const Topology &t, typename StoreTraits<S>::compute *out, bool flip, hipStream_t s)
+void launch_macros(const S *f, const Topology &t, typename StoreTraits<S>::compute *out, const Codec &cd, bool flip,
+                   hipStream_t s)
 {
     if (t.N <= 0) return;
-    if (flip) hipLaunchKernelGGL((k_macros<S, true>), dim3(blocks_for(t.N)), dim3(256), 0, s, f, t, out);
-    else hipLaunchKernelGGL((k_macros<S, false>), dim3(blocks_for(t.N)), dim3(256), 0, s, f, t, out);
+    if (flip) hipLaunchKernelGGL((k_macros<S, true>), dim3(blocks_for(t.N)), dim3(256), 0, s, f, t, out, cd);
+    else hipLaunchKernelGGL((k_macros<S, false>), dim3(blocks_for(t.N)), dim3(256), 0, s, f, t, out, cd);
 }
 
 template <class S>
 int launch_kinetic_energy(const S *f, const Topology &t, const int *wallMap, const uint32_t *wallData, double *partial,
-                          double *out, bool flip, hipStream_t s)
+                          double *out, const Codec &cd, bool flip, hipStream_t s)
 {
     const unsigned nb = blocks_for(t.N);
-    if (flip) hipLaunchKernelGGL((k_kinetic_energy<S, true>), dim3(nb), dim3(256), 0, s, f, t, wallMap, wallData, partial);
-    else hipLaunchKernelGGL((k_kinetic_energy<S, false>), dim3(nb), dim3(256), 0, s, f, t, wallMap, wallData, partial);
+    if (flip) hipLaunchKernelGGL((k_kinetic_energy<S, true>), dim3(nb), dim3(256), 0, s, f, t, wallMap, wallData, partial, cd);
+    else hipLaunchKernelGGL((k_kinetic_energy<S, false>), dim3(nb), dim3(256), 0, s, f, t, wallMap, wallData, partial, cd);
     hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, partial, (int)nb, out);
     return (int)nb;
 }
@@ -548,11 +555,13 @@ void launch_collide_batch(T *f, int n, const Relax<T> &R, hipStream_t s)
     template void launch_collide_wall<S>(const StepArgs<S> &, bool, hipStream_t);                                   \
     template void launch_open<S>(const OpenArgs<S> &, bool, hipStream_t);                                           \
     template void launch_periodic<S>(S *, const Topology &, int, const int *, const int *, bool, hipStream_t);        \
-    template void launch_init<S>(S *, const Topology &, const InitDesc &, const LevelGeom &, bool, hipStream_t);     \
+    template void launch_init<S>(S *, const Topology &, const InitDesc &, const LevelGeom &, const Codec &, bool,     \
+                                 hipStream_t);                                                                    \
     template void launch_open_ref_init<S>(const OpenArgs<S> &, int, int, int, bool, hipStream_t);                   \
-    template void launch_macros<S>(const S *, const Topology &, typename StoreTraits<S>::compute *, bool, hipStream_t); \
+    template void launch_macros<S>(const S *, const Topology &, typename StoreTraits<S>::compute *, const Codec &,    \
+                                   bool, hipStream_t);                                                            \
     template int launch_kinetic_energy<S>(const S *, const Topology &, const int *, const uint32_t *, double *,       \
-                                          double *, bool, hipStream_t);
+                                          double *, const Codec &, bool, hipStream_t);
 LBM3_INSTANTIATE(float)
 LBM3_INSTANTIATE(double)
 LBM3_INSTANTIATE(__half)

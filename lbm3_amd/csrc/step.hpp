@@ -19,11 +19,13 @@ template <class S> void launch_collide_stream(const StepArgs<S> &a, bool flip, h
 template <class S> void launch_collide_wall(const StepArgs<S> &a, bool flip, hipStream_t s);
 template <class S> void launch_open(const OpenArgs<S> &a, bool flip, hipStream_t s);
 template <class S> void launch_periodic(S *f, const Topology &t, int count, const int *index, const int *source, bool flip, hipStream_t s);
-template <class S> void launch_init(S *f, const Topology &t, const InitDesc &d, const LevelGeom &g, bool flip, hipStream_t s);
+template <class S> void launch_init(S *f, const Topology &t, const InitDesc &d, const LevelGeom &g, const Codec &cd, bool flip,
+                                    hipStream_t s);
 template <class S> void launch_open_ref_init(const OpenArgs<S> &a, int X, int Y, int Z, bool flip, hipStream_t s);
-template <class S> void launch_macros(const S *f, const Topology &t, typename StoreTraits<S>::compute *out, bool flip, hipStream_t s);
+template <class S> void launch_macros(const S *f, const Topology &t, typename StoreTraits<S>::compute *out, const Codec &cd,
+                                      bool flip, hipStream_t s);
 template <class S> int launch_kinetic_energy(const S *f, const Topology &t, const int *wallMap, const uint32_t *wallData,
-                                             double *partial, double *out, bool flip, hipStream_t s);
+                                             double *partial, double *out, const Codec &cd, bool flip, hipStream_t s);
 void launch_wall_force(const double *gWall, int W, const uint32_t *wallData, const int *wallIndex, const Topology &t,
                        const LevelGeom &g, const double origin[3], int nBodies, double *partial, hipStream_t s);
 template <class T> void launch_collide_batch(T *f, int n, const Relax<T> &R, hipStream_t s);

@@ -39,44 +39,54 @@ template <class V> __device__ __forceinline__ void st(V *p, V v)
 #endif
 }
 
+// Encoding of the 16-bit stores: the stored value of direction q is
+// (f_q - w_q - off_q) / w_q, i.e. the deviation of the shifted population from
+// the shifted equilibrium of a reference velocity, in units of the weight.
+// The 32/64-bit stores hold f_q - w_q and ignore it.
+struct Codec {
+    float off[27];
+    float w[27];
+    float iw[27];
+};
+
 template <class S> struct StoreTraits;
 
 template <> struct StoreTraits<float> {
     using compute = float;
-    static __device__ __forceinline__ float load(const float *p) { return ld(p); }
-    static __device__ __forceinline__ void store(float *p, float v) { st(p, v); }
+    static __device__ __forceinline__ float load(const float *p, int, const Codec &) { return ld(p); }
+    static __device__ __forceinline__ void store(float *p, float v, int, const Codec &) { st(p, v); }
 };
 
 template <> struct StoreTraits<double> {
     using compute = double;
-    static __device__ __forceinline__ double load(const double *p) { return *p; }
-    static __device__ __forceinline__ void store(double *p, double v) { *p = v; }
+    static __device__ __forceinline__ double load(const double *p, int, const Codec &) { return *p; }
+    static __device__ __forceinline__ void store(double *p, double v, int, const Codec &) { *p = v; }
 };
 
 template <> struct StoreTraits<__half> {
     using compute = float;
-    static __device__ __forceinline__ float load(const __half *p)
+    static __device__ __forceinline__ float load(const __half *p, int q, const Codec &c)
     {
         const unsigned short bits = ld(reinterpret_cast<const unsigned short *>(p));
-        return __half2float(__ushort_as_half(bits));
+        return fmaf(__half2float(__ushort_as_half(bits)), c.w[q], c.off[q]);
     }
-    static __device__ __forceinline__ void store(__half *p, float v)
+    static __device__ __forceinline__ void store(__half *p, float v, int q, const Codec &c)
     {
-        st(reinterpret_cast<unsigned short *>(p), __half_as_ushort(__float2half_rn(v)));
+        st(reinterpret_cast<unsigned short *>(p), __half_as_ushort(__float2half_rn((v - c.off[q]) * c.iw[q])));
     }
 };
 
 template <> struct StoreTraits<__hip_bfloat16> {
     using compute = float;
-    static __device__ __forceinline__ float load(const __hip_bfloat16 *p)
+    static __device__ __forceinline__ float load(const __hip_bfloat16 *p, int q, const Codec &c)
     {
         const unsigned short bits = ld(reinterpret_cast<const unsigned short *>(p));
-        return __uint_as_float(static_cast<unsigned int>(bits) << 16);
+        return fmaf(__uint_as_float(static_cast<unsigned int>(bits) << 16), c.w[q], c.off[q]);
     }
-    static __device__ __forceinline__ void store(__hip_bfloat16 *p, float v)
+    static __device__ __forceinline__ void store(__hip_bfloat16 *p, float v, int q, const Codec &c)
     {
         // round to nearest even on the upper 16 bits (finite values)
-        unsigned int u = __float_as_uint(v);
+        unsigned int u = __float_as_uint((v - c.off[q]) * c.iw[q]);
         u += 0x7FFFu + ((u >> 16) & 1u);
         st(reinterpret_cast<unsigned short *>(p), static_cast<unsigned short>(u >> 16));
     }

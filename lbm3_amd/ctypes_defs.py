@@ -71,4 +71,5 @@ class Case(C.Structure):
                 ("init", Init),
                 ("nOpenRules", C.c_int), ("openRules", OpenRule * MAX_RULES), ("openDefault", BC),
                 ("nLocalRules", C.c_int), ("localRules", LocalRule * MAX_RULES),
-                ("nRefineRules", C.c_int), ("refineRules", RefineRule * MAX_RULES)]
+                ("nRefineRules", C.c_int), ("refineRules", RefineRule * MAX_RULES),
+                ("storeRef", C.c_float * 3)]

@@ -242,3 +242,26 @@ def test_multilevel_f32(name):
     sens = max(rel_max(o.macros(L)[:, 1:], o64.macros(L)[:, 1:]) for L in range(o.levels))
     print("%s FP32 after 50 coarse steps: rel max rho %.2e, u %.2e (oracle FP32 vs FP64 %.2e)" % (name, r, u, sens))
     assert r < 1e-5 and u < max(1e-5, 3 * sens + 1e-6), (r, u, sens)
+
+
+# ---------------------------------------------------------------- 16-bit storage accuracy
+@pytest.mark.parametrize("storage,bar", [("f16", 1e-3), ("bf16", 1e-2)])
+def test_16bit_storage_velocity_error_vs_fp64(storage, bar):
+    """Shifted 16-bit populations (deviation from the free-stream equilibrium,
+    in units of the weight): relative L2 velocity error against the FP64 run of
+    the same kernels after 1000 steps of the sphere case."""
+    case, dom, tris = _small_sphere()
+    runs = {}
+    for st in (storage, "f64"):
+        g = lib.Sim(case, st)
+        g.build(dom, tris)
+        g.init()
+        g.run(1000)
+        g.sync()
+        runs[st] = g.macros()[:, 1:].astype(np.float64)
+        fluid = g.array("wallMap") != -3      # solid cells hold no flow
+        g.close()
+    a, b = runs[storage][fluid], runs["f64"][fluid]
+    err = float(np.linalg.norm(a - b) / np.linalg.norm(b))
+    print("%s sphere 64x32x32, 1000 steps: rel L2 u error vs FP64 %.2e" % (storage, err))
+    assert err < bar, err
