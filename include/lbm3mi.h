@@ -97,7 +97,8 @@ int lbm3_synchronize(lbm3_sim *s);
  * out[0] = total ms; with kernelEvents != 0 also out[1] = summed ms of the
  * level-0 bulk collide-stream kernel (events around each launch), out[2] = the
  * number of those launches and out[3] = summed ms of the level-0 wall-adjacent
- * collide kernel.  out holds 4 doubles.  No counterpart in the reference
+ * and open-cell kernels.  With kernelEvents the kernels run in order on one
+ * stream (no overlap).  out holds 4 doubles.  No counterpart in the reference
  * (TNL::Timer). */
 int lbm3_run_timed(lbm3_sim *s, int steps, int kernelEvents, double *out);
 
@@ -112,7 +113,10 @@ int lbm3_download_populations(lbm3_sim *s, int level, void *out, long long bytes
 int lbm3_upload_populations(lbm3_sim *s, int level, const void *in, long long bytes);
 /* Named built arrays (shifter, rowI.., wallMap, wallIndex, wallData, linkLength,
  * periodicIndex, periodicSource, openIndex, openSource, openMeta, rayHitScan,
- * rayMap, f2cIndex, ...).  With out == NULL only *count is returned. */
+ * rayMap, f2cIndex, ...).  "cellRow" is the per-step kernels' cell record, 8
+ * bytes per cell: {kind, row} with kind >= 0 wall-adjacent entry, -1/-2 bulk,
+ * -3 solid, <= -4 open cell updated by its own kernel.  With out == NULL only
+ * *count is returned. */
 int lbm3_download_array(lbm3_sim *s, int level, const char *name, void *out, long long capacity, long long *count);
 
 /* FP64 observables. */
@@ -128,6 +132,34 @@ int lbm3_reset_wall_forces(lbm3_sim *s);
  * populations each (cell-major), on the device. */
 int lbm3_collide_cells_f32(float *f, int n, const lbm3_bc *bc, float nu);
 int lbm3_collide_cells_f64(double *f, int n, const lbm3_bc *bc, double nu);
+
+/* Continuous-batching server of single-level simulations.  No counterpart in
+ * the reference (its batch runner, run_intakes_automated.py, solves cases one
+ * after another); for serving many independent cases on one GPU.
+ *
+ * submit queues a request (its arrival time is the call time); each
+ * lbm3_server_step admits queued requests in arrival order while at most
+ * maxActive run and their estimated device bytes fit hbmBudgetBytes (building
+ * their grids on the GPU), advances every active request by one time step in
+ * one batched launch sequence, and retires those that reached their step
+ * count: their ids are written to completed[0..cap) and *nCompleted counts
+ * them.  Requests that fail to build complete immediately with an error status.
+ * lbm3_server_result: out[0] queue wait s, out[1] time to solution s (submit to
+ * finish), out[2] fluid updates, out[3] final kinetic energy, out[4..9] force
+ * and torque of wall body 0 about the origin (FP64), out[10] status (0 or an
+ * LBM3_E_* code), out[11] steps; n >= 12.  lbm3_server_macros: (dRho, ux, uy,
+ * uz) per cell at the end, for requests submitted with keepMacros.
+ * lbm3_server_stats: out[0] active, out[1] queued, out[2] device bytes held,
+ * out[3] batched steps run, out[4] fluid updates run (5 doubles). */
+typedef struct lbm3_server lbm3_server;
+int lbm3_server_create(long long hbmBudgetBytes, int maxActive, lbm3_server **out);
+int lbm3_server_destroy(lbm3_server *s);
+int lbm3_server_submit(lbm3_server *s, const lbm3_case *c, int storage, const float *triangles, const int *triCounts,
+                       int nBodies, lbm3_bounds domain, int steps, int keepMacros, long long *id);
+int lbm3_server_step(lbm3_server *s, long long *completed, int cap, int *nCompleted);
+int lbm3_server_result(lbm3_server *s, long long id, double *out, int n);
+int lbm3_server_macros(lbm3_server *s, long long id, void *out, long long bytes);
+int lbm3_server_stats(lbm3_server *s, double *out);
 
 /* Plain device-to-device streaming copy of `bytes` (read + write), timed over
  * `reps` launches: the achievable-bandwidth probe and the reference kernel for

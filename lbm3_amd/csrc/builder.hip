@@ -938,7 +938,6 @@ int walls_and_grid(Sim &sim, std::vector<BLevel> &G, std::vector<DevSTL> &stl, i
     for (int L = 0; L < levels; ++L) {
         BLevel &B = G[L];
         Level &l = sim.L[L];
-        const bool finest = L == levels - 1;
         const int N = B.N, W = B.W;
         const Views v = views(B);
         l.info = B.I;

@@ -5,6 +5,7 @@
 #include <mutex>
 #include <new>
 #include "sim.hpp"
+#include "levelargs.hpp"
 
 namespace lbm3 {
 
@@ -51,10 +52,6 @@ Codec make_codec(const lbm3_case &c)
     return k;
 }
 
-static Topology topo_of(const Level &l)
-{
-    return Topology{l.N, l.shifter.p, l.ri.p, l.rj.p, l.rk.p, l.rjp.p, l.rkp.p, l.rjkp.p};
-}
 
 template <class T> static int upload(DBuf<T> &b, const T *host, long long n, hipStream_t s)
 {
@@ -81,7 +78,10 @@ int finish_level_setup(Sim &s, int level)
     if (upload(l.wallBC, wb, 32, s.stream)) return LBM3_E_HIP;
     HIPCHK(l.cellRow.alloc(l.N > 0 ? l.N : 1));
     HIPCHK(l.rowTab.alloc(l.nRows > 0 ? l.nRows : 1));
-    launch_fast_topo(topo_of(l), l.wallMap.p, l.cellRow.p, l.rowTab.p, s.stream);
+    launch_fast_topo(topo_of(l), l.wallMap.p, nullptr, l.cellRow.p, l.rowTab.p, s.stream);
+    l.openFusionChecked = false;
+    l.fusedOpen = false;
+    l.openSettled = true;
     HIPCHK(l.openNb.alloc(2LL * (l.nOpen > 0 ? l.nOpen : 1)));
     launch_open_nb(topo_of(l), l.nOpen, l.openCell.p, l.openNb.p, s.stream);
     HIPCHK(hipGetLastError());
@@ -99,45 +99,66 @@ int finish_level_setup(Sim &s, int level)
     return 0;
 }
 
-template <class S> static StepArgs<S> step_args(Sim &s, Level &l)
+
+
+// Decide whether level L defers its open boundary condition to the next step's
+// open-cell kernel (which then loads, completes, collides and streams those
+// cells, beside the bulk kernel): only level 0, and only when no open cell is
+// wall-adjacent and no periodic copy or interface transfer touches an open cell
+// (those passes read or write whole cells and need the condition applied in the
+// same step).
+static int prepare_open_fusion(Sim &s, int L)
 {
-    StepArgs<S> a;
-    a.f = reinterpret_cast<S *>(l.f.p);
-    a.topo = topo_of(l);
-    a.fast = FastTopo{l.N, l.cellRow.p, l.rowTab.p};
-    a.wallMap = l.wallMap.p;
-    a.wallIndex = l.wallIndex.p;
-    a.wallData = l.wallData.p;
-    a.linkLength = l.linkLength.p;
-    a.W = l.W;
-    a.gWall = s.C.trackWallForce ? l.gWall.p : nullptr;
-    a.nu = l.info.nu;
-    a.bulk = l.bulk;
-    a.wallBC = l.wallBC.p;
-    a.geom = l.geom;
-    a.cd = s.codec;
-    return a;
+    Level &l = s.L[L];
+    l.openFusionChecked = true;
+    l.fusedOpen = false;
+    l.openSettled = true;
+    if (L != 0 || l.nOpen <= 0) return 0;
+    DBuf<int> openOf, flag;
+    HIPCHK(openOf.alloc(l.N));
+    HIPCHK(flag.alloc(1));
+    HIPCHK(hipMemsetAsync(openOf.p, 0xFF, sizeof(int) * (size_t)l.N, s.stream));
+    HIPCHK(hipMemsetAsync(flag.p, 0, sizeof(int), s.stream));
+    launch_scatter_index(l.openCell.p, l.nOpen, openOf.p, s.stream);
+    const FastTopo t{l.N, l.cellRow.p, l.rowTab.p};
+    launch_open_conflicts(l.openCell.p, l.nOpen, 3, openOf.p, l.wallMap.p, t, flag.p, s.stream);
+    launch_open_conflicts(l.perIndex.p, l.nPeriodic, 0, openOf.p, l.wallMap.p, t, flag.p, s.stream);
+    launch_open_conflicts(l.perSource.p, l.nPeriodic, 0, openOf.p, l.wallMap.p, t, flag.p, s.stream);
+    launch_open_conflicts(l.F2C.index.p, l.F2C.interfaceCount, 0, openOf.p, l.wallMap.p, t, flag.p, s.stream);
+    launch_open_conflicts(l.C2F.index.p, l.C2F.interfaceCount, 2, openOf.p, l.wallMap.p, t, flag.p, s.stream);
+    launch_open_conflicts(l.C2F.leftoverParent.p, l.C2F.leftoverCount, 0, openOf.p, l.wallMap.p, t, flag.p, s.stream);
+    launch_open_conflicts(l.C2F.leftoverNbrI.p, l.C2F.leftoverCount, 1, openOf.p, l.wallMap.p, t, flag.p, s.stream);
+    launch_open_conflicts(l.C2F.leftoverNbrJ.p, l.C2F.leftoverCount, 1, openOf.p, l.wallMap.p, t, flag.p, s.stream);
+    launch_open_conflicts(l.C2F.leftoverNbrK.p, l.C2F.leftoverCount, 1, openOf.p, l.wallMap.p, t, flag.p, s.stream);
+    int conflicts = 0;
+    HIPCHK(hipMemcpyAsync(&conflicts, flag.p, sizeof(int), hipMemcpyDeviceToHost, s.stream));
+    HIPCHK(hipStreamSynchronize(s.stream));
+    if (conflicts) return 0;
+    launch_fast_topo(topo_of(l), l.wallMap.p, openOf.p, l.cellRow.p, l.rowTab.p, s.stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(s.stream));
+    l.fusedOpen = true;
+    return 0;
 }
 
-template <class S> static OpenArgs<S> open_args(Sim &s, Level &l)
+// apply a pending open boundary condition in place (before anything reads the level's state)
+template <class S> static void settle_typed(Sim &s, Level &l)
 {
-    using T = typename StoreTraits<S>::compute;
-    OpenArgs<S> a;
-    a.f = reinterpret_cast<S *>(l.f.p);
-    a.topo = topo_of(l);
-    a.count = l.nOpen;
-    a.cell = l.openCell.p;
-    a.nb = l.openNb.p;
-    a.meta = l.openMeta.p;
-    a.records = l.openRecords.p;
-    T *refs = reinterpret_cast<T *>(l.openRefs.p);
-    a.dRhoRef = refs;
-    a.uNormalRef = refs + (l.nOpen > 0 ? l.nOpen : 1);
-    a.dRhoCum = s.C.trackOpenBoundaries ? l.openCum.p : nullptr;
-    a.uNormalCum = s.C.trackOpenBoundaries ? l.openCum.p + (l.nOpen > 0 ? l.nOpen : 1) : nullptr;
-    a.iterationsFinished = l.info.iterationsFinished;
-    a.cd = s.codec;
-    return a;
+    launch_open<S>(open_args<S>(s, l, l.info.iterationsFinished - 1), l.info.esotwistFlipper != 0, s.stream);
+}
+
+static int settle_open(Sim &s, int L)
+{
+    Level &l = s.L[L];
+    if (!l.fusedOpen || l.openSettled) return 0;
+    switch (s.storage) {
+    case LBM3_STORE_F16: settle_typed<__half>(s, l); break;
+    case LBM3_STORE_BF16: settle_typed<__hip_bfloat16>(s, l); break;
+    case LBM3_STORE_F64: settle_typed<double>(s, l); break;
+    default: settle_typed<float>(s, l); break;
+    }
+    l.openSettled = true;
+    return check_hip(hipGetLastError(), "open boundary settle");
 }
 
 template <class S> static void step_level(Sim &s, int level)
@@ -153,29 +174,40 @@ template <class S> static void step_level(Sim &s, int level)
         }
         (void)hipEventRecord(s.ev[s.evUsed][0], s.stream);
     }
-    const StepArgs<S> a = step_args<S>(s, l);
-    if (timed || l.W == 0) {
+    StepArgs<S> a = step_args<S>(s, l);
+    a.open = FusedOpen<typename StoreTraits<S>::compute>{};
+    if (l.fusedOpen) {
+        // open cells have their own kernel, which first applies the previous
+        // step's boundary condition when it is still pending
+        const OpenArgs<S> o = open_args<S>(s, l, l.info.iterationsFinished - 1);
+        a.open = {o.cell, o.count, l.openSettled ? 0 : 1, o.meta, o.records, o.dRhoRef, o.uNormalRef, o.dRhoCum,
+                  o.uNormalCum, o.iterationsFinished};
+    }
+    if (timed || (l.W == 0 && a.open.count == 0)) {
         // in order on one stream, so that each kernel's events bracket it alone
         launch_collide_stream<S>(a, flip, s.stream);
         if (timed) (void)hipEventRecord(s.ev[s.evUsed][1], s.stream);
         launch_collide_wall<S>(a, flip, s.stream);
+        launch_open_cells<S>(a, flip, s.stream);
         if (timed) {
             (void)hipEventRecord(s.ev[s.evUsed][2], s.stream);
             s.evUsed++;
         }
     } else {
-        // the bulk and wall-adjacent kernels touch disjoint cells: run the small,
-        // latency-bound wall kernel on the second stream beside the bulk kernel
+        // the kernels touch disjoint cells: run the small, latency-bound wall and
+        // open-cell kernels on the second stream beside the bulk kernel
         (void)hipEventRecord(s.fork, s.stream);
         (void)hipStreamWaitEvent(s.aux, s.fork, 0);
         launch_collide_wall<S>(a, flip, s.aux);
+        launch_open_cells<S>(a, flip, s.aux);
         (void)hipEventRecord(s.join, s.aux);
         launch_collide_stream<S>(a, flip, s.stream);
         (void)hipStreamWaitEvent(s.stream, s.join, 0);
     }
     const bool nflip = !flip;
     l.info.esotwistFlipper = nflip ? 1 : 0;
-    launch_open<S>(open_args<S>(s, l), nflip, s.stream);
+    if (l.fusedOpen) l.openSettled = false;   // this step's boundary condition is applied by the next step's open-cell kernel
+    else launch_open<S>(open_args<S>(s, l, l.info.iterationsFinished), nflip, s.stream);
     launch_periodic<S>(reinterpret_cast<S *>(l.f.p), topo_of(l), l.nPeriodic, l.perIndex.p, l.perSource.p, nflip, s.stream);
     l.info.iterationsSinceTrackerReset++;
     l.info.iterationsFinished++;
@@ -211,8 +243,9 @@ template <class S> static int init_level(Sim &s, int level, const float *fieldsD
     d.u0 = s.C.init.u0; d.length = s.C.init.length; d.pressure = s.C.init.pressure;
     d.fields = fieldsDev;
     const bool flip = l.info.esotwistFlipper != 0;
+    l.openSettled = true;
     launch_init<S>(reinterpret_cast<S *>(l.f.p), topo_of(l), d, l.geom, s.codec, flip, s.stream);
-    launch_open_ref_init<S>(open_args<S>(s, l), l.info.cellCountX, l.info.cellCountY, l.info.cellCountZ, flip, s.stream);
+    launch_open_ref_init<S>(open_args<S>(s, l, l.info.iterationsFinished), l.info.cellCountX, l.info.cellCountY, l.info.cellCountZ, flip, s.stream);
     return check_hip(hipGetLastError(), "init kernels");
 }
 
@@ -233,6 +266,8 @@ static int check_level(Sim &s, int level)
 }
 
 // every level a sub-cycled update touches is built, and every interface it runs is checked
+static int prepare_open_fusion(Sim &s, int L);
+
 static int check_run(Sim &s, int from)
 {
     for (int L = from; L < s.levels; ++L) {
@@ -240,6 +275,9 @@ static int check_run(Sim &s, int from)
         if (L + 1 < s.levels && !s.L[L].interfaceChecked)
             return set_error(LBM3_E_STATE, "interface lists of a level are not checked against the finer level");
     }
+    for (int L = from; L < s.levels; ++L)
+        if (!s.L[L].openFusionChecked)
+            if (const int rc = prepare_open_fusion(s, L)) return rc;
     return 0;
 }
 
@@ -471,6 +509,8 @@ int lbm3_update_single_grid(lbm3_sim *h, int level)
 {
     if (!h) return set_error(LBM3_E_ARG, "null sim");
     if (const int rc = check_level(h->s, level)) return rc;
+    if (!h->s.L[level].openFusionChecked)
+        if (const int rc = prepare_open_fusion(h->s, level)) return rc;
     step_level_any(h->s, level);
     return check_hip(hipGetLastError(), "updateSingleGrid launch");
 }
@@ -545,6 +585,7 @@ int lbm3_download_macros(lbm3_sim *h, int level, void *out)
     if (!h || !out) return set_error(LBM3_E_ARG, "null argument");
     Sim &s = h->s;
     if (const int rc = check_level(s, level)) return rc;
+    if (const int rc = settle_open(s, level)) return rc;
     Level &l = s.L[level];
     const size_t cb = compute_bytes(s.storage);
     DBuf<unsigned char> d;
@@ -568,6 +609,7 @@ int lbm3_download_populations(lbm3_sim *h, int level, void *out, long long bytes
     if (!h || !out) return set_error(LBM3_E_ARG, "null argument");
     Sim &s = h->s;
     if (const int rc = check_level(s, level)) return rc;
+    if (const int rc = settle_open(s, level)) return rc;
     if (bytes < s.L[level].f.n) return set_error(LBM3_E_ARG, "buffer too small");
     HIPCHK(hipMemcpyAsync(out, s.L[level].f.p, (size_t)s.L[level].f.n, hipMemcpyDeviceToHost, s.stream));
     HIPCHK(hipStreamSynchronize(s.stream));
@@ -582,6 +624,7 @@ int lbm3_upload_populations(lbm3_sim *h, int level, const void *in, long long by
     if (bytes != s.L[level].f.n) return set_error(LBM3_E_ARG, "size mismatch");
     HIPCHK(hipMemcpyAsync(s.L[level].f.p, in, (size_t)bytes, hipMemcpyHostToDevice, s.stream));
     HIPCHK(hipStreamSynchronize(s.stream));
+    s.L[level].openSettled = true;   // an uploaded state is complete
     return 0;
 }
 
@@ -598,6 +641,7 @@ int lbm3_download_array(lbm3_sim *h, int level, const char *name, void *out, lon
     ARR("shifter", l.shifter, l.N) ARR("rowI", l.ri, l.nRows) ARR("rowJ", l.rj, l.nRows) ARR("rowK", l.rk, l.nRows)
     ARR("rowJPlus", l.rjp, l.nRows) ARR("rowKPlus", l.rkp, l.nRows) ARR("rowJKPlus", l.rjkp, l.nRows)
     ARR("wallMap", l.wallMap, l.N) ARR("wallIndex", l.wallIndex, l.W) ARR("wallData", l.wallData, l.W)
+    ARR("cellRow", l.cellRow, l.N)
     ARR("linkLength", l.linkLength, 7LL * l.W) ARR("periodicIndex", l.perIndex, l.nPeriodic)
     ARR("periodicSource", l.perSource, l.nPeriodic) ARR("openIndex", l.openCell, l.nOpen)
     ARR("openSource", l.openSource, l.nOpen) ARR("openMeta", l.openMeta, l.nOpen)
@@ -618,6 +662,8 @@ int lbm3_download_array(lbm3_sim *h, int level, const char *name, void *out, lon
         }
         return 0;
     } else if (!strcmp(name, "openRefs")) {
+        if (l.built)
+            if (const int rc = settle_open(s, level)) return rc;
         src = l.openRefs.p;
         es = compute_bytes(s.storage);
         n = 2LL * (l.nOpen > 0 ? l.nOpen : 1);
@@ -637,6 +683,7 @@ int lbm3_kinetic_energy(lbm3_sim *h, double *out)
     double total = 0.0;
     for (int level = 0; level < s.levels; ++level) {
         if (const int rc = check_level(s, level)) return rc;
+        if (const int rc = settle_open(s, level)) return rc;
         Level &l = s.L[level];
         const long long nb = (l.N + 255) / 256;
         if (s.partial.n < nb) HIPCHK(s.partial.alloc(nb));

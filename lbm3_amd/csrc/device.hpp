@@ -87,6 +87,21 @@ struct LevelGeom {          // what the hooks need about the level
     int cellCountX, cellCountY, cellCountZ;
 };
 
+// Open-boundary cells updated by their own kernel (k_open_cells): each step
+// completes the populations the previous step left unknown (the deferred
+// boundary condition), then collides and streams the cell.  count == 0: the
+// level's open cells go through the bulk kernel and the separate pass.
+template <class T> struct FusedOpen {
+    const int *cell;              // [count] open-list cells, sorted by BCID then cell
+    int count;
+    int pending;                  // the previous step's boundary condition is still to apply
+    const uint32_t *meta;         // per open-list entry: normal index, record, track flag
+    const OpenBCRecord *records;
+    T *dRhoRef, *uNormalRef;
+    double *dRhoCum, *uNormalCum;
+    int iter;                     // iteration the pending boundary condition belongs to
+};
+
 template <class S> struct StepArgs {
     S *f;
     Topology topo;
@@ -102,6 +117,7 @@ template <class S> struct StepArgs {
     const LocalBC *wallBC;        // [32] by wallID
     LevelGeom geom;
     Codec cd;                     // 16-bit population encoding
+    FusedOpen<typename StoreTraits<S>::compute> open;
 };
 
 // Open boundary cells of one level, all BCIDs concatenated (BCID order, then cell order)

@@ -32,9 +32,11 @@ template <class T> struct DBuf {
     hipError_t alloc(long long count)
     {
         release();
-        n = count;
         if (count <= 0) return hipSuccess;
-        return hipMalloc((void **)&p, sizeof(T) * (size_t)count);
+        const hipError_t e = hipMalloc((void **)&p, sizeof(T) * (size_t)count);
+        if (e == hipSuccess) n = count;
+        else p = nullptr;
+        return e;
     }
 };
 
@@ -67,6 +69,10 @@ struct Level {
     InterfaceLists C2F, F2C;
     std::vector<int> fineRefs;       // imported lists: fine cells they name, checked against the finer level
     bool interfaceChecked = true;
+    // open boundary condition applied by the bulk kernel when it loads the cell
+    bool openFusionChecked = false;  // decided for the current lists
+    bool fusedOpen = false;          // the level uses it
+    bool openSettled = true;         // no boundary condition pending (after init, upload or settle)
     LocalBC bulk{};
     DBuf<LocalBC> wallBC;            // [32]
     DBuf<unsigned char> f;           // [27][N] storage type

@@ -1,11 +1,15 @@
 // Per-step kernels of one grid level for gfx950.
 //
-//  k_collide_stream  fused read -> cumulant collision -> esotwist write, with
-//                    interpolated bounceback and FP64 momentum-exchange force on
-//                    wall-adjacent cells (reference updateGrid.h:51-178,
-//                    interpolatedBouncebackFunctions.h:43-157)
-//  k_open            moment-based open boundaries, all BCIDs in one launch
-//                    (updateGrid.h:183-355)
+//  k_collide_stream  fused read -> cumulant collision -> esotwist write of the
+//                    cells that are neither wall-adjacent nor open boundary
+//                    cells (reference updateGrid.h:51-178)
+//  k_collide_wall    the same with interpolated bounceback and FP64 momentum-
+//                    exchange force on wall-adjacent cells
+//                    (interpolatedBouncebackFunctions.h:43-157)
+//  k_open_cells      the same for open-boundary cells, which first complete
+//                    the populations the previous step left unknown
+//  k_open            moment-based open boundaries as a pass of their own after
+//                    streaming, all BCIDs in one launch (updateGrid.h:183-355)
 //  k_periodic        periodic ghost copy (updateGrid.h:357-382)
 //  k_init, k_open_ref_init   initial condition (applyInitialCondition.h:8-84)
 //  k_macros, k_kinetic_energy, k_wall_force   observables, FP64 sums
@@ -86,6 +90,62 @@ __device__ __forceinline__ void bounce_back(T (&f)[27], const T (&pre)[27], uint
     }
 }
 
+// Moment-based open boundary of one cell with normal NI (updateGrid.h:183-355):
+// f holds the known populations (the unknown ones are overwritten), the
+// unknowns are solved for, and the reference state (EMA) and flow accumulators
+// of open-list entry x are updated.  `iter` is the iteration the boundary
+// condition belongs to.
+template <int NI, class T>
+__device__ __forceinline__ void open_fill(T (&f)[27], int x, uint32_t meta, const OpenBCRecord *records, T *dRhoRef,
+                                          T *uNormalRef, double *dRhoCum, double *uNormalCum, int iter)
+{
+    constexpr int nx = LBM3_BT_NORMAL[NI][0], ny = LBM3_BT_NORMAL[NI][1], nz = LBM3_BT_NORMAL[NI][2];
+    // open_cell leaves the known populations as full (unshifted) values: keep
+    // the stored ones, so the cell collides exactly what the separate pass would
+    // leave in memory (it writes back only the unknown ones)
+    T known[27];
+#pragma unroll
+    for (int q = 0; q < 27; ++q) {
+        known[q] = f[q];
+        if (unknown_dir(nx, ny, nz, q)) f[q] = T(0);
+    }
+    const OpenBCRecord R = records[(meta >> 5) & 255u];
+    const T dRef = dRhoRef[x], uRef = uNormalRef[x];
+    T dRho, un;
+    open_cell<NI>(f, R, dRef, uRef, dRho, un);
+#pragma unroll
+    for (int q = 0; q < 27; ++q)
+        if (!unknown_dir(nx, ny, nz, q)) f[q] = known[q];
+    // exponential moving average of the reference state
+    const T beta = T(R.beta);
+    T alpha = beta / (T(iter) + beta + T(10));
+    int every = 1;
+    while (alpha < T(1.e-5f)) { alpha *= T(3); every *= 3; }
+    if (iter % every == 0) {
+        dRhoRef[x] = fma(alpha, dRho - dRef, dRef);
+        uNormalRef[x] = fma(alpha, un - uRef, uRef);
+    }
+    if (dRhoCum && ((meta >> 13) & 1u)) {
+        dRhoCum[x] += double(dRho);
+        uNormalCum[x] += double(un);
+    }
+}
+
+template <class T>
+__device__ __forceinline__ void open_dispatch(T (&f)[27], int x, uint32_t meta, const OpenBCRecord *records, T *dRhoRef,
+                                              T *uNormalRef, double *dRhoCum, double *uNormalCum, int iter)
+{
+    switch (meta & 31u) {
+#define OPEN_CASE(n) case n: open_fill<n, T>(f, x, meta, records, dRhoRef, uNormalRef, dRhoCum, uNormalCum, iter); break;
+        OPEN_CASE(0) OPEN_CASE(1) OPEN_CASE(2) OPEN_CASE(3) OPEN_CASE(4) OPEN_CASE(5) OPEN_CASE(6)
+        OPEN_CASE(7) OPEN_CASE(8) OPEN_CASE(9) OPEN_CASE(10) OPEN_CASE(11) OPEN_CASE(12) OPEN_CASE(13)
+        OPEN_CASE(14) OPEN_CASE(15) OPEN_CASE(16) OPEN_CASE(17) OPEN_CASE(18) OPEN_CASE(19) OPEN_CASE(20)
+        OPEN_CASE(21) OPEN_CASE(22) OPEN_CASE(23) OPEN_CASE(24) OPEN_CASE(25)
+#undef OPEN_CASE
+    default: break;
+    }
+}
+
 // neighbour addressing inside one population plane: a wave-uniform plane base
 // plus a 32-bit byte offset per lane (planes are limited to 4 GiB)
 template <class S> __device__ __forceinline__ S *at(S *plane, uint32_t off)
@@ -94,23 +154,24 @@ template <class S> __device__ __forceinline__ S *at(S *plane, uint32_t off)
 }
 
 // Cells are independent under esotwist streaming (each one reads and writes
-// only its own 27 slots), so the level is updated by two kernels: one over all
-// cells that skips the wall-adjacent ones, and one over the short list of
-// wall-adjacent cells.  The bulk kernel then carries no bounceback code, which
-// keeps its register budget and instruction stream to the collision itself.
+// only its own 27 slots), so the level is updated by kernels over disjoint cell
+// sets: one over all cells that skips the special ones, one over the short list
+// of wall-adjacent cells and one over the open-boundary cells.  The bulk kernel
+// then carries neither bounceback nor boundary-condition code, which keeps its
+// register budget and instruction stream to the collision itself, and no wave
+// of it diverges on the odd boundary cell at the start or end of a row.
 //
 // BULK: collision variant of the cells that are not next to a wall, fixed for
-// the level on the host (1: K17, 2: K15).
+// the level on the host (1: K17, 2: K15).  Topology record >= 0: wall-adjacent,
+// -3: solid, <= -4: open cell with its own kernel.
 template <class S, bool FLIP, int BULK>
-__global__ __launch_bounds__(256) void k_collide_stream(StepArgs<S> a)
+__device__ __forceinline__ void bulk_cell(const StepArgs<S> &a, int cell)
 {
     using T = typename StoreTraits<S>::compute;
     const int N = a.fast.N;
-    const int cell = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
-    if (cell >= N) return;
     const long long rec = __builtin_nontemporal_load(reinterpret_cast<const long long *>(a.fast.cellRow) + cell);
     const int wm = int(rec), row = int(rec >> 32);
-    if (wm >= 0 || wm == -3) return;   // wall-adjacent (other kernel) or solid
+    if (wm >= 0 || wm <= -3) return;   // handled by another kernel, or solid
     int nb[8];
     decode_fast(a.fast, cell, row, nb);
     uint32_t off[8];
@@ -125,15 +186,48 @@ __global__ __launch_bounds__(256) void k_collide_stream(StepArgs<S> a)
         StoreTraits<S>::store(at(a.f + (size_t)post_plane(q, FLIP) * (size_t)N, off[post_nb(q)]), f[q], q, a.cd);
 }
 
+template <class S, bool FLIP, int BULK>
+__global__ __launch_bounds__(256) void k_collide_stream(StepArgs<S> a)
+{
+    const int cell = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+    if (cell >= a.fast.N) return;
+    bulk_cell<S, FLIP, BULK>(a, cell);
+}
+
+// Open-boundary cells (never wall-adjacent when this kernel is used): load,
+// complete the populations the previous step left unknown -- the moment-based
+// boundary condition of updateGrid.h:183-355, deferred from the end of that
+// step to this load -- then collide and stream like a bulk cell.  The open
+// list is sorted by cell within each BCID, so the face rows load coalesced.
+template <class S, bool FLIP, int BULK>
+__global__ __launch_bounds__(256) void k_open_cells(StepArgs<S> a)
+{
+    using T = typename StoreTraits<S>::compute;
+    const int x = blockIdx.x * 256 + threadIdx.x;
+    if (x >= a.open.count) return;
+    const int N = a.fast.N;
+    const int cell = a.open.cell[x];
+    int nb[8];
+    decode_fast(a.fast, cell, a.fast.cellRow[cell].y, nb);
+    T f[27];
+#pragma unroll
+    for (int q = 0; q < 27; ++q) f[q] = StoreTraits<S>::load(a.f + (size_t)pre_plane(q, FLIP) * (size_t)N + nb[pre_nb(q)], q, a.cd);
+    if (a.open.pending)
+        open_dispatch<T>(f, x, a.open.meta[x], a.open.records, a.open.dRhoRef, a.open.uNormalRef, a.open.dRhoCum,
+                         a.open.uNormalCum, a.open.iter);
+    collide<T, BULK>(f, relax_of<T>(a.bulk));
+#pragma unroll
+    for (int q = 0; q < 27; ++q)
+        StoreTraits<S>::store(a.f + (size_t)post_plane(q, FLIP) * (size_t)N + nb[post_nb(q)], f[q], q, a.cd);
+}
+
 // wall-adjacent cells: collision with the wall id's local BC, interpolated
 // bounceback of the links that cross the wall, momentum-exchange force
 template <class S, bool FLIP>
-__global__ __launch_bounds__(256) void k_collide_wall(StepArgs<S> a)
+__device__ __forceinline__ void wall_cell(const StepArgs<S> &a, int wm)
 {
     using T = typename StoreTraits<S>::compute;
     const int N = a.fast.N;
-    const int wm = blockIdx.x * 256 + threadIdx.x;
-    if (wm >= a.W) return;
     const int cell = a.wallIndex[wm];
     const int row = a.fast.cellRow[cell].y;
     int nb[8];
@@ -195,15 +289,52 @@ __global__ __launch_bounds__(256) void k_open_nb(Topology t, int count, const in
     out[2 * (size_t)x + 1] = make_int4(nb[4], nb[5], nb[6], nb[7]);
 }
 
-// derive the hot-loop topology records from the compressed IJK/NBR arrays
-__global__ __launch_bounds__(256) void k_fast_topo(Topology t, const int *wallMap, int2 *cellRow, int4 *rows)
+template <class S, bool FLIP>
+__global__ __launch_bounds__(256) void k_collide_wall(StepArgs<S> a)
+{
+    const int wm = blockIdx.x * 256 + threadIdx.x;
+    if (wm >= a.W) return;
+    wall_cell<S, FLIP>(a, wm);
+}
+
+// derive the hot-loop topology records from the compressed IJK/NBR arrays;
+// with openOf, the open-boundary cells get -(4 + open-list index) in place of
+// their wall-map code (they are never wall-adjacent when this is used)
+__global__ __launch_bounds__(256) void k_fast_topo(Topology t, const int *wallMap, const int *openOf, int2 *cellRow, int4 *rows)
 {
     const int cell = blockIdx.x * 256 + threadIdx.x;
     if (cell >= t.N) return;
     const int shift = t.shifter[cell];
     const int row = shift >= 0 ? shift : t.shifter[cell + shift];
-    cellRow[cell] = make_int2(wallMap[cell], row);
+    int code = wallMap[cell];
+    if (openOf && openOf[cell] >= 0) code = -(4 + openOf[cell]);
+    cellRow[cell] = make_int2(code, row);
     if (shift >= 0) rows[row] = make_int4(cell, t.rjp[row], t.rkp[row], t.rjkp[row]);
+}
+
+// cells a later pass of the step reads or writes as a whole (periodic copies,
+// interface stencils) must not carry a boundary condition that is still pending
+__global__ __launch_bounds__(256) void k_open_conflicts(const int *cells, int count, int mode, const int *openOf,
+                                                        const int *wallMap, FastTopo t, int *flag)
+{
+    const int x = blockIdx.x * 256 + threadIdx.x;
+    if (x >= count) return;
+    int c = cells[x];
+    bool hit = false;
+    if (mode == 0) hit = openOf[c] >= 0;                                  // listed cell
+    else if (mode == 1) { if (c < 0) c = -c - 1; hit = openOf[c] >= 0; }  // signed (leftover neighbours)
+    else if (mode == 2) {                                                  // 2x2x2 stencil from its first cell
+        int nb[8];
+        decode_fast(t, c, t.cellRow[c].y, nb);
+        for (int k = 0; k < 8; ++k) hit = hit || openOf[nb[k]] >= 0;
+    } else hit = wallMap[c] >= 0;                                          // open cell next to a wall
+    if (hit) atomicAdd(flag, 1);
+}
+
+__global__ __launch_bounds__(256) void k_scatter_index(const int *cell, int count, int *openOf)
+{
+    const int x = blockIdx.x * 256 + threadIdx.x;
+    if (x < count) openOf[cell[x]] = x;
 }
 
 template <int NI, class S, bool FLIP>
@@ -214,38 +345,19 @@ __device__ __forceinline__ void open_one(const OpenArgs<S> &a, int x, const int 
     const int N = a.topo.N;
     T f[27];
 #pragma unroll
-    for (int q = 0; q < 27; ++q) {
-        if (unknown_dir(nx, ny, nz, q)) { f[q] = T(0); continue; }
-        f[q] = StoreTraits<S>::load(a.f + (size_t)pre_plane(q, FLIP) * (size_t)N + nb[pre_nb(q)], q, a.cd);
-    }
-    const OpenBCRecord R = a.records[(meta >> 5) & 255u];
-    const T dRef = a.dRhoRef[x], uRef = a.uNormalRef[x];
-    T dRho, un;
-    open_cell<NI>(f, R, dRef, uRef, dRho, un);
+    for (int q = 0; q < 27; ++q)
+        f[q] = unknown_dir(nx, ny, nz, q) ? T(0)
+                                          : StoreTraits<S>::load(a.f + (size_t)pre_plane(q, FLIP) * (size_t)N + nb[pre_nb(q)], q, a.cd);
+    open_fill<NI, T>(f, x, meta, a.records, a.dRhoRef, a.uNormalRef, a.dRhoCum, a.uNormalCum, a.iterationsFinished);
 #pragma unroll
     for (int q = 0; q < 27; ++q)
         if (unknown_dir(nx, ny, nz, q))
             StoreTraits<S>::store(a.f + (size_t)pre_plane(q, FLIP) * (size_t)N + nb[pre_nb(q)], f[q], q, a.cd);
-    // exponential moving average of the reference state
-    const T beta = T(R.beta);
-    T alpha = beta / (T(a.iterationsFinished) + beta + T(10));
-    int every = 1;
-    while (alpha < T(1.e-5f)) { alpha *= T(3); every *= 3; }
-    if (a.iterationsFinished % every == 0) {
-        a.dRhoRef[x] = fma(alpha, dRho - dRef, dRef);
-        a.uNormalRef[x] = fma(alpha, un - uRef, uRef);
-    }
-    if (a.dRhoCum && ((meta >> 13) & 1u)) {
-        a.dRhoCum[x] += double(dRho);
-        a.uNormalCum[x] += double(un);
-    }
 }
 
 template <class S, bool FLIP>
-__global__ __launch_bounds__(256) void k_open(OpenArgs<S> a)
+__device__ __forceinline__ void open_entry(const OpenArgs<S> &a, int x)
 {
-    const int x = blockIdx.x * 256 + threadIdx.x;
-    if (x >= a.count) return;
     const int4 n0 = a.nb[2 * (size_t)x], n1 = a.nb[2 * (size_t)x + 1];
     const int nb[8] = {n0.x, n0.y, n0.z, n0.w, n1.x, n1.y, n1.z, n1.w};
     const uint32_t meta = a.meta[x];
@@ -258,6 +370,14 @@ __global__ __launch_bounds__(256) void k_open(OpenArgs<S> a)
 #undef OPEN_CASE
     default: break;
     }
+}
+
+template <class S, bool FLIP>
+__global__ __launch_bounds__(256) void k_open(OpenArgs<S> a)
+{
+    const int x = blockIdx.x * 256 + threadIdx.x;
+    if (x >= a.count) return;
+    open_entry<S, FLIP>(a, x);
 }
 
 // the stored values are copied as they are: source and ghost read them as the same directions
@@ -433,6 +553,85 @@ __global__ __launch_bounds__(256) void k_collide_batch(T *f, int n, Relax<T> R)
 }
 
 // ---------------------------------------------------------------------------
+// many domains in one launch (continuous-batching server): block b belongs to
+// the domain d with start[d] <= b < start[d + 1]; each domain carries its own
+// arguments, esotwist parity and collision variant
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int domain_of(const int *start, int n, int b)
+{
+    int lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (start[mid] <= b) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+template <class S>
+__global__ __launch_bounds__(256) void k_multi_bulk(const MultiDesc<S> *D, const int *start, int n)
+{
+    const int d = domain_of(start, n, blockIdx.x);
+    const MultiDesc<S> &m = D[d];
+    const int cell = (blockIdx.x - start[d]) * 256 + threadIdx.x;
+    if (cell >= m.step.fast.N) return;
+    if (m.flip) {
+        if (m.k17) bulk_cell<S, true, 1>(m.step, cell);
+        else bulk_cell<S, true, 2>(m.step, cell);
+    } else {
+        if (m.k17) bulk_cell<S, false, 1>(m.step, cell);
+        else bulk_cell<S, false, 2>(m.step, cell);
+    }
+}
+
+template <class S>
+__global__ __launch_bounds__(256) void k_multi_wall(const MultiDesc<S> *D, const int *start, int n)
+{
+    const int d = domain_of(start, n, blockIdx.x);
+    const MultiDesc<S> &m = D[d];
+    const int wm = (blockIdx.x - start[d]) * 256 + threadIdx.x;
+    if (wm >= m.step.W) return;
+    if (m.flip) wall_cell<S, true>(m.step, wm);
+    else wall_cell<S, false>(m.step, wm);
+}
+
+// after streaming: the open boundary and the periodic copy at the new parity
+template <class S>
+__global__ __launch_bounds__(256) void k_multi_open(const MultiDesc<S> *D, const int *start, int n)
+{
+    const int d = domain_of(start, n, blockIdx.x);
+    const MultiDesc<S> &m = D[d];
+    const int x = (blockIdx.x - start[d]) * 256 + threadIdx.x;
+    if (x >= m.open.count) return;
+    if (m.flip) open_entry<S, false>(m.open, x);
+    else open_entry<S, true>(m.open, x);
+}
+
+template <class S, bool FLIP>
+__device__ __forceinline__ void periodic_entry(S *f, const Topology &topo, const int *index, const int *source, int x)
+{
+    int ns[8], nd[8];
+    decode(topo, source[x], ns, nullptr);
+    decode(topo, index[x], nd, nullptr);
+    S v[27];
+#pragma unroll
+    for (int q = 0; q < 27; ++q) v[q] = f[(size_t)pre_plane(q, FLIP) * (size_t)topo.N + ns[pre_nb(q)]];
+#pragma unroll
+    for (int q = 0; q < 27; ++q) f[(size_t)pre_plane(q, FLIP) * (size_t)topo.N + nd[pre_nb(q)]] = v[q];
+}
+
+template <class S>
+__global__ __launch_bounds__(256) void k_multi_periodic(const MultiDesc<S> *D, const int *start, int n)
+{
+    const int d = domain_of(start, n, blockIdx.x);
+    const MultiDesc<S> &m = D[d];
+    const int x = (blockIdx.x - start[d]) * 256 + threadIdx.x;
+    if (x >= m.nPeriodic) return;
+    if (m.flip) periodic_entry<S, false>(m.step.f, m.step.topo, m.perIndex, m.perSource, x);
+    else periodic_entry<S, true>(m.step.f, m.step.topo, m.perIndex, m.perSource, x);
+}
+
+// ---------------------------------------------------------------------------
 // host launchers
 // ---------------------------------------------------------------------------
 static inline unsigned blocks_for(long long n) { return (unsigned)((n + 255) / 256); }
@@ -462,13 +661,20 @@ void launch_collide_stream(const StepArgs<S> &a, bool flip, hipStream_t s)
     const dim3 g(blocks_for(a.fast.N)), b(256);
     const unsigned lds = LBM3_BULK_LDS;
     const bool k17 = relax_of<typename StoreTraits<S>::compute>(a.bulk).k17 != 0;
-    if (flip) {
-        if (k17) hipLaunchKernelGGL((k_collide_stream<S, true, 1>), g, b, lds, s, a);
-        else hipLaunchKernelGGL((k_collide_stream<S, true, 2>), g, b, lds, s, a);
-    } else {
-        if (k17) hipLaunchKernelGGL((k_collide_stream<S, false, 1>), g, b, lds, s, a);
-        else hipLaunchKernelGGL((k_collide_stream<S, false, 2>), g, b, lds, s, a);
-    }
+#define LBM3_BULK_LAUNCH(F, K) hipLaunchKernelGGL((k_collide_stream<S, F, K>), g, b, lds, s, a)
+    if (flip) { if (k17) LBM3_BULK_LAUNCH(true, 1); else LBM3_BULK_LAUNCH(true, 2); }
+    else { if (k17) LBM3_BULK_LAUNCH(false, 1); else LBM3_BULK_LAUNCH(false, 2); }
+#undef LBM3_BULK_LAUNCH
+}
+
+template <class S>
+void launch_open_cells(const StepArgs<S> &a, bool flip, hipStream_t s)
+{
+    if (a.open.count <= 0) return;
+    const dim3 g(blocks_for(a.open.count)), b(256);
+    const bool k17 = relax_of<typename StoreTraits<S>::compute>(a.bulk).k17 != 0;
+    if (flip) { if (k17) hipLaunchKernelGGL((k_open_cells<S, true, 1>), g, b, 0, s, a); else hipLaunchKernelGGL((k_open_cells<S, true, 2>), g, b, 0, s, a); }
+    else { if (k17) hipLaunchKernelGGL((k_open_cells<S, false, 1>), g, b, 0, s, a); else hipLaunchKernelGGL((k_open_cells<S, false, 2>), g, b, 0, s, a); }
 }
 
 void launch_open_nb(const Topology &t, int count, const int *cell, int4 *out, hipStream_t s)
@@ -477,10 +683,23 @@ void launch_open_nb(const Topology &t, int count, const int *cell, int4 *out, hi
     hipLaunchKernelGGL(k_open_nb, dim3(blocks_for(count)), dim3(256), 0, s, t, count, cell, out);
 }
 
-void launch_fast_topo(const Topology &t, const int *wallMap, int2 *cellRow, int4 *rows, hipStream_t s)
+void launch_fast_topo(const Topology &t, const int *wallMap, const int *openOf, int2 *cellRow, int4 *rows, hipStream_t s)
 {
     if (t.N <= 0) return;
-    hipLaunchKernelGGL(k_fast_topo, dim3(blocks_for(t.N)), dim3(256), 0, s, t, wallMap, cellRow, rows);
+    hipLaunchKernelGGL(k_fast_topo, dim3(blocks_for(t.N)), dim3(256), 0, s, t, wallMap, openOf, cellRow, rows);
+}
+
+void launch_open_conflicts(const int *cells, int count, int mode, const int *openOf, const int *wallMap, const FastTopo &t,
+                           int *flag, hipStream_t s)
+{
+    if (count <= 0) return;
+    hipLaunchKernelGGL(k_open_conflicts, dim3(blocks_for(count)), dim3(256), 0, s, cells, count, mode, openOf, wallMap, t, flag);
+}
+
+void launch_scatter_index(const int *cell, int count, int *openOf, hipStream_t s)
+{
+    if (count <= 0) return;
+    hipLaunchKernelGGL(k_scatter_index, dim3(blocks_for(count)), dim3(256), 0, s, cell, count, openOf);
 }
 
 template <class S>
@@ -550,9 +769,23 @@ void launch_collide_batch(T *f, int n, const Relax<T> &R, hipStream_t s)
     hipLaunchKernelGGL((k_collide_batch<T>), dim3(blocks_for(n)), dim3(256), 0, s, f, n, R);
 }
 
+template <class S>
+void launch_multi_step(const MultiDesc<S> *D, const int *starts, int n, const int (&blocks)[4], hipStream_t s)
+{
+    // starts: four prefix arrays of n + 1 entries (bulk, wall, open, periodic blocks)
+    if (n <= 0) return;
+    const unsigned lds = LBM3_BULK_LDS;
+    if (blocks[0] > 0) hipLaunchKernelGGL((k_multi_bulk<S>), dim3(blocks[0]), dim3(256), lds, s, D, starts, n);
+    if (blocks[1] > 0) hipLaunchKernelGGL((k_multi_wall<S>), dim3(blocks[1]), dim3(256), 0, s, D, starts + (n + 1), n);
+    if (blocks[2] > 0) hipLaunchKernelGGL((k_multi_open<S>), dim3(blocks[2]), dim3(256), 0, s, D, starts + 2 * (n + 1), n);
+    if (blocks[3] > 0) hipLaunchKernelGGL((k_multi_periodic<S>), dim3(blocks[3]), dim3(256), 0, s, D, starts + 3 * (n + 1), n);
+}
+
 #define LBM3_INSTANTIATE(S)                                                                                          \
     template void launch_collide_stream<S>(const StepArgs<S> &, bool, hipStream_t);                                 \
     template void launch_collide_wall<S>(const StepArgs<S> &, bool, hipStream_t);                                   \
+    template void launch_open_cells<S>(const StepArgs<S> &, bool, hipStream_t);                                     \
+    template void launch_multi_step<S>(const MultiDesc<S> *, const int *, int, const int (&)[4], hipStream_t);       \
     template void launch_open<S>(const OpenArgs<S> &, bool, hipStream_t);                                           \
     template void launch_periodic<S>(S *, const Topology &, int, const int *, const int *, bool, hipStream_t);        \
     template void launch_init<S>(S *, const Topology &, const InitDesc &, const LevelGeom &, const Codec &, bool,     \

@@ -54,6 +54,13 @@ def load():
         "lbm3_wall_forces": (i32, [vp, i32, C.POINTER(f64), i32, C.POINTER(f64)]),
         "lbm3_reset_wall_forces": (i32, [vp]),
         "lbm3_copy_probe": (i32, [i64, i32, C.POINTER(f64)]),
+        "lbm3_server_create": (i32, [i64, i32, C.POINTER(vp)]),
+        "lbm3_server_destroy": (i32, [vp]),
+        "lbm3_server_submit": (i32, [vp, C.POINTER(T.Case), i32, vp, vp, i32, T.Bounds, i32, i32, C.POINTER(i64)]),
+        "lbm3_server_step": (i32, [vp, C.POINTER(i64), i32, C.POINTER(i32)]),
+        "lbm3_server_result": (i32, [vp, i64, C.POINTER(f64), i32]),
+        "lbm3_server_macros": (i32, [vp, i64, vp, i64]),
+        "lbm3_server_stats": (i32, [vp, C.POINTER(f64)]),
         "lbm3_collide_cells_f32": (i32, [vp, i32, C.POINTER(T.BC), f32]),
         "lbm3_collide_cells_f64": (i32, [vp, i32, C.POINTER(T.BC), f64]),
     }
@@ -199,7 +206,7 @@ class Sim:
         cnt = C.c_longlong()
         check(load().lbm3_download_array(self.h, level, name.encode(), None, 0, C.byref(cnt)))
         n = max(cnt.value, 0)
-        if name in ("rayHitScan",):
+        if name in ("rayHitScan", "cellRow"):
             dt = np.int64
         elif name in ("wallForce",):
             dt = np.float64
@@ -213,6 +220,14 @@ class Sim:
         if n:
             check(load().lbm3_download_array(self.h, level, name.encode(), out.ctypes.data, n, C.byref(cnt)))
         return out
+
+    def cell_kinds(self, level=0):
+        """Counts of the cells each per-step kernel updates on a level:
+        {"bulk", "wall", "open", "solid"} (open: cells with their own kernel;
+        where the open boundary runs as a separate pass they count as bulk)."""
+        code = (self.array("cellRow", level) & 0xFFFFFFFF).astype(np.uint32).view(np.int32)
+        return {"bulk": int(((code == -1) | (code == -2)).sum()), "wall": int((code >= 0).sum()),
+                "open": int((code <= -4).sum()), "solid": int((code == -3).sum())}
 
     def kinetic_energy(self):
         v = C.c_double()
@@ -259,3 +274,57 @@ def copy_probe(nbytes, reps=20):
     ms = C.c_double()
     check(load().lbm3_copy_probe(int(nbytes), int(reps), C.byref(ms)))
     return ms.value
+
+
+class Server:
+    """Continuous-batching server of single-level simulations (lbm3_server_*)."""
+
+    RESULT_FIELDS = ("wait_s", "time_to_solution_s", "fluid_updates", "kinetic_energy", "fx", "fy", "fz", "tx", "ty",
+                     "tz", "status", "steps")
+
+    def __init__(self, hbm_budget_bytes, max_active=64):
+        h = C.c_void_p()
+        check(load().lbm3_server_create(int(hbm_budget_bytes), int(max_active), C.byref(h)))
+        self.h = h
+        self._cells = {}
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().lbm3_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def submit(self, case, domain, triangles=(), steps=100, storage="f32", keep_macros=False):
+        tris = [np.ascontiguousarray(t, np.float32).reshape(-1, 9) for t in triangles]
+        allt = np.concatenate(tris) if tris else np.zeros((1, 9), np.float32)
+        counts = np.array([len(t) for t in tris] or [0], np.int32)
+        rid = C.c_longlong()
+        check(load().lbm3_server_submit(self.h, C.byref(case), STORE[storage], allt.ctypes.data, counts.ctypes.data,
+                                         len(tris), domain, int(steps), int(keep_macros), C.byref(rid)))
+        self._cells[rid.value] = storage
+        return rid.value
+
+    def step(self, cap=1024):
+        ids = (C.c_longlong * cap)()
+        n = C.c_int()
+        check(load().lbm3_server_step(self.h, ids, cap, C.byref(n)))
+        return [ids[i] for i in range(min(n.value, cap))]
+
+    def result(self, rid):
+        out = (C.c_double * 12)()
+        check(load().lbm3_server_result(self.h, int(rid), out, 12))
+        return dict(zip(self.RESULT_FIELDS, list(out)))
+
+    def macros(self, rid, cells):
+        dt = np.float64 if self._cells.get(rid) == "f64" else np.float32
+        out = np.zeros((cells, 4), dt)
+        check(load().lbm3_server_macros(self.h, int(rid), out.ctypes.data, out.nbytes))
+        return out
+
+    def stats(self):
+        out = (C.c_double * 5)()
+        check(load().lbm3_server_stats(self.h, out))
+        return {"active": int(out[0]), "queued": int(out[1]), "bytes": out[2], "steps": int(out[3]),
+                "updates": out[4]}
