@@ -66,8 +66,7 @@ int finish_level_setup(Sim &s, int level)
     Level &l = s.L[level];
     l.geom = LevelGeom{l.info.res, l.info.ox, l.info.oy, l.info.oz, l.info.dtPhys, l.info.cellCountX, l.info.cellCountY,
                        l.info.cellCountZ};
-    if ((unsigned long long)l.N * storage_bytes(s.storage) >= (1ULL << 32))
-        return set_error(LBM3_E_MEMORY, "a level's population plane exceeds 4 GiB; split the domain over more levels or GPUs");
+    if (l.N > (1LL << 31) - 1 - LBM3_TILE) return set_error(LBM3_E_MEMORY, "a level exceeds 2^31 cells");
     l.bulk = local_bc(s.C, level, -1);
     set_relax(l.bulk, double(l.info.nu));
     LocalBC wb[32];
@@ -94,7 +93,7 @@ int finish_level_setup(Sim &s, int level)
     HIPCHK(hipMemsetAsync(l.openRefs.p, 0, (size_t)l.openRefs.n, s.stream));
     HIPCHK(l.openCum.alloc(2LL * (l.nOpen > 0 ? l.nOpen : 1)));
     HIPCHK(hipMemsetAsync(l.openCum.p, 0, sizeof(double) * (size_t)l.openCum.n, s.stream));
-    HIPCHK(l.f.alloc(27LL * l.N * (long long)storage_bytes(s.storage)));
+    HIPCHK(l.f.alloc(27LL * padded_cells(l.N) * (long long)storage_bytes(s.storage)));
     l.built = true;
     return 0;
 }
@@ -610,8 +609,14 @@ int lbm3_download_populations(lbm3_sim *h, int level, void *out, long long bytes
     Sim &s = h->s;
     if (const int rc = check_level(s, level)) return rc;
     if (const int rc = settle_open(s, level)) return rc;
-    if (bytes < s.L[level].f.n) return set_error(LBM3_E_ARG, "buffer too small");
-    HIPCHK(hipMemcpyAsync(out, s.L[level].f.p, (size_t)s.L[level].f.n, hipMemcpyDeviceToHost, s.stream));
+    Level &l = s.L[level];
+    const long long es = storage_bytes(s.storage), n = 27LL * l.N * es;
+    if (bytes < n) return set_error(LBM3_E_ARG, "buffer too small");
+    DBuf<unsigned char> d;
+    HIPCHK(d.alloc(n > 0 ? n : 1));
+    launch_relayout(l.f.p, d.p, l.N, int(es), true, s.stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out, d.p, (size_t)n, hipMemcpyDeviceToHost, s.stream));
     HIPCHK(hipStreamSynchronize(s.stream));
     return 0;
 }
@@ -621,8 +626,14 @@ int lbm3_upload_populations(lbm3_sim *h, int level, const void *in, long long by
     if (!h || !in) return set_error(LBM3_E_ARG, "null argument");
     Sim &s = h->s;
     if (const int rc = check_level(s, level)) return rc;
-    if (bytes != s.L[level].f.n) return set_error(LBM3_E_ARG, "size mismatch");
-    HIPCHK(hipMemcpyAsync(s.L[level].f.p, in, (size_t)bytes, hipMemcpyHostToDevice, s.stream));
+    Level &l = s.L[level];
+    const long long es = storage_bytes(s.storage), n = 27LL * l.N * es;
+    if (bytes != n) return set_error(LBM3_E_ARG, "size mismatch");
+    DBuf<unsigned char> d;
+    HIPCHK(d.alloc(n > 0 ? n : 1));
+    HIPCHK(hipMemcpyAsync(d.p, in, (size_t)n, hipMemcpyHostToDevice, s.stream));
+    launch_relayout(d.p, l.f.p, l.N, int(es), false, s.stream);
+    HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(s.stream));
     s.L[level].openSettled = true;   // an uploaded state is complete
     return 0;

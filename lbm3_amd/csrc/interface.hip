@@ -26,7 +26,7 @@ __device__ __forceinline__ void load_cell(const S *f, const FastTopo &t, int cel
 {
     decode_fast(t, cell, t.cellRow[cell].y, nb);
 #pragma unroll
-    for (int q = 0; q < 27; ++q) v[q] = StoreTraits<S>::load(f + (size_t)pre_plane(q, FLIP) * (size_t)t.N + nb[pre_nb(q)], q, cd);
+    for (int q = 0; q < 27; ++q) v[q] = StoreTraits<S>::load(f + pop_index(pre_plane(q, FLIP), nb[pre_nb(q)]), q, cd);
 }
 
 // the stencil whose lowest corner is `c0`: self, i+, j+, ij+, k+, ik+, jk+, ijk+
@@ -95,7 +95,7 @@ __global__ __launch_bounds__(256) void k_fine_to_coarse(IfArgs<typename StoreTra
     // lane n writes populations n, n+8, n+16, n+24
 #pragma unroll
     for (int q = 0; q < 27; ++q)
-        if ((q & 7) == n) StoreTraits<S>::store(fc + (size_t)pre_plane(q, FLIPC) * (size_t)a.coarse.N + nb[pre_nb(q)], f[q], q, a.cd);
+        if ((q & 7) == n) StoreTraits<S>::store(fc + pop_index(pre_plane(q, FLIPC), nb[pre_nb(q)]), f[q], q, a.cd);
 }
 
 // C->F: the eight fine cells inside a 2x2x2 coarse block, at +-1/4
@@ -133,7 +133,7 @@ __global__ __launch_bounds__(256) void k_coarse_to_fine(IfArgs<typename StoreTra
     decode_fast(a.fine, cell, a.fine.cellRow[cell].y, nb);
     S *ff = reinterpret_cast<S *>(a.ff);
 #pragma unroll
-    for (int q = 0; q < 27; ++q) StoreTraits<S>::store(ff + (size_t)pre_plane(q, FLIPF) * (size_t)a.fine.N + nb[pre_nb(q)], f[q], q, a.cd);
+    for (int q = 0; q < 27; ++q) StoreTraits<S>::store(ff + pop_index(pre_plane(q, FLIPF), nb[pre_nb(q)]), f[q], q, a.cd);
 }
 
 // fine cells without a full coarse block (updateInterface.h:689-884)
@@ -199,7 +199,7 @@ __global__ __launch_bounds__(256) void k_leftovers(IfArgs<typename StoreTraits<S
     decode_fast(a.fine, cell, a.fine.cellRow[cell].y, nb);
     S *ff = reinterpret_cast<S *>(a.ff);
 #pragma unroll
-    for (int m = 0; m < 27; ++m) StoreTraits<S>::store(ff + (size_t)pre_plane(m, FLIPF) * (size_t)a.fine.N + nb[pre_nb(m)], f[m], m, a.cd);
+    for (int m = 0; m < 27; ++m) StoreTraits<S>::store(ff + pop_index(pre_plane(m, FLIPF), nb[pre_nb(m)]), f[m], m, a.cd);
 }
 
 inline unsigned groups_of_8(long long items) { return (unsigned)((8 * items + 255) / 256); }

@@ -117,6 +117,7 @@ static int admit(Server &sv, Request &r)
     long long updates = 0;
     if (!rc) rc = lbm3_build_grids(r.sim, r.tri.empty() ? nullptr : r.tri.data(), r.counts.empty() ? nullptr : r.counts.data(),
                                    (int)r.counts.size(), r.dom, &updates);
+    if (!rc && wide_level<double>(r.sim->s.L[0].N)) rc = set_error(LBM3_E_MEMORY, "request too large for the batched kernels");
     if (!rc) rc = lbm3_apply_initial_condition(r.sim, 0);
     if (!rc) rc = check_hip(hipStreamSynchronize(r.sim->s.stream), "request build");
     if (rc) {

@@ -28,7 +28,7 @@ __device__ __forceinline__ void load_pre(const S *f, int N, const int (&nb)[8], 
                                          const Codec &cd)
 {
 #pragma unroll
-    for (int q = 0; q < 27; ++q) v[q] = StoreTraits<S>::load(f + (size_t)pre_plane(q, FLIP) * (size_t)N + nb[pre_nb(q)], q, cd);
+    for (int q = 0; q < 27; ++q) v[q] = StoreTraits<S>::load(f + pop_index(pre_plane(q, FLIP), nb[pre_nb(q)]), q, cd);
 }
 
 template <class S, bool FLIP>
@@ -36,7 +36,7 @@ __device__ __forceinline__ void store_pre(S *f, int N, const int (&nb)[8], const
                                           const Codec &cd)
 {
 #pragma unroll
-    for (int q = 0; q < 27; ++q) StoreTraits<S>::store(f + (size_t)pre_plane(q, FLIP) * (size_t)N + nb[pre_nb(q)], v[q], q, cd);
+    for (int q = 0; q < 27; ++q) StoreTraits<S>::store(f + pop_index(pre_plane(q, FLIP), nb[pre_nb(q)]), v[q], q, cd);
 }
 
 // interpolated bounceback (Zhao & Yong 2017, single node, l = gamma) on the
@@ -146,11 +146,11 @@ __device__ __forceinline__ void open_dispatch(T (&f)[27], int x, uint32_t meta, 
     }
 }
 
-// neighbour addressing inside one population plane: a wave-uniform plane base
-// plus a 32-bit byte offset per lane (planes are limited to 4 GiB)
-template <class S> __device__ __forceinline__ S *at(S *plane, uint32_t off)
+// neighbour addressing: a wave-uniform base (array + plane * TILE) plus a byte
+// offset per lane, 32-bit where the level's array is below 4 GiB
+template <class S, class Off> __device__ __forceinline__ S *at(S *base, Off off)
 {
-    return reinterpret_cast<S *>(reinterpret_cast<char *>(plane) + off);
+    return reinterpret_cast<S *>(reinterpret_cast<char *>(base) + off);
 }
 
 // Cells are independent under esotwist streaming (each one reads and writes
@@ -164,34 +164,38 @@ template <class S> __device__ __forceinline__ S *at(S *plane, uint32_t off)
 // BULK: collision variant of the cells that are not next to a wall, fixed for
 // the level on the host (1: K17, 2: K15).  Topology record >= 0: wall-adjacent,
 // -3: solid, <= -4: open cell with its own kernel.
-template <class S, bool FLIP, int BULK>
+template <class S, bool FLIP, int BULK, class Off>
 __device__ __forceinline__ void bulk_cell(const StepArgs<S> &a, int cell)
 {
     using T = typename StoreTraits<S>::compute;
-    const int N = a.fast.N;
     const long long rec = __builtin_nontemporal_load(reinterpret_cast<const long long *>(a.fast.cellRow) + cell);
     const int wm = int(rec), row = int(rec >> 32);
     if (wm >= 0 || wm <= -3) return;   // handled by another kernel, or solid
     int nb[8];
     decode_fast(a.fast, cell, row, nb);
-    uint32_t off[8];
+    // byte offset of each neighbour's slot run in its tile; the plane adds a
+    // wave-uniform plane * TILE to the base
+    Off off[8];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) off[k] = uint32_t(nb[k]) * uint32_t(sizeof(S));
+    for (int k = 0; k < 8; ++k)
+        off[k] = (Off(nb[k] / LBM3_TILE) * Off(27 * LBM3_TILE) + Off(nb[k] % LBM3_TILE)) * Off(sizeof(S));
     T f[27];
 #pragma unroll
-    for (int q = 0; q < 27; ++q) f[q] = StoreTraits<S>::load(at(a.f + (size_t)pre_plane(q, FLIP) * (size_t)N, off[pre_nb(q)]), q, a.cd);
+    for (int q = 0; q < 27; ++q) f[q] = StoreTraits<S>::load(at(a.f + pre_plane(q, FLIP) * LBM3_TILE, off[pre_nb(q)]), q, a.cd);
     collide<T, BULK>(f, relax_of<T>(a.bulk));
 #pragma unroll
     for (int q = 0; q < 27; ++q)
-        StoreTraits<S>::store(at(a.f + (size_t)post_plane(q, FLIP) * (size_t)N, off[post_nb(q)]), f[q], q, a.cd);
+        StoreTraits<S>::store(at(a.f + post_plane(q, FLIP) * LBM3_TILE, off[post_nb(q)]), f[q], q, a.cd);
 }
 
-template <class S, bool FLIP, int BULK>
+// WIDE: the level's population array exceeds 4 GiB (64-bit lane offsets)
+template <class S, bool FLIP, int BULK, bool WIDE>
 __global__ __launch_bounds__(256) void k_collide_stream(StepArgs<S> a)
 {
     const int cell = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
     if (cell >= a.fast.N) return;
-    bulk_cell<S, FLIP, BULK>(a, cell);
+    if (WIDE) bulk_cell<S, FLIP, BULK, uint64_t>(a, cell);
+    else bulk_cell<S, FLIP, BULK, uint32_t>(a, cell);
 }
 
 // Open-boundary cells (never wall-adjacent when this kernel is used): load,
@@ -205,20 +209,19 @@ __global__ __launch_bounds__(256) void k_open_cells(StepArgs<S> a)
     using T = typename StoreTraits<S>::compute;
     const int x = blockIdx.x * 256 + threadIdx.x;
     if (x >= a.open.count) return;
-    const int N = a.fast.N;
     const int cell = a.open.cell[x];
     int nb[8];
     decode_fast(a.fast, cell, a.fast.cellRow[cell].y, nb);
     T f[27];
 #pragma unroll
-    for (int q = 0; q < 27; ++q) f[q] = StoreTraits<S>::load(a.f + (size_t)pre_plane(q, FLIP) * (size_t)N + nb[pre_nb(q)], q, a.cd);
+    for (int q = 0; q < 27; ++q) f[q] = StoreTraits<S>::load(a.f + pop_index(pre_plane(q, FLIP), nb[pre_nb(q)]), q, a.cd);
     if (a.open.pending)
         open_dispatch<T>(f, x, a.open.meta[x], a.open.records, a.open.dRhoRef, a.open.uNormalRef, a.open.dRhoCum,
                          a.open.uNormalCum, a.open.iter);
     collide<T, BULK>(f, relax_of<T>(a.bulk));
 #pragma unroll
     for (int q = 0; q < 27; ++q)
-        StoreTraits<S>::store(a.f + (size_t)post_plane(q, FLIP) * (size_t)N + nb[post_nb(q)], f[q], q, a.cd);
+        StoreTraits<S>::store(a.f + pop_index(post_plane(q, FLIP), nb[post_nb(q)]), f[q], q, a.cd);
 }
 
 // wall-adjacent cells: collision with the wall id's local BC, interpolated
@@ -267,9 +270,9 @@ __device__ __forceinline__ void wall_cell(const StepArgs<S> &a, int wm)
 #pragma unroll
     for (int d = 0; d < 27; ++d) {
         if (d != 0 && ((wd >> d) & 1u))
-            StoreTraits<S>::store(a.f + (size_t)pre_plane(inv(d), !FLIP) * (size_t)N + nb[pre_nb(inv(d))], f[d], inv(d), a.cd);
+            StoreTraits<S>::store(a.f + pop_index(pre_plane(inv(d), !FLIP), nb[pre_nb(inv(d))]), f[d], inv(d), a.cd);
         else
-            StoreTraits<S>::store(a.f + (size_t)post_plane(d, FLIP) * (size_t)N + nb[post_nb(d)], f[d], d, a.cd);
+            StoreTraits<S>::store(a.f + pop_index(post_plane(d, FLIP), nb[post_nb(d)]), f[d], d, a.cd);
     }
     if (a.gWall && !(wd & 1u)) {
         a.gWall[wm] += g3[0];
@@ -342,17 +345,16 @@ __device__ __forceinline__ void open_one(const OpenArgs<S> &a, int x, const int 
 {
     using T = typename StoreTraits<S>::compute;
     constexpr int nx = LBM3_BT_NORMAL[NI][0], ny = LBM3_BT_NORMAL[NI][1], nz = LBM3_BT_NORMAL[NI][2];
-    const int N = a.topo.N;
     T f[27];
 #pragma unroll
     for (int q = 0; q < 27; ++q)
         f[q] = unknown_dir(nx, ny, nz, q) ? T(0)
-                                          : StoreTraits<S>::load(a.f + (size_t)pre_plane(q, FLIP) * (size_t)N + nb[pre_nb(q)], q, a.cd);
+                                          : StoreTraits<S>::load(a.f + pop_index(pre_plane(q, FLIP), nb[pre_nb(q)]), q, a.cd);
     open_fill<NI, T>(f, x, meta, a.records, a.dRhoRef, a.uNormalRef, a.dRhoCum, a.uNormalCum, a.iterationsFinished);
 #pragma unroll
     for (int q = 0; q < 27; ++q)
         if (unknown_dir(nx, ny, nz, q))
-            StoreTraits<S>::store(a.f + (size_t)pre_plane(q, FLIP) * (size_t)N + nb[pre_nb(q)], f[q], q, a.cd);
+            StoreTraits<S>::store(a.f + pop_index(pre_plane(q, FLIP), nb[pre_nb(q)]), f[q], q, a.cd);
 }
 
 template <class S, bool FLIP>
@@ -391,9 +393,9 @@ __global__ __launch_bounds__(256) void k_periodic(S *f, Topology topo, int count
     decode(topo, index[x], nd, nullptr);
     S v[27];
 #pragma unroll
-    for (int q = 0; q < 27; ++q) v[q] = f[(size_t)pre_plane(q, FLIP) * (size_t)topo.N + ns[pre_nb(q)]];
+    for (int q = 0; q < 27; ++q) v[q] = f[pop_index(pre_plane(q, FLIP), ns[pre_nb(q)])];
 #pragma unroll
-    for (int q = 0; q < 27; ++q) f[(size_t)pre_plane(q, FLIP) * (size_t)topo.N + nd[pre_nb(q)]] = v[q];
+    for (int q = 0; q < 27; ++q) f[pop_index(pre_plane(q, FLIP), nd[pre_nb(q)])] = v[q];
 }
 
 template <class S, bool FLIP>
@@ -575,12 +577,13 @@ __global__ __launch_bounds__(256) void k_multi_bulk(const MultiDesc<S> *D, const
     const MultiDesc<S> &m = D[d];
     const int cell = (blockIdx.x - start[d]) * 256 + threadIdx.x;
     if (cell >= m.step.fast.N) return;
+    // server domains are far below 4 GiB each (checked at admission): 32-bit offsets
     if (m.flip) {
-        if (m.k17) bulk_cell<S, true, 1>(m.step, cell);
-        else bulk_cell<S, true, 2>(m.step, cell);
+        if (m.k17) bulk_cell<S, true, 1, uint32_t>(m.step, cell);
+        else bulk_cell<S, true, 2, uint32_t>(m.step, cell);
     } else {
-        if (m.k17) bulk_cell<S, false, 1>(m.step, cell);
-        else bulk_cell<S, false, 2>(m.step, cell);
+        if (m.k17) bulk_cell<S, false, 1, uint32_t>(m.step, cell);
+        else bulk_cell<S, false, 2, uint32_t>(m.step, cell);
     }
 }
 
@@ -615,9 +618,9 @@ __device__ __forceinline__ void periodic_entry(S *f, const Topology &topo, const
     decode(topo, index[x], nd, nullptr);
     S v[27];
 #pragma unroll
-    for (int q = 0; q < 27; ++q) v[q] = f[(size_t)pre_plane(q, FLIP) * (size_t)topo.N + ns[pre_nb(q)]];
+    for (int q = 0; q < 27; ++q) v[q] = f[pop_index(pre_plane(q, FLIP), ns[pre_nb(q)])];
 #pragma unroll
-    for (int q = 0; q < 27; ++q) f[(size_t)pre_plane(q, FLIP) * (size_t)topo.N + nd[pre_nb(q)]] = v[q];
+    for (int q = 0; q < 27; ++q) f[pop_index(pre_plane(q, FLIP), nd[pre_nb(q)])] = v[q];
 }
 
 template <class S>
@@ -661,9 +664,15 @@ void launch_collide_stream(const StepArgs<S> &a, bool flip, hipStream_t s)
     const dim3 g(blocks_for(a.fast.N)), b(256);
     const unsigned lds = LBM3_BULK_LDS;
     const bool k17 = relax_of<typename StoreTraits<S>::compute>(a.bulk).k17 != 0;
-#define LBM3_BULK_LAUNCH(F, K) hipLaunchKernelGGL((k_collide_stream<S, F, K>), g, b, lds, s, a)
-    if (flip) { if (k17) LBM3_BULK_LAUNCH(true, 1); else LBM3_BULK_LAUNCH(true, 2); }
-    else { if (k17) LBM3_BULK_LAUNCH(false, 1); else LBM3_BULK_LAUNCH(false, 2); }
+    const bool wide = wide_level<S>(a.fast.N);
+#define LBM3_BULK_LAUNCH(F, K, W) hipLaunchKernelGGL((k_collide_stream<S, F, K, W>), g, b, lds, s, a)
+    if (wide) {
+        if (flip) { if (k17) LBM3_BULK_LAUNCH(true, 1, true); else LBM3_BULK_LAUNCH(true, 2, true); }
+        else { if (k17) LBM3_BULK_LAUNCH(false, 1, true); else LBM3_BULK_LAUNCH(false, 2, true); }
+    } else {
+        if (flip) { if (k17) LBM3_BULK_LAUNCH(true, 1, false); else LBM3_BULK_LAUNCH(true, 2, false); }
+        else { if (k17) LBM3_BULK_LAUNCH(false, 1, false); else LBM3_BULK_LAUNCH(false, 2, false); }
+    }
 #undef LBM3_BULK_LAUNCH
 }
 
@@ -817,6 +826,25 @@ __global__ __launch_bounds__(256) void k_copy(const float4 *__restrict__ a, floa
 }
 
 // 16-byte vector copy of `bytes` (the buffers come from hipMalloc, 256-byte aligned)
+// tiled level array <-> plain planes q * N + cell (the reference's layout), elements of E bytes
+template <class E> __global__ __launch_bounds__(256) void k_relayout(const E *src, E *dst, int N, int toPlanes)
+{
+    const long long x = blockIdx.x * 256LL + threadIdx.x;
+    if (x >= 27LL * N) return;
+    const int q = int(x / N), c = int(x % N);
+    if (toPlanes) dst[x] = src[pop_index(q, c)];
+    else dst[pop_index(q, c)] = src[x];
+}
+
+void launch_relayout(const void *src, void *dst, int N, int elemBytes, bool toPlanes, hipStream_t s)
+{
+    if (N <= 0) return;
+    const unsigned g = blocks_for(27LL * N);
+    if (elemBytes == 2) hipLaunchKernelGGL(k_relayout<uint16_t>, dim3(g), dim3(256), 0, s, (const uint16_t *)src, (uint16_t *)dst, N, int(toPlanes));
+    else if (elemBytes == 4) hipLaunchKernelGGL(k_relayout<uint32_t>, dim3(g), dim3(256), 0, s, (const uint32_t *)src, (uint32_t *)dst, N, int(toPlanes));
+    else hipLaunchKernelGGL(k_relayout<uint64_t>, dim3(g), dim3(256), 0, s, (const uint64_t *)src, (uint64_t *)dst, N, int(toPlanes));
+}
+
 void launch_copy(const void *a, void *b, long long bytes, hipStream_t s)
 {
     const long long n4 = bytes / 16;

@@ -50,5 +50,6 @@ void launch_scatter_index(const int *cell, int count, int *openOf, hipStream_t s
 void launch_open_conflicts(const int *cells, int count, int mode, const int *openOf, const int *wallMap, const FastTopo &t,
                            int *flag, hipStream_t s);
 void launch_copy(const void *a, void *b, long long bytes, hipStream_t s);
+void launch_relayout(const void *src, void *dst, int N, int elemBytes, bool toPlanes, hipStream_t s);
 
 } // namespace lbm3

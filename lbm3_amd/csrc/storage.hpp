@@ -22,6 +22,39 @@
 
 namespace lbm3 {
 
+// Tiled population layout.  Cells are grouped in tiles of LBM3_TILE
+// consecutive cells; a tile holds its 27 slot planes one after another, each
+// LBM3_TILE entries long: slot (plane, cell) sits at
+//   (cell / TILE) * 27 * TILE + plane * TILE + cell % TILE.
+// A wave of 64 consecutive cells then reads and writes 27 runs of 64 entries
+// inside one few-KiB block (plus the blocks of its i+/j+/k+ neighbours), not
+// 27 runs spread over the whole level as with plain planes q * N + cell, which
+// keeps DRAM pages and translation entries per wave few.  The level array
+// holds 27 * padded_cells(N) entries; the padding cells are never touched.
+#ifndef LBM3_TILE_CELLS
+#define LBM3_TILE_CELLS 256
+#endif
+constexpr int LBM3_TILE = LBM3_TILE_CELLS;
+
+__host__ __device__ __forceinline__ size_t tile_base(int cell)
+{
+    return (size_t)(cell / LBM3_TILE) * (27 * LBM3_TILE) + (size_t)(cell % LBM3_TILE);
+}
+__host__ __device__ __forceinline__ size_t pop_index(int plane, int cell)
+{
+    return tile_base(cell) + (size_t)plane * LBM3_TILE;
+}
+__host__ __device__ __forceinline__ long long padded_cells(long long n)
+{
+    return (n + LBM3_TILE - 1) / LBM3_TILE * LBM3_TILE;
+}
+
+// a level whose population array does not fit 32-bit byte offsets
+template <class S> __host__ __device__ inline bool wide_level(long long n)
+{
+    return 27LL * padded_cells(n) * (long long)sizeof(S) > 0xFFFFFFFFLL;
+}
+
 template <class V> __device__ __forceinline__ V ld(const V *p)
 {
 #if LBM3_LOAD_NT
