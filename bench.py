@@ -122,6 +122,23 @@ def counters_workload():
     print("counters workload: 3 copies of 1 GiB", flush=True)
 
 
+def serving(n, steps_scale, storages, seed=0):
+    """Serving workload (continuous batching vs one-at-a-time solves) on the
+    same seeded Poisson arrival sequence; one entry per storage type."""
+    from lbm3_amd import serving as sv
+    out = {"requests": n, "seed": seed, "steps_scale": steps_scale}
+    for st in storages:
+        reqs = sv.make_requests(n=n, seed=seed, steps_scale=steps_scale)
+        served = sv.run_served(reqs, storage=st)
+        print("serving %s: served %s" % (st, served), file=sys.stderr, flush=True)
+        seq = sv.run_sequential(reqs, storage=st)
+        print("serving %s: sequential %s" % (st, seq), file=sys.stderr, flush=True)
+        out[st] = {"served": {k: round(v, 4) for k, v in served.items()},
+                   "sequential": {k: round(v, 4) for k, v in seq.items()},
+                   "throughput_ratio": round(served["mlups"] / seq["mlups"], 3)}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -131,6 +148,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16")))
+    ap.add_argument("--serving", choices=["quick", "full", "none"], default="quick",
+                    help="serving workload: quick (32 requests, FP32), full (256 requests, "
+                         "500-4000 steps, FP32 and FP16), none")
     ap.add_argument("--counters-workload", action="store_true",
                     help="run only the short sequence that the HBM byte counters are collected on")
     args = ap.parse_args()
@@ -177,6 +197,11 @@ def main():
     for run in [main_run] + [v for k, v in variants.items() if k != "k17"]:
         run["roofline"]["traffic"] = traffic_from_profiles(run["storage"])
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_threads)
+    serve = None
+    if args.serving == "quick":
+        serve = serving(32, 1.0, ["f32"])
+    elif args.serving == "full":
+        serve = serving(256, 1.0, ["f32", "f16"])
     line = {
         "metric": "MLUPS (sphere 256x128x128, FP32 population storage)",
         "value": round(main_run["mlups"], 1),
@@ -206,6 +231,7 @@ def main():
                          "roofline": v["roofline"],
                          "speedup_vs_f32": round(v["mlups"] / main_run["mlups"], 3)} for k, v in variants.items()},
         "accuracy": accuracy,
+        "serving": serve,
         "copy_probe_gbs": round(2 * copy_bytes / (copy_ms / 1e3) / 1e9, 1),
         "device": {"name": name, "hbm_bytes": mem, "cus": cus},
     }

@@ -162,3 +162,15 @@ def test_server_rejects_multilevel_and_unfinished_queries():
     with pytest.raises(lib.LBM3Error):
         srv.macros(rid, 10)          # macros were not kept
     srv.close()
+
+
+def test_serving_runners_complete_a_small_mix():
+    """The seeded workload through the served and the one-at-a-time runners:
+    every request finishes and both count the same fluid updates."""
+    from lbm3_amd import serving
+    reqs = serving.make_requests(n=6, seed=3, steps_scale=0.02)
+    a = serving.run_served(reqs, "f32")
+    b = serving.run_sequential(reqs, "f32")
+    assert a["requests"] == b["requests"] == 6
+    assert a["p99_s"] >= a["p50_s"] > 0 and b["p99_s"] >= b["p50_s"] > 0
+    assert abs(a["mlups"] * a["makespan_s"] / (b["mlups"] * b["makespan_s"]) - 1.0) < 1e-9
