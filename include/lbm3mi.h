@@ -113,11 +113,38 @@ int lbm3_download_populations(lbm3_sim *s, int level, void *out, long long bytes
 int lbm3_upload_populations(lbm3_sim *s, int level, const void *in, long long bytes);
 /* Named built arrays (shifter, rowI.., wallMap, wallIndex, wallData, linkLength,
  * periodicIndex, periodicSource, openIndex, openSource, openMeta, rayHitScan,
- * rayMap, f2cIndex, ...).  "cellRow" is the per-step kernels' cell record, 8
+ * rayMap, f2cIndex, ..., rotor<r>Map/Index/Interp/Force).  "cellRow" is the per-step kernels' cell record, 8
  * bytes per cell: {kind, row} with kind >= 0 wall-adjacent entry, -1/-2 bulk,
- * -3 solid, <= -4 open cell updated by its own kernel.  With out == NULL only
+ * -3 solid, -4 a cell the rotors can reach, <= -8 open cell (the last two
+ * updated by kernels of their own).  With out == NULL only
  * *count is returned. */
 int lbm3_download_array(lbm3_sim *s, int level, const char *name, void *out, long long capacity, long long *count);
+
+/* buildRotors (rotorFunctions.h:315-585, called by buildGrids for the levels
+ * with Info.useRotors, gridBuilderFunctions.h:1612-1619): every level whose bit
+ * is set in levelMask gets all nRotors rotor STLs (9 floats per triangle,
+ * triCounts[r] triangles each), voxelized on the GPU at half the level spacing,
+ * with 4x4x4 blocks of interpolation cells marked where solid voxels are near
+ * and the 8 corner counts of each interpolation cell packed 4 bits each.
+ * Replaces the level's rotors; a later build of the grids drops them. */
+int lbm3_build_rotors(lbm3_sim *s, const float *triangles, const int *triCounts, int nRotors, int levelMask);
+/* The case's writes to grids[level].rotors[r].Info after buildGrids
+ * (BruteforcePump.cu:226-229): angular velocity [rad/s], axis LBM3_AXIS_X/Y/Z
+ * (rotateAlongX/Y/Z) or LBM3_AXIS_NONE, and a point (ox, oy, oz) [mm] of the
+ * axis.  Each updateSingleGrid advances the angle by dtPhys * radiansPerSecond
+ * (updateGrid.h:36-49) and forces the cells inside the rotor toward rigid
+ * rotation (processRotor, rotorFunctions.h:210-304). */
+int lbm3_set_rotor(lbm3_sim *s, int level, int rotor, float radiansPerSecond, int axis, float ox, float oy, float oz);
+/* RotorInfoStruct (types.h:206-211): out = {res, xMin, yMin, zMin, xMax, yMax,
+ * zMax, cellCountX, cellCountY, cellCountZ, radiansElapsed, marked blocks}. */
+int lbm3_rotor_info(lbm3_sim *s, int level, int rotor, double *out12);
+/* Rotor force tracker sums (trackerFunctions.h:225-330) with FP64
+ * accumulation, before the division by the iteration span and the unit
+ * conversion: out = {Fx, Fy, Fz, Tx, Ty, Tz} in the rotor frame, the torque
+ * about the rotation point with the force cells' centres as lever arms [mm].
+ * Accumulated while the case tracks rotor force (trackRotorForce); reset by
+ * lbm3_reset_wall_forces. */
+int lbm3_rotor_forces(lbm3_sim *s, int level, int rotor, double *out6);
 
 /* FP64 observables. */
 /* kinetic energy sum over counted cells (taylorGreenVortexMain.cu:116-180, lattice units,
@@ -126,7 +153,7 @@ int lbm3_kinetic_energy(lbm3_sim *s, double *out);
 /* per-body sums of the accumulated momentum-exchange force and its torque about
  * `origin` (trackerFunctions.h:356-474, FP64): out[body*6 + {Fx,Fy,Fz,Tx,Ty,Tz}] */
 int lbm3_wall_forces(lbm3_sim *s, int level, const double *origin, int nBodies, double *out);
-int lbm3_reset_wall_forces(lbm3_sim *s);
+int lbm3_reset_wall_forces(lbm3_sim *s);   /* wall and rotor force accumulators */
 
 /* applyCollision (applyCollision.h:17) on n independent cells, 27 shifted
  * populations each (cell-major), on the device. */

@@ -27,6 +27,13 @@ extern "C" {
 #define LBM3_MAX_LEVELS 8
 #define LBM3_MAX_RULES 16
 #define LBM3_MAX_WALLS 31
+#define LBM3_MAX_ROTORS 8        /* rotor STLs per level (rotorSTLs, buildGrids) */
+
+/* rotation axis of a rotor (RotorInfoStruct rotateAlongX/Y/Z, types.h:206-211) */
+#define LBM3_AXIS_NONE (-1)
+#define LBM3_AXIS_X 0
+#define LBM3_AXIS_Y 1
+#define LBM3_AXIS_Z 2
 
 typedef struct lbm3_bounds {
     float xMin, yMin, zMin, xMax, yMax, zMax;

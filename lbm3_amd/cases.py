@@ -162,6 +162,24 @@ def sphere(nx=256, ny=128, nz=128, diameter=32.0, center=(64.0, 64.0, 64.0), u_i
     return c, bounds(0, 0, 0, nx, ny, nz), [tri]
 
 
+def periodic_box(n=24, nu=0.05, u=(0.0, 0.0, 0.0), limiter=0.0):
+    """Fully periodic cube of n^3 active cells of 1 mm (one ghost layer per
+    face), centred on the origin, uniform initial state, lattice viscosity nu."""
+    res = f32(1.0)
+    c = _case(1, res, f32(1.0), f32(1.0))
+    c.nuPhys = float(f32(nu) * (res / f32(1000.0)) * (res / f32(1000.0)))
+    c.trackWallForce = 0
+    c.init.kind = T.INIT_UNIFORM
+    c.init.ux, c.init.uy, c.init.uz = [float(f32(v)) for v in u]
+    add_open_rule(c, periodic=1)
+    if limiter is not None:
+        add_local_rule(c, limiter=limiter)
+    h = f32(0.5) * f32(n)
+    eps = f32(1e-6) * f32(n)
+    lo, hi = -h - res - eps, h + res + eps
+    return c, bounds(lo, lo, lo, hi, hi, hi), []
+
+
 def channel(nx=64, ny=32, nz=32, force=1e-6, nu=0.1, limiter=0.0, periodic_z=False):
     """Body-force driven duct: periodic in x, no-slip (moment-based u=0) on y and z faces
     (with periodic_z: periodic in z too, i.e. plane Poiseuille flow between the y walls).

@@ -1601,9 +1601,155 @@ int gpu_build(Sim &sim, const float *tri, const int *triCounts, int nBodies, lbm
     return walls_and_grid(sim, G, stl, nBodies, dcase.p, sc, fluidUpdates);
 }
 
+// ---------------------------------------------------------------------------
+// buildRotors (rotorFunctions.h:315-585): each rotor STL voxelized at half the
+// level spacing; 4x4x4 blocks of interpolation cells marked where their 6^3
+// voxel neighbourhood holds a solid voxel; per interpolation cell the 8 corner
+// counts (solid voxels among the 2x2x2 around each corner) packed 4 bits each
+// ---------------------------------------------------------------------------
+namespace {
+
+struct RotorGrid {
+    int VX, VY, bx, by;
+    const long long *scan;
+    const int *map;
+    // voxel (i, j, k) inside: a hit pair [kStart, kEnd) of ray (i, j) holds k
+    // (rotorFunctions.h:384-404; pairs read only while both entries exist)
+    __device__ bool solid(int i, int j, int k) const
+    {
+        if (i < 0 || i >= VX || j < 0 || j >= VY) return false;
+        const int ray = VX * j + i;
+        const long long e = scan[ray + 1];
+        for (long long h = scan[ray]; h + 1 < e; h += 2) {
+            if (map[h] > k) break;
+            if (map[h + 1] > k) return true;
+        }
+        return false;
+    }
+};
+
+int build_rotor(Sim &sim, Level &l, const float *tri, int n, Level::Rotor &Rt, Scratch &sc, hipStream_t s)
+{
+    DevSTL S;
+    if (const int rc = stl_upload(S, tri, n, sim.C.resGlobal, s)) return rc;
+    lbm3_info V;
+    memset(&V, 0, sizeof V);
+    V.ox = S.B.xMin; V.oy = S.B.yMin; V.oz = S.B.zMin;
+    V.res = l.info.res * 0.5f;
+    V.cellCountX = (int)((S.B.xMax - S.B.xMin) / V.res) + 2;
+    V.cellCountY = (int)((S.B.yMax - S.B.yMin) / V.res) + 2;
+    V.cellCountZ = (int)((S.B.zMax - S.B.zMin) / V.res) + 2;
+    RayMap R;
+    if (const int rc = voxelize(R, S, V, sc, s)) return rc;
+    RotorDev &d = Rt.dev;
+    d = RotorDev{};
+    d.axis = LBM3_AXIS_NONE;
+    d.res = V.res;
+    d.xMin = V.ox - 1.5f * V.res;
+    d.yMin = V.oy - 1.5f * V.res;
+    d.zMin = V.oz - 1.5f * V.res;
+    d.cx = ((V.cellCountX + 2 + 3) / 4) * 4;
+    d.cy = ((V.cellCountY + 2 + 3) / 4) * 4;
+    d.cz = ((V.cellCountZ + 2 + 3) / 4) * 4;
+    d.xMax = d.xMin + d.res * (float)d.cx;
+    d.yMax = d.yMin + d.res * (float)d.cy;
+    d.zMax = d.zMin + d.res * (float)d.cz;
+    const int bx = d.cx / 4, by = d.cy / 4, bz = d.cz / 4;
+    const long long blocks = (long long)bx * by * bz;
+    if (blocks > (1LL << 30)) return set_error(LBM3_E_MEMORY, "rotor interpolation grid too large");
+    Rt.blockCount = (int)blocks;
+    const RotorGrid G{V.cellCountX, V.cellCountY, bx, by, R.scan.p, R.map.p};
+    DBuf<uint8_t> mark;
+    BCHK(mark.alloc(blocks > 0 ? blocks : 1));
+    uint8_t *mk = mark.p;
+    for_n(blocks, s, [=] __device__(long long b) {
+        const int kb = int(b / (G.bx * G.by)), jb = int((b % (G.bx * G.by)) / G.bx), ib = int(b % G.bx);
+        uint8_t hit = 0;
+        for (int j = jb * 4 - 2; j < jb * 4 + 4 && !hit; ++j)
+            for (int i = ib * 4 - 2; i < ib * 4 + 4 && !hit; ++i)
+                for (int k = kb * 4 - 2; k < kb * 4 + 4; ++k)
+                    if (G.solid(i, j, k)) { hit = 1; break; }
+        mk[b] = hit;
+    });
+    const int nz = compact(sc, mark.p, blocks, Rt.index, s);
+    if (nz < 0) return nz;
+    d.nonZero = nz;
+    BCHK(Rt.rotorMap.alloc(blocks > 0 ? blocks : 1));
+    int *rm = Rt.rotorMap.p;
+    const int *ix = Rt.index.p;
+    for_n(blocks, s, [=] __device__(long long b) { rm[b] = -1; });
+    for_n(nz, s, [=] __device__(long long x) { rm[ix[x]] = (int)x; });
+    BCHK(Rt.interp.alloc(64LL * (nz > 0 ? nz : 1)));
+    uint32_t *ip = Rt.interp.p;
+    for_n(64LL * nz, s, [=] __device__(long long t) {
+        const int x = int(t / 64), q = int(t % 64);
+        const int b = ix[x];
+        const int kb = b / (G.bx * G.by), jb = (b % (G.bx * G.by)) / G.bx, ib = b % G.bx;
+        const int ki = q / 16, ji = (q / 4) % 4, ii = q % 4;
+        uint32_t packed = 0u;
+        int corner = 0;
+        for (int kc = ki; kc <= ki + 1; ++kc)
+            for (int jc = ji; jc <= ji + 1; ++jc)
+                for (int ic = ii; ic <= ii + 1; ++ic) {
+                    uint32_t count = 0u;
+                    for (int km = kc; km <= kc + 1; ++km)
+                        for (int jm = jc; jm <= jc + 1; ++jm)
+                            for (int im = ic; im <= ic + 1; ++im)
+                                count += G.solid(ib * 4 - 2 + im, jb * 4 - 2 + jm, kb * 4 - 2 + km) ? 1u : 0u;
+                    packed |= count << (4u * (unsigned)corner++);
+                }
+        ip[t] = packed;
+    });
+    BCHK(Rt.g.alloc(81LL * (nz > 0 ? nz : 1)));
+    BCHK(hipMemsetAsync(Rt.g.p, 0, sizeof(double) * (size_t)Rt.g.n, s));
+    d.rotorMap = Rt.rotorMap.p;
+    d.interp = Rt.interp.p;
+    d.g = Rt.g.p;
+    BCHK(hipStreamSynchronize(s));
+    return 0;
+}
+
+} // namespace
+
+int upload_rotors(Sim &sim, int level);   // capi.hip
+
+int gpu_build_rotors(Sim &sim, const float *tri, const int *triCounts, int nRotors, int levelMask)
+{
+    Scratch sc;
+    const hipStream_t s = sim.stream;
+    for (int L = 0; L < sim.levels; ++L) {
+        Level &l = sim.L[L];
+        l.rotors.clear();
+        l.info.useRotors = (levelMask >> L) & 1;
+        if (!l.info.useRotors || nRotors == 0) {
+            if (const int rc = upload_rotors(sim, L)) return rc;
+            continue;
+        }
+        l.rotors.resize(nRotors);
+        long long off = 0;
+        for (int r = 0; r < nRotors; ++r) {
+            if (const int rc = build_rotor(sim, l, tri + 9 * off, triCounts[r], l.rotors[r], sc, s)) return rc;
+            off += triCounts[r];
+        }
+        if (const int rc = upload_rotors(sim, L)) return rc;
+    }
+    return 0;
+}
+
 } // namespace lbm3
 
 using namespace lbm3;
+
+extern "C" int lbm3_build_rotors(lbm3_sim *h, const float *triangles, const int *triCounts, int nRotors, int levelMask)
+{
+    if (!h || nRotors < 0 || nRotors > LBM3_MAX_ROTORS || (nRotors > 0 && (!triangles || !triCounts)))
+        return set_error(LBM3_E_ARG, "bad rotor arguments");
+    for (int L = 0; L < h->s.levels; ++L)
+        if (((levelMask >> L) & 1) && !h->s.L[L].built) return set_error(LBM3_E_STATE, "rotors need the level built first");
+    for (int r = 0; r < nRotors; ++r)
+        if (triCounts[r] <= 0) return set_error(LBM3_E_ARG, "a rotor needs triangles");
+    return gpu_build_rotors(h->s, triangles, triCounts, nRotors, levelMask);
+}
 
 extern "C" int lbm3_build_grids(lbm3_sim *h, const float *triangles, const int *triCounts, int nBodies, lbm3_bounds domain,
                                 long long *fluidUpdatesPerIteration)

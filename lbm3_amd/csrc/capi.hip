@@ -1,7 +1,9 @@
 // C ABI of the hot path (include/lbm3mi.h): simulation objects, level import,
 // the per-step driver with 2:1 sub-cycling, observables and timing.
+#include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <vector>
 #include <mutex>
 #include <new>
 #include "sim.hpp"
@@ -77,7 +79,10 @@ int finish_level_setup(Sim &s, int level)
     if (upload(l.wallBC, wb, 32, s.stream)) return LBM3_E_HIP;
     HIPCHK(l.cellRow.alloc(l.N > 0 ? l.N : 1));
     HIPCHK(l.rowTab.alloc(l.nRows > 0 ? l.nRows : 1));
-    launch_fast_topo(topo_of(l), l.wallMap.p, nullptr, l.cellRow.p, l.rowTab.p, s.stream);
+    launch_fast_topo(topo_of(l), l.wallMap.p, nullptr, nullptr, l.cellRow.p, l.rowTab.p, s.stream);
+    l.rotors.clear();              // rotors belong to the grid they were built on
+    l.rotorDev.release();
+    l.rotorCellCount = 0;
     l.openFusionChecked = false;
     l.fusedOpen = false;
     l.openSettled = true;
@@ -100,43 +105,91 @@ int finish_level_setup(Sim &s, int level)
 
 
 
-// Decide whether level L defers its open boundary condition to the next step's
-// open-cell kernel (which then loads, completes, collides and streams those
-// cells, beside the bulk kernel): only level 0, and only when no open cell is
-// wall-adjacent and no periodic copy or interface transfer touches an open cell
-// (those passes read or write whole cells and need the condition applied in the
-// same step).
+static int settle_open(Sim &s, int L);
+
+// Decide which special cells of level L get kernels of their own and rebuild
+// the per-cell records the bulk kernel skips them by:
+// * the cells a rotor can reach at some angle (record -4, k_rotor_cells);
+// * on level 0, the open-boundary cells (record -(8 + open index),
+//   k_open_cells, which applies the boundary condition when it next loads the
+//   cell), unless an open cell is wall-adjacent or reachable by a rotor, or a
+//   periodic copy or an interface transfer touches one (those passes read or
+//   write whole cells and need the condition applied in the same step).
 static int prepare_open_fusion(Sim &s, int L)
 {
     Level &l = s.L[L];
+    if (const int rc = settle_open(s, L)) return rc;   // a condition pending under the old records
     l.openFusionChecked = true;
     l.fusedOpen = false;
     l.openSettled = true;
-    if (L != 0 || l.nOpen <= 0) return 0;
-    DBuf<int> openOf, flag;
-    HIPCHK(openOf.alloc(l.N));
-    HIPCHK(flag.alloc(1));
-    HIPCHK(hipMemsetAsync(openOf.p, 0xFF, sizeof(int) * (size_t)l.N, s.stream));
-    HIPCHK(hipMemsetAsync(flag.p, 0, sizeof(int), s.stream));
-    launch_scatter_index(l.openCell.p, l.nOpen, openOf.p, s.stream);
-    const FastTopo t{l.N, l.cellRow.p, l.rowTab.p};
-    launch_open_conflicts(l.openCell.p, l.nOpen, 3, openOf.p, l.wallMap.p, t, flag.p, s.stream);
-    launch_open_conflicts(l.perIndex.p, l.nPeriodic, 0, openOf.p, l.wallMap.p, t, flag.p, s.stream);
-    launch_open_conflicts(l.perSource.p, l.nPeriodic, 0, openOf.p, l.wallMap.p, t, flag.p, s.stream);
-    launch_open_conflicts(l.F2C.index.p, l.F2C.interfaceCount, 0, openOf.p, l.wallMap.p, t, flag.p, s.stream);
-    launch_open_conflicts(l.C2F.index.p, l.C2F.interfaceCount, 2, openOf.p, l.wallMap.p, t, flag.p, s.stream);
-    launch_open_conflicts(l.C2F.leftoverParent.p, l.C2F.leftoverCount, 0, openOf.p, l.wallMap.p, t, flag.p, s.stream);
-    launch_open_conflicts(l.C2F.leftoverNbrI.p, l.C2F.leftoverCount, 1, openOf.p, l.wallMap.p, t, flag.p, s.stream);
-    launch_open_conflicts(l.C2F.leftoverNbrJ.p, l.C2F.leftoverCount, 1, openOf.p, l.wallMap.p, t, flag.p, s.stream);
-    launch_open_conflicts(l.C2F.leftoverNbrK.p, l.C2F.leftoverCount, 1, openOf.p, l.wallMap.p, t, flag.p, s.stream);
-    int conflicts = 0;
-    HIPCHK(hipMemcpyAsync(&conflicts, flag.p, sizeof(int), hipMemcpyDeviceToHost, s.stream));
-    HIPCHK(hipStreamSynchronize(s.stream));
-    if (conflicts) return 0;
-    launch_fast_topo(topo_of(l), l.wallMap.p, openOf.p, l.cellRow.p, l.rowTab.p, s.stream);
+    l.rotorCellCount = 0;
+    const uint8_t *cand = nullptr;
+    if (!l.rotors.empty()) {
+        RotorReachSet R{};
+        R.n = (int)l.rotors.size();
+        for (int r = 0; r < R.n; ++r) {
+            const RotorDev &d = l.rotors[r].dev;
+            R.r[r] = RotorReach{{d.xMin, d.yMin, d.zMin}, {d.xMax, d.yMax, d.zMax}, {d.ox, d.oy, d.oz}, d.axis};
+        }
+        HIPCHK(l.rotorCand.alloc(l.N > 0 ? l.N : 1));
+        launch_rotor_reach(topo_of(l), l.geom, R, l.rotorCand.p, s.stream);
+        cand = l.rotorCand.p;
+    }
+    DBuf<int> openOf;
+    bool fuse = false;
+    if (L == 0 && l.nOpen > 0) {
+        DBuf<int> flag;
+        HIPCHK(openOf.alloc(l.N));
+        HIPCHK(flag.alloc(1));
+        HIPCHK(hipMemsetAsync(openOf.p, 0xFF, sizeof(int) * (size_t)l.N, s.stream));
+        HIPCHK(hipMemsetAsync(flag.p, 0, sizeof(int), s.stream));
+        launch_scatter_index(l.openCell.p, l.nOpen, openOf.p, s.stream);
+        const FastTopo t{l.N, l.cellRow.p, l.rowTab.p};
+        const int *wm = l.wallMap.p;
+        int *fl = flag.p;
+        launch_open_conflicts(l.openCell.p, l.nOpen, 3, openOf.p, wm, cand, t, fl, s.stream);
+        if (cand) launch_open_conflicts(l.openCell.p, l.nOpen, 4, openOf.p, wm, cand, t, fl, s.stream);
+        launch_open_conflicts(l.perIndex.p, l.nPeriodic, 0, openOf.p, wm, cand, t, fl, s.stream);
+        launch_open_conflicts(l.perSource.p, l.nPeriodic, 0, openOf.p, wm, cand, t, fl, s.stream);
+        launch_open_conflicts(l.F2C.index.p, l.F2C.interfaceCount, 0, openOf.p, wm, cand, t, fl, s.stream);
+        launch_open_conflicts(l.C2F.index.p, l.C2F.interfaceCount, 2, openOf.p, wm, cand, t, fl, s.stream);
+        launch_open_conflicts(l.C2F.leftoverParent.p, l.C2F.leftoverCount, 0, openOf.p, wm, cand, t, fl, s.stream);
+        launch_open_conflicts(l.C2F.leftoverNbrI.p, l.C2F.leftoverCount, 1, openOf.p, wm, cand, t, fl, s.stream);
+        launch_open_conflicts(l.C2F.leftoverNbrJ.p, l.C2F.leftoverCount, 1, openOf.p, wm, cand, t, fl, s.stream);
+        launch_open_conflicts(l.C2F.leftoverNbrK.p, l.C2F.leftoverCount, 1, openOf.p, wm, cand, t, fl, s.stream);
+        int conflicts = 0;
+        HIPCHK(hipMemcpyAsync(&conflicts, flag.p, sizeof(int), hipMemcpyDeviceToHost, s.stream));
+        HIPCHK(hipStreamSynchronize(s.stream));
+        fuse = conflicts == 0;
+    }
+    launch_fast_topo(topo_of(l), l.wallMap.p, fuse ? openOf.p : nullptr, cand, l.cellRow.p, l.rowTab.p, s.stream);
     HIPCHK(hipGetLastError());
+    if (cand) {
+        // the cells with record -4, in cell order (coalesced rows)
+        std::vector<int2> rec((size_t)l.N);
+        HIPCHK(hipMemcpyAsync(rec.data(), l.cellRow.p, sizeof(int2) * rec.size(), hipMemcpyDeviceToHost, s.stream));
+        HIPCHK(hipStreamSynchronize(s.stream));
+        std::vector<int> cells;
+        for (int c = 0; c < l.N; ++c)
+            if (rec[(size_t)c].x == -4) cells.push_back(c);
+        l.rotorCellCount = (int)cells.size();
+        if (upload(l.rotorCells, cells.data(), (long long)cells.size(), s.stream)) return LBM3_E_HIP;
+    }
     HIPCHK(hipStreamSynchronize(s.stream));
-    l.fusedOpen = true;
+    l.fusedOpen = fuse;
+    return 0;
+}
+
+// rotor descriptors of a level to the device; the cell records follow at the next run
+int upload_rotors(Sim &s, int level)
+{
+    Level &l = s.L[level];
+    std::vector<RotorDev> h;
+    for (const Level::Rotor &r : l.rotors) h.push_back(r.dev);
+    if (h.empty()) l.rotorDev.release();
+    else if (upload(l.rotorDev, h.data(), (long long)h.size(), s.stream)) return LBM3_E_HIP;
+    HIPCHK(hipStreamSynchronize(s.stream));
+    l.openFusionChecked = false;
     return 0;
 }
 
@@ -174,7 +227,30 @@ template <class S> static void step_level(Sim &s, int level)
         (void)hipEventRecord(s.ev[s.evUsed][0], s.stream);
     }
     StepArgs<S> a = step_args<S>(s, l);
-    a.open = FusedOpen<typename StoreTraits<S>::compute>{};
+    if (!l.rotors.empty()) {
+        // advance the rotors (updateGrid.h:36-49); the frame's cosine and sine of
+        // this step once per rotor, in the compute type
+        using T = typename StoreTraits<S>::compute;
+        constexpr double twoPi = 6.283185307179586476925286766559;
+        for (size_t r = 0; r < l.rotors.size(); ++r) {
+            Level::Rotor &Rt = l.rotors[r];
+            Rt.radiansElapsed = std::fmod(Rt.radiansElapsed + (double)l.info.dtPhys * (double)Rt.dev.radiansPerSecond, twoPi);
+            if (Rt.radiansElapsed < 0.0) Rt.radiansElapsed += twoPi;
+            if (Rt.radiansElapsed >= twoPi) Rt.radiansElapsed = 0.0;
+            if (sizeof(T) == sizeof(double)) {
+                a.rotorCS[2 * r] = std::cos(Rt.radiansElapsed);
+                a.rotorCS[2 * r + 1] = std::sin(Rt.radiansElapsed);
+            } else {
+                a.rotorCS[2 * r] = (double)std::cos((float)Rt.radiansElapsed);
+                a.rotorCS[2 * r + 1] = (double)std::sin((float)Rt.radiansElapsed);
+            }
+        }
+        a.rotors = l.rotorDev.p;
+        a.nRotors = (int)l.rotors.size();
+        a.trackRotor = s.C.trackRotorForce;
+        a.rotorCells = l.rotorCells.p;
+        a.rotorCellCount = l.rotorCellCount;
+    }
     if (l.fusedOpen) {
         // open cells have their own kernel, which first applies the previous
         // step's boundary condition when it is still pending
@@ -182,12 +258,13 @@ template <class S> static void step_level(Sim &s, int level)
         a.open = {o.cell, o.count, l.openSettled ? 0 : 1, o.meta, o.records, o.dRhoRef, o.uNormalRef, o.dRhoCum,
                   o.uNormalCum, o.iterationsFinished};
     }
-    if (timed || (l.W == 0 && a.open.count == 0)) {
+    if (timed || (l.W == 0 && a.open.count == 0 && a.rotorCellCount == 0)) {
         // in order on one stream, so that each kernel's events bracket it alone
         launch_collide_stream<S>(a, flip, s.stream);
         if (timed) (void)hipEventRecord(s.ev[s.evUsed][1], s.stream);
         launch_collide_wall<S>(a, flip, s.stream);
         launch_open_cells<S>(a, flip, s.stream);
+        launch_rotor_cells<S>(a, flip, s.stream);
         if (timed) {
             (void)hipEventRecord(s.ev[s.evUsed][2], s.stream);
             s.evUsed++;
@@ -199,6 +276,7 @@ template <class S> static void step_level(Sim &s, int level)
         (void)hipStreamWaitEvent(s.aux, s.fork, 0);
         launch_collide_wall<S>(a, flip, s.aux);
         launch_open_cells<S>(a, flip, s.aux);
+        launch_rotor_cells<S>(a, flip, s.aux);
         (void)hipEventRecord(s.join, s.aux);
         launch_collide_stream<S>(a, flip, s.stream);
         (void)hipStreamWaitEvent(s.stream, s.join, 0);
@@ -672,6 +750,16 @@ int lbm3_download_array(lbm3_sim *h, int level, const char *name, void *out, lon
             memcpy(out, l.openCount.data(), sizeof(int) * (size_t)l.nOpenIDs);
         }
         return 0;
+    } else if (!strncmp(name, "rotor", 5) && name[5] >= '0' && name[5] <= '9') {
+        const int r = name[5] - '0';
+        if (r >= (int)l.rotors.size()) return set_error(LBM3_E_ARG, "no such rotor");
+        Level::Rotor &R = l.rotors[r];
+        const char *f = name + 6;
+        if (!strcmp(f, "Map")) { src = R.rotorMap.p; n = R.blockCount; }
+        else if (!strcmp(f, "Index")) { src = R.index.p; n = R.dev.nonZero; }
+        else if (!strcmp(f, "Interp")) { src = R.interp.p; n = 64LL * R.dev.nonZero; }
+        else if (!strcmp(f, "Force")) { src = R.g.p; n = 81LL * R.dev.nonZero; es = 8; }
+        else return set_error(LBM3_E_ARG, "unknown rotor array");
     } else if (!strcmp(name, "openRefs")) {
         if (l.built)
             if (const int rc = settle_open(s, level)) return rc;
@@ -742,9 +830,84 @@ int lbm3_wall_forces(lbm3_sim *h, int level, const double *origin, int nBodies, 
 int lbm3_reset_wall_forces(lbm3_sim *h)
 {
     if (!h) return set_error(LBM3_E_ARG, "null sim");
-    for (int level = 0; level < h->s.levels; ++level)
-        if (h->s.L[level].gWall.p)
-            HIPCHK(hipMemsetAsync(h->s.L[level].gWall.p, 0, sizeof(double) * (size_t)h->s.L[level].gWall.n, h->s.stream));
+    for (int level = 0; level < h->s.levels; ++level) {
+        Level &l = h->s.L[level];
+        if (l.gWall.p) HIPCHK(hipMemsetAsync(l.gWall.p, 0, sizeof(double) * (size_t)l.gWall.n, h->s.stream));
+        for (Level::Rotor &r : l.rotors)
+            if (r.g.p) HIPCHK(hipMemsetAsync(r.g.p, 0, sizeof(double) * (size_t)r.g.n, h->s.stream));
+    }
+    return 0;
+}
+
+static int check_rotor(lbm3_sim *h, int level, int rotor)
+{
+    if (!h) return set_error(LBM3_E_ARG, "null sim");
+    if (const int rc = check_level(h->s, level)) return rc;
+    if (rotor < 0 || rotor >= (int)h->s.L[level].rotors.size()) return set_error(LBM3_E_ARG, "no such rotor on this level");
+    return 0;
+}
+
+int lbm3_set_rotor(lbm3_sim *h, int level, int rotor, float radiansPerSecond, int axis, float ox, float oy, float oz)
+{
+    if (const int rc = check_rotor(h, level, rotor)) return rc;
+    if (axis < LBM3_AXIS_NONE || axis > LBM3_AXIS_Z) return set_error(LBM3_E_ARG, "bad rotor axis");
+    RotorDev &d = h->s.L[level].rotors[rotor].dev;
+    d.radiansPerSecond = radiansPerSecond;
+    d.axis = axis;
+    d.ox = ox; d.oy = oy; d.oz = oz;
+    return upload_rotors(h->s, level);
+}
+
+int lbm3_rotor_info(lbm3_sim *h, int level, int rotor, double *out)
+{
+    if (const int rc = check_rotor(h, level, rotor)) return rc;
+    if (!out) return set_error(LBM3_E_ARG, "null argument");
+    const Level::Rotor &R = h->s.L[level].rotors[rotor];
+    const RotorDev &d = R.dev;
+    const double v[12] = {d.res, d.xMin, d.yMin, d.zMin, d.xMax, d.yMax, d.zMax, (double)d.cx, (double)d.cy, (double)d.cz,
+                          R.radiansElapsed, (double)d.nonZero};
+    memcpy(out, v, sizeof v);
+    return 0;
+}
+
+int lbm3_rotor_forces(lbm3_sim *h, int level, int rotor, double *out)
+{
+    if (const int rc = check_rotor(h, level, rotor)) return rc;
+    if (!out) return set_error(LBM3_E_ARG, "null argument");
+    Sim &s = h->s;
+    const Level::Rotor &R = s.L[level].rotors[rotor];
+    const RotorDev &d = R.dev;
+    for (int i = 0; i < 6; ++i) out[i] = 0.0;
+    const int nz = d.nonZero;
+    if (nz == 0) return 0;
+    std::vector<double> g((size_t)81 * nz);
+    std::vector<int> index((size_t)nz);
+    HIPCHK(hipMemcpyAsync(g.data(), R.g.p, sizeof(double) * g.size(), hipMemcpyDeviceToHost, s.stream));
+    HIPCHK(hipMemcpyAsync(index.data(), R.index.p, sizeof(int) * index.size(), hipMemcpyDeviceToHost, s.stream));
+    HIPCHK(hipStreamSynchronize(s.stream));
+    // force cells of 3x3x3 per block, their centres as lever arms about the
+    // rotation point (trackerFunctions.h:244-297), FP64 sums
+    const float resBlock = d.res * 4.f, resForce = resBlock / 3.f;
+    const int bx = d.cx / 4, bxy = bx * (d.cy / 4);
+    const size_t nf = (size_t)nz * 27;
+    for (int x = 0; x < nz; ++x) {
+        const int b = index[(size_t)x];
+        const int kb = b / bxy, jb = (b % bxy) / bx, ib = (b % bxy) % bx;
+        const float x0 = (float)ib * resBlock + d.xMin - d.ox + 0.5f * resForce;
+        const float y0 = (float)jb * resBlock + d.yMin - d.oy + 0.5f * resForce;
+        const float z0 = (float)kb * resBlock + d.zMin - d.oz + 0.5f * resForce;
+        for (int kF = 0; kF < 3; ++kF)
+            for (int jF = 0; jF < 3; ++jF)
+                for (int iF = 0; iF < 3; ++iF) {
+                    const double px = x0 + (float)iF * resForce, py = y0 + (float)jF * resForce, pz = z0 + (float)kF * resForce;
+                    const size_t fi = (size_t)x * 27 + (size_t)(kF * 9 + jF * 3 + iF);
+                    const double gx = g[fi], gy = g[nf + fi], gz = g[2 * nf + fi];
+                    out[0] += gx; out[1] += gy; out[2] += gz;
+                    out[3] += py * gz - pz * gy;
+                    out[4] += pz * gx - px * gz;
+                    out[5] += px * gy - py * gx;
+                }
+    }
     return 0;
 }
 

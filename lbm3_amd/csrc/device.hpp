@@ -6,6 +6,7 @@
 #include "cumulant.hpp"
 #include "boundary.hpp"
 #include "storage.hpp"
+#include "../../include/lbm3mi_types.h"
 
 namespace lbm3 {
 
@@ -102,6 +103,31 @@ template <class T> struct FusedOpen {
     int iter;                     // iteration the pending boundary condition belongs to
 };
 
+// One rotor of a level (RotorStruct, types.h:206-219) as the kernels read it;
+// the angle enters as the step's cosine and sine (StepArgs::rotorCS).
+struct RotorDev {
+    float res;                    // interpolation cell: half the level spacing
+    float xMin, yMin, zMin, xMax, yMax, zMax;
+    int cx, cy, cz;               // interpolation cells (multiples of 4)
+    float ox, oy, oz;             // a point of the rotation axis
+    int axis;                     // LBM3_AXIS_*
+    float radiansPerSecond;
+    int nonZero;
+    const int *rotorMap;          // [cx/4 * cy/4 * cz/4] compact block index or -1
+    const uint32_t *interp;       // [nonZero * 64] packed corner counts
+    double *g;                    // [3][nonZero * 27] forcing per force cell (FP64)
+};
+
+// what a rotor can reach: its interpolation bounds, axis and a point of it
+struct RotorReach {
+    double lo[3], hi[3], o[3];
+    int axis;
+};
+struct RotorReachSet {
+    RotorReach r[LBM3_MAX_ROTORS];
+    int n;
+};
+
 template <class S> struct StepArgs {
     S *f;
     Topology topo;
@@ -118,6 +144,14 @@ template <class S> struct StepArgs {
     LevelGeom geom;
     Codec cd;                     // 16-bit population encoding
     FusedOpen<typename StoreTraits<S>::compute> open;
+    // rotors of the level: every kernel that collides a cell the rotors can
+    // reach applies their forcing (nRotors == 0: none)
+    const RotorDev *rotors;
+    int nRotors;
+    int trackRotor;               // TRACK_ROTOR_FORCE
+    const int *rotorCells;        // [rotorCellCount] cells with their own kernel (record -4)
+    int rotorCellCount;
+    double rotorCS[2 * LBM3_MAX_ROTORS];   // cos, sin of each rotor's angle this step, in the compute type
 };
 
 // Open boundary cells of one level, all BCIDs concatenated (BCID order, then cell order)

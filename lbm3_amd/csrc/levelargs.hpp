@@ -26,6 +26,13 @@ template <class S> inline StepArgs<S> step_args(Sim &s, Level &l)
     a.wallBC = l.wallBC.p;
     a.geom = l.geom;
     a.cd = s.codec;
+    a.open = FusedOpen<typename StoreTraits<S>::compute>{};
+    a.rotors = nullptr;
+    a.nRotors = 0;
+    a.trackRotor = 0;
+    a.rotorCells = nullptr;
+    a.rotorCellCount = 0;
+    for (double &v : a.rotorCS) v = 0.0;
     return a;
 }
 

@@ -76,6 +76,21 @@ struct Level {
     LocalBC bulk{};
     DBuf<LocalBC> wallBC;            // [32]
     DBuf<unsigned char> f;           // [27][N] storage type
+    // rotors of the level (buildRotors), device arrays and host-side motion
+    struct Rotor {
+        RotorDev dev{};              // arrays point into the buffers below
+        double radiansElapsed = 0.0;
+        int blockCount = 0;
+        DBuf<int> rotorMap, index;
+        DBuf<uint32_t> interp;
+        DBuf<double> g;
+    };
+    std::vector<Rotor> rotors;
+    DBuf<RotorDev> rotorDev;          // [rotors.size()]
+    bool rotorChecked = true;         // candidate cells and records match the rotors' motion
+    DBuf<int> rotorCells;             // cells the rotors can reach that have their own kernel
+    int rotorCellCount = 0;
+    DBuf<uint8_t> rotorCand;          // [N] 1 where a rotor can reach the cell
     // voxelizer output kept for flag checks
     DBuf<long long> rayHitScan;
     DBuf<int> rayMap;

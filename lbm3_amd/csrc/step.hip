@@ -46,7 +46,7 @@ __device__ __forceinline__ void store_pre(S *f, int N, const int (&nb)[8], const
 // this step), `codes` the cell's seven words of 8-bit link lengths in link order.
 template <class T>
 __device__ __forceinline__ void bounce_back(T (&f)[27], const T (&pre)[27], uint32_t wd, bool useIBB,
-                                            const uint32_t (&codes)[7], const float (&wu)[3], double (&g3)[3])
+                                            const uint32_t (&codes)[7], const T (&wu)[3], double (&g3)[3])
 {
     T dRho, ux_, uy_, uz_;
     moments(f, dRho, ux_, uy_, uz_);
@@ -153,6 +153,101 @@ template <class S, class Off> __device__ __forceinline__ S *at(S *base, Off off)
     return reinterpret_cast<S *>(reinterpret_cast<char *>(base) + off);
 }
 
+// ---------------------------------------------------------------------------
+// rotors (rotorFunctions.h): immersed forcing toward rigid rotation inside a
+// voxelized rotor, evaluated in the rotor's frame
+// ---------------------------------------------------------------------------
+
+// getRotorFraction (rotorFunctions.h:155-208, projectXYZIntoRotorFrame :8-35,
+// interpolateRotorCube :135-153): interpolation cell of the point in the rotor
+// frame and its solid fraction; false outside every marked block
+template <class T>
+__device__ __forceinline__ bool rotor_locate(const RotorDev &R, T c, T s, T x, T y, T z, int &rm, int (&ijk)[3], T (&w)[3],
+                                             T &fr)
+{
+    T xr = x, yr = y, zr = z;
+    const T dx = x - T(R.ox), dy = y - T(R.oy), dz = z - T(R.oz);
+    if (R.axis == LBM3_AXIS_X) { yr = T(R.oy) + c * dy + s * dz; zr = T(R.oz) - s * dy + c * dz; }
+    else if (R.axis == LBM3_AXIS_Y) { xr = T(R.ox) + c * dx - s * dz; zr = T(R.oz) + s * dx + c * dz; }
+    else if (R.axis == LBM3_AXIS_Z) { xr = T(R.ox) + c * dx + s * dy; yr = T(R.oy) - s * dx + c * dy; }
+    if (xr < T(R.xMin) || xr >= T(R.xMax) || yr < T(R.yMin) || yr >= T(R.yMax) || zr < T(R.zMin) || zr >= T(R.zMax))
+        return false;
+    const T xRel = xr - T(R.xMin), yRel = yr - T(R.yMin), zRel = zr - T(R.zMin);
+    const T res = T(R.res);
+    const int ii = int(xRel / res), ji = int(yRel / res), ki = int(zRel / res);
+    if (ii >= R.cx || ji >= R.cy || ki >= R.cz) return false;
+    const int bx = R.cx / 4, by = R.cy / 4;
+    rm = R.rotorMap[(ki / 4) * bx * by + (ji / 4) * bx + ii / 4];
+    if (rm < 0) return false;
+    const uint32_t packed = R.interp[rm * 64 + (ki % 4) * 16 + (ji % 4) * 4 + (ii % 4)];
+    w[0] = xRel / res - T(ii);
+    w[1] = yRel / res - T(ji);
+    w[2] = zRel / res - T(ki);
+    T v[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) v[q] = T((packed >> (4u * q)) & 0xFu);
+    const T v00 = v[0] + w[0] * (v[1] - v[0]), v10 = v[2] + w[0] * (v[3] - v[2]);
+    const T v01 = v[4] + w[0] * (v[5] - v[4]), v11 = v[6] + w[0] * (v[7] - v[6]);
+    const T v0 = v00 + w[1] * (v10 - v00), v1 = v01 + w[1] * (v11 - v01);
+    fr = (v0 + w[2] * (v1 - v0)) * T(0.125f);
+    ijk[0] = ii; ijk[1] = ji; ijk[2] = ki;
+    return true;
+}
+
+// the rotor loop of the cell update (updateGrid.h:98-129) with processRotor
+// (rotorFunctions.h:210-304): each rotor adds its smoothstep share of the
+// forcing rho (u_target - u_pre) to the cell's force, the cumulative share is
+// capped at 1 and scales the collision limiter down; the forcing is tracked,
+// turned into the rotor frame, per force cell in FP64
+template <class S, class T>
+__device__ __forceinline__ void rotors_apply(const StepArgs<S> &a, int cell, T dRho, T ux, T uy, T uz, Relax<T> &R,
+                                             bool track)
+{
+    int nbx[8], ijk[3];
+    decode(a.topo, cell, nbx, ijk);
+    const T x = T(float(ijk[0]) * a.geom.res + a.geom.ox), y = T(float(ijk[1]) * a.geom.res + a.geom.oy),
+            z = T(float(ijk[2]) * a.geom.res + a.geom.oz);
+    const T rho = dRho + T(1), rhoInv = T(1) / rho;
+    const T uxPre = (ux * rho + R.gx) * rhoInv, uyPre = (uy * rho + R.gy) * rhoInv, uzPre = (uz * rho + R.gz) * rhoInv;
+    T cum = T(0);
+    for (int r = 0; r < a.nRotors; ++r) {
+        const RotorDev &Rt = a.rotors[r];
+        const T c = T(a.rotorCS[2 * r]), s = T(a.rotorCS[2 * r + 1]);
+        int rm, cI[3];
+        T w[3], fr;
+        if (!rotor_locate<T>(Rt, c, s, x, y, z, rm, cI, w, fr) || fr == T(0)) continue;
+        const T dx = x - T(Rt.ox), dy = y - T(Rt.oy), dz = z - T(Rt.oz);
+        const T scale = T(Rt.radiansPerSecond) * T(a.geom.dtPhys) / T(a.geom.res);
+        T ut0 = T(0), ut1 = T(0), ut2 = T(0);
+        if (Rt.axis == LBM3_AXIS_X) { ut1 = -scale * dz; ut2 = scale * dy; }
+        else if (Rt.axis == LBM3_AXIS_Y) { ut0 = scale * dz; ut2 = -scale * dx; }
+        else if (Rt.axis == LBM3_AXIS_Z) { ut0 = -scale * dy; ut1 = scale * dx; }
+        T gx = rho * (ut0 - uxPre), gy = rho * (ut1 - uyPre), gz = rho * (ut2 - uzPre);
+        T ff = fr * fr * (T(3) - T(2) * fr);
+        if (cum + ff > T(1)) { ff = T(1) - cum; cum = T(1); }
+        else cum += ff;
+        gx *= ff; gy *= ff; gz *= ff;
+        R.gx += gx; R.gy += gy; R.gz += gz;
+        if (track) {
+            if (Rt.axis == LBM3_AXIS_X) { const T t = gy; gy = c * t + s * gz; gz = -s * t + c * gz; }
+            else if (Rt.axis == LBM3_AXIS_Y) { const T t = gx; gx = c * t - s * gz; gz = s * t + c * gz; }
+            else if (Rt.axis == LBM3_AXIS_Z) { const T t = gx; gx = c * t + s * gy; gy = -s * t + c * gy; }
+            const int iF = min(2, max(0, int(T(0.75f) * (T(cI[0] % 4) + w[0]))));
+            const int jF = min(2, max(0, int(T(0.75f) * (T(cI[1] % 4) + w[1]))));
+            const int kF = min(2, max(0, int(T(0.75f) * (T(cI[2] % 4) + w[2]))));
+            const size_t fi = (size_t)rm * 27 + (size_t)(kF * 9 + jF * 3 + iF), nf = (size_t)Rt.nonZero * 27;
+            atomicAdd(Rt.g + fi, double(gx));
+            atomicAdd(Rt.g + nf + fi, double(gy));
+            atomicAdd(Rt.g + 2 * nf + fi, double(gz));
+        }
+        if (cum >= T(1)) break;
+    }
+    if (cum > T(0)) {
+        R.lambda *= T(1) - (cum < T(0) ? T(0) : (cum > T(1) ? T(1) : cum));
+        R.k17 = R.k17 && R.lambda > T(0);
+    }
+}
+
 // Cells are independent under esotwist streaming (each one reads and writes
 // only its own 27 slots), so the level is updated by kernels over disjoint cell
 // sets: one over all cells that skips the special ones, one over the short list
@@ -163,7 +258,8 @@ template <class S, class Off> __device__ __forceinline__ S *at(S *base, Off off)
 //
 // BULK: collision variant of the cells that are not next to a wall, fixed for
 // the level on the host (1: K17, 2: K15).  Topology record >= 0: wall-adjacent,
-// -3: solid, <= -4: open cell with its own kernel.
+// -3: solid, -4: a cell the rotors can reach, <= -8: open cell (each with its
+// own kernel).
 template <class S, bool FLIP, int BULK, class Off>
 __device__ __forceinline__ void bulk_cell(const StepArgs<S> &a, int cell)
 {
@@ -224,6 +320,31 @@ __global__ __launch_bounds__(256) void k_open_cells(StepArgs<S> a)
         StoreTraits<S>::store(a.f + pop_index(post_plane(q, FLIP), nb[post_nb(q)]), f[q], q, a.cd);
 }
 
+// cells the rotors can reach (never wall-adjacent or open; record -4): the
+// bulk update with the rotors' forcing and the scaled limiter per cell
+template <class S, bool FLIP>
+__global__ __launch_bounds__(256) void k_rotor_cells(StepArgs<S> a)
+{
+    using T = typename StoreTraits<S>::compute;
+    const int x = blockIdx.x * 256 + threadIdx.x;
+    if (x >= a.rotorCellCount) return;
+    const int cell = a.rotorCells[x];
+    int nb[8];
+    decode_fast(a.fast, cell, a.fast.cellRow[cell].y, nb);
+    T f[27];
+#pragma unroll
+    for (int q = 0; q < 27; ++q) f[q] = StoreTraits<S>::load(a.f + pop_index(pre_plane(q, FLIP), nb[pre_nb(q)]), q, a.cd);
+    T dRho, ux, uy, uz;
+    moments(f, dRho, ux, uy, uz);
+    Relax<T> R = relax_of<T>(a.bulk);
+    const int wm = a.wallMap[cell];
+    rotors_apply<S, T>(a, cell, dRho, ux, uy, uz, R, a.trackRotor && wm != -2);
+    collide<T, 0>(f, R);
+#pragma unroll
+    for (int q = 0; q < 27; ++q)
+        StoreTraits<S>::store(a.f + pop_index(post_plane(q, FLIP), nb[post_nb(q)]), f[q], q, a.cd);
+}
+
 // wall-adjacent cells: collision with the wall id's local BC, interpolated
 // bounceback of the links that cross the wall, momentum-exchange force
 template <class S, bool FLIP>
@@ -242,12 +363,14 @@ __device__ __forceinline__ void wall_cell(const StepArgs<S> &a, int wm)
 #pragma unroll
     for (int k = 0; k < 7; ++k) codes[k] = a.linkLength[(size_t)k * (size_t)a.W + wm];
     const LocalBC &b = a.wallBC[(wd >> 27) & 31u];
-    float wu[3] = {b.ux, b.uy, b.uz};
+    // wall velocity in the compute type: a wall without a velocity rule moves
+    // with the fluid's own (pre-collision) velocity
+    T wu[3] = {T(b.ux), T(b.uy), T(b.uz)};
     const int velMode = b.velMode;
     if (velMode == 0) {
         T d, u0, u1, u2;
         moments(f, d, u0, u1, u2);
-        wu[0] = float(u0); wu[1] = float(u1); wu[2] = float(u2);
+        wu[0] = u0; wu[1] = u1; wu[2] = u2;
     } else if (velMode == 2) {
         int nbx[8], ijk[3];
         decode(a.topo, cell, nbx, ijk);
@@ -255,11 +378,16 @@ __device__ __forceinline__ void wall_cell(const StepArgs<S> &a, int wm)
         const float y = float(ijk[1]) * a.geom.res + a.geom.oy - b.origin[1];
         const float z = float(ijk[2]) * a.geom.res + a.geom.oz - b.origin[2];
         const float s = a.geom.dtPhys / a.geom.res;
-        wu[0] = (b.omega[1] * z - b.omega[2] * y) * s;
-        wu[1] = (b.omega[2] * x - b.omega[0] * z) * s;
-        wu[2] = (b.omega[0] * y - b.omega[1] * x) * s;
+        wu[0] = T((b.omega[1] * z - b.omega[2] * y) * s);
+        wu[1] = T((b.omega[2] * x - b.omega[0] * z) * s);
+        wu[2] = T((b.omega[0] * y - b.omega[1] * x) * s);
     }
-    const Relax<T> R = relax_of<T>(b);
+    Relax<T> R = relax_of<T>(b);
+    if (a.nRotors > 0) {
+        T d, u0, u1, u2;
+        moments(f, d, u0, u1, u2);
+        rotors_apply<S, T>(a, cell, d, u0, u1, u2, R, a.trackRotor && !(wd & 1u));
+    }
     T pre[27];
 #pragma unroll
     for (int q = 0; q < 27; ++q) pre[q] = f[q];
@@ -301,16 +429,19 @@ __global__ __launch_bounds__(256) void k_collide_wall(StepArgs<S> a)
 }
 
 // derive the hot-loop topology records from the compressed IJK/NBR arrays;
-// with openOf, the open-boundary cells get -(4 + open-list index) in place of
+// with rotorCand, the fluid cells the rotors can reach get -4, and with openOf
+// the open-boundary cells get -(8 + open-list index), in place of
 // their wall-map code (they are never wall-adjacent when this is used)
-__global__ __launch_bounds__(256) void k_fast_topo(Topology t, const int *wallMap, const int *openOf, int2 *cellRow, int4 *rows)
+__global__ __launch_bounds__(256) void k_fast_topo(Topology t, const int *wallMap, const int *openOf, const uint8_t *rotorCand,
+                                                   int2 *cellRow, int4 *rows)
 {
     const int cell = blockIdx.x * 256 + threadIdx.x;
     if (cell >= t.N) return;
     const int shift = t.shifter[cell];
     const int row = shift >= 0 ? shift : t.shifter[cell + shift];
     int code = wallMap[cell];
-    if (openOf && openOf[cell] >= 0) code = -(4 + openOf[cell]);
+    if (rotorCand && rotorCand[cell] && code >= -2 && code < 0) code = -4;
+    if (openOf && openOf[cell] >= 0) code = -(8 + openOf[cell]);
     cellRow[cell] = make_int2(code, row);
     if (shift >= 0) rows[row] = make_int4(cell, t.rjp[row], t.rkp[row], t.rjkp[row]);
 }
@@ -318,7 +449,7 @@ __global__ __launch_bounds__(256) void k_fast_topo(Topology t, const int *wallMa
 // cells a later pass of the step reads or writes as a whole (periodic copies,
 // interface stencils) must not carry a boundary condition that is still pending
 __global__ __launch_bounds__(256) void k_open_conflicts(const int *cells, int count, int mode, const int *openOf,
-                                                        const int *wallMap, FastTopo t, int *flag)
+                                                        const int *wallMap, const uint8_t *cand, FastTopo t, int *flag)
 {
     const int x = blockIdx.x * 256 + threadIdx.x;
     if (x >= count) return;
@@ -330,8 +461,47 @@ __global__ __launch_bounds__(256) void k_open_conflicts(const int *cells, int co
         int nb[8];
         decode_fast(t, c, t.cellRow[c].y, nb);
         for (int k = 0; k < 8; ++k) hit = hit || openOf[nb[k]] >= 0;
-    } else hit = wallMap[c] >= 0;                                          // open cell next to a wall
+    } else if (mode == 3) hit = wallMap[c] >= 0;                           // open cell next to a wall
+    else hit = cand && cand[c];                                            // open cell a rotor can reach
     if (hit) atomicAdd(flag, 1);
+}
+
+// cells a rotor can reach at some angle: within its bounds' distance from the
+// axis (plus two level spacings) and inside its axial extent (the same margin)
+__global__ __launch_bounds__(256) void k_rotor_reach(Topology t, LevelGeom g, RotorReachSet R, uint8_t *cand)
+{
+    const int cell = blockIdx.x * 256 + threadIdx.x;
+    if (cell >= t.N) return;
+    int nb[8], ijk[3];
+    decode(t, cell, nb, ijk);
+    const double p[3] = {double(float(ijk[0]) * g.res + g.ox), double(float(ijk[1]) * g.res + g.oy),
+                         double(float(ijk[2]) * g.res + g.oz)};
+    const double m = 2.0 * double(g.res);
+    uint8_t hit = 0;
+    for (int r = 0; r < R.n && !hit; ++r) {
+        const RotorReach &q = R.r[r];
+        const double lo[3] = {q.lo[0] - m, q.lo[1] - m, q.lo[2] - m}, hi[3] = {q.hi[0] + m, q.hi[1] + m, q.hi[2] + m};
+        if (q.axis < 0) {
+            hit = p[0] >= lo[0] && p[0] <= hi[0] && p[1] >= lo[1] && p[1] <= hi[1] && p[2] >= lo[2] && p[2] <= hi[2];
+            continue;
+        }
+        const int a = q.axis, b = (a + 1) % 3, c = (a + 2) % 3;
+        if (p[a] < lo[a] || p[a] > hi[a]) continue;
+        double rmax2 = 0.0;
+        for (int k = 0; k < 4; ++k) {
+            const double db = ((k & 1) ? q.hi[b] : q.lo[b]) - q.o[b], dc = ((k & 2) ? q.hi[c] : q.lo[c]) - q.o[c];
+            rmax2 = fmax(rmax2, db * db + dc * dc);
+        }
+        const double db = p[b] - q.o[b], dc = p[c] - q.o[c];
+        hit = sqrt(db * db + dc * dc) <= sqrt(rmax2) + m;
+    }
+    cand[cell] = hit;
+}
+
+void launch_rotor_reach(const Topology &t, const LevelGeom &g, const RotorReachSet &R, uint8_t *cand, hipStream_t s)
+{
+    if (t.N <= 0) return;
+    hipLaunchKernelGGL(k_rotor_reach, dim3((unsigned)((t.N + 255) / 256)), dim3(256), 0, s, t, g, R, cand);
 }
 
 __global__ __launch_bounds__(256) void k_scatter_index(const int *cell, int count, int *openOf)
@@ -677,6 +847,14 @@ void launch_collide_stream(const StepArgs<S> &a, bool flip, hipStream_t s)
 }
 
 template <class S>
+void launch_rotor_cells(const StepArgs<S> &a, bool flip, hipStream_t s)
+{
+    if (a.rotorCellCount <= 0) return;
+    if (flip) hipLaunchKernelGGL((k_rotor_cells<S, true>), dim3(blocks_for(a.rotorCellCount)), dim3(256), 0, s, a);
+    else hipLaunchKernelGGL((k_rotor_cells<S, false>), dim3(blocks_for(a.rotorCellCount)), dim3(256), 0, s, a);
+}
+
+template <class S>
 void launch_open_cells(const StepArgs<S> &a, bool flip, hipStream_t s)
 {
     if (a.open.count <= 0) return;
@@ -692,17 +870,19 @@ void launch_open_nb(const Topology &t, int count, const int *cell, int4 *out, hi
     hipLaunchKernelGGL(k_open_nb, dim3(blocks_for(count)), dim3(256), 0, s, t, count, cell, out);
 }
 
-void launch_fast_topo(const Topology &t, const int *wallMap, const int *openOf, int2 *cellRow, int4 *rows, hipStream_t s)
+void launch_fast_topo(const Topology &t, const int *wallMap, const int *openOf, const uint8_t *rotorCand, int2 *cellRow,
+                      int4 *rows, hipStream_t s)
 {
     if (t.N <= 0) return;
-    hipLaunchKernelGGL(k_fast_topo, dim3(blocks_for(t.N)), dim3(256), 0, s, t, wallMap, openOf, cellRow, rows);
+    hipLaunchKernelGGL(k_fast_topo, dim3(blocks_for(t.N)), dim3(256), 0, s, t, wallMap, openOf, rotorCand, cellRow, rows);
 }
 
-void launch_open_conflicts(const int *cells, int count, int mode, const int *openOf, const int *wallMap, const FastTopo &t,
-                           int *flag, hipStream_t s)
+void launch_open_conflicts(const int *cells, int count, int mode, const int *openOf, const int *wallMap,
+                           const uint8_t *cand, const FastTopo &t, int *flag, hipStream_t s)
 {
     if (count <= 0) return;
-    hipLaunchKernelGGL(k_open_conflicts, dim3(blocks_for(count)), dim3(256), 0, s, cells, count, mode, openOf, wallMap, t, flag);
+    hipLaunchKernelGGL(k_open_conflicts, dim3(blocks_for(count)), dim3(256), 0, s, cells, count, mode, openOf, wallMap, cand,
+                       t, flag);
 }
 
 void launch_scatter_index(const int *cell, int count, int *openOf, hipStream_t s)
@@ -794,6 +974,7 @@ void launch_multi_step(const MultiDesc<S> *D, const int *starts, int n, const in
     template void launch_collide_stream<S>(const StepArgs<S> &, bool, hipStream_t);                                 \
     template void launch_collide_wall<S>(const StepArgs<S> &, bool, hipStream_t);                                   \
     template void launch_open_cells<S>(const StepArgs<S> &, bool, hipStream_t);                                     \
+    template void launch_rotor_cells<S>(const StepArgs<S> &, bool, hipStream_t);                                    \
     template void launch_multi_step<S>(const MultiDesc<S> *, const int *, int, const int (&)[4], hipStream_t);       \
     template void launch_open<S>(const OpenArgs<S> &, bool, hipStream_t);                                           \
     template void launch_periodic<S>(S *, const Topology &, int, const int *, const int *, bool, hipStream_t);        \

@@ -17,6 +17,7 @@ struct InitDesc {
 // bulk cells (all but wall-adjacent and solid) and wall-adjacent cells; together one collide-stream step
 template <class S> void launch_collide_stream(const StepArgs<S> &a, bool flip, hipStream_t s);
 template <class S> void launch_open_cells(const StepArgs<S> &a, bool flip, hipStream_t s);
+template <class S> void launch_rotor_cells(const StepArgs<S> &a, bool flip, hipStream_t s);
 template <class S> void launch_collide_wall(const StepArgs<S> &a, bool flip, hipStream_t s);
 
 // one time step of many single-level domains in four launches (bulk, wall,
@@ -44,11 +45,13 @@ void launch_wall_force(const double *gWall, int W, const uint32_t *wallData, con
                        const LevelGeom &g, const double origin[3], int nBodies, double *partial, hipStream_t s);
 template <class T> void launch_collide_batch(T *f, int n, const Relax<T> &R, hipStream_t s);
 void launch_open_nb(const Topology &t, int count, const int *cell, int4 *out, hipStream_t s);
-void launch_fast_topo(const Topology &t, const int *wallMap, const int *openOf, int2 *cellRow, int4 *rows, hipStream_t s);
+void launch_fast_topo(const Topology &t, const int *wallMap, const int *openOf, const uint8_t *rotorCand, int2 *cellRow,
+                      int4 *rows, hipStream_t s);
 void launch_scatter_index(const int *cell, int count, int *openOf, hipStream_t s);
+void launch_rotor_reach(const Topology &t, const LevelGeom &g, const RotorReachSet &R, uint8_t *cand, hipStream_t s);
 // mode 0: listed cells, 1: signed cells (-c-1), 2: 2x2x2 stencils from their first cell, 3: cells next to a wall
-void launch_open_conflicts(const int *cells, int count, int mode, const int *openOf, const int *wallMap, const FastTopo &t,
-                           int *flag, hipStream_t s);
+void launch_open_conflicts(const int *cells, int count, int mode, const int *openOf, const int *wallMap,
+                           const uint8_t *cand, const FastTopo &t, int *flag, hipStream_t s);
 void launch_copy(const void *a, void *b, long long bytes, hipStream_t s);
 void launch_relayout(const void *src, void *dst, int N, int elemBytes, bool toPlanes, hipStream_t s);
 

@@ -64,6 +64,10 @@ class RefineRule(C.Structure):
                [(n, C.c_float) for n in ("xMin", "yMin", "zMin", "xMax", "yMax", "zMax")]
 
 
+MAX_ROTORS = 8
+AXIS_NONE, AXIS_X, AXIS_Y, AXIS_Z = -1, 0, 1, 2
+
+
 class Case(C.Structure):
     _fields_ = [("gridLevelCount", C.c_int), ("wallRefinementCount", C.c_int),
                 ("resGlobal", C.c_float), ("dtPhysGlobal", C.c_float), ("nuPhys", C.c_float), ("rhoPhys", C.c_float),

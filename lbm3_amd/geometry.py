@@ -43,6 +43,70 @@ def box(lo, hi):
     return np.array([np.concatenate([v[a], v[b], v[c]]) for a, b, c in faces], np.float32)
 
 
+def _axis_frame(axis):
+    """(a, b, c): the axis index and the two in-plane indices, right-handed."""
+    return axis, (axis + 1) % 3, (axis + 2) % 3
+
+
+def rotor(n_blades=4, radius=10.0, hub_ratio=0.2, twist_deg=25.0, length=6.0, center=(0.0, 0.0, 0.0), axis=2,
+          n_theta=1024, n_axial=48, sharpness=24.0):
+    """Closed N-blade rotor: a star-shaped section r(theta) = r_hub + (R - r_hub)
+    |cos(N theta / 2)|^p (N blades of hub radius hub_ratio * R), extruded
+    along `axis` over `length` with the section turned by twist_deg from one
+    end to the other.  2 * n_theta * n_axial side triangles plus two fans of
+    n_theta, outward orientation, every edge shared by two triangles."""
+    R, rh = float(radius), float(hub_ratio) * float(radius)
+    th = 2.0 * np.pi * np.arange(n_theta) / n_theta
+    r = rh + (R - rh) * np.abs(np.cos(0.5 * n_blades * th)) ** sharpness
+    z = -0.5 * length + length * np.arange(n_axial + 1) / n_axial
+    tw = np.deg2rad(twist_deg) * (z / length + 0.5)
+    a, b, c = _axis_frame(axis)
+    ring = np.zeros((n_axial + 1, n_theta, 3))
+    ring[..., a] = z[:, None] + center[a]
+    ring[..., b] = center[b] + r[None, :] * np.cos(th[None, :] + tw[:, None])
+    ring[..., c] = center[c] + r[None, :] * np.sin(th[None, :] + tw[:, None])
+    ring = ring.astype(np.float32)
+    ends = np.zeros((2, 3), np.float32)
+    ends[:, a] = (z[0] + center[a], z[-1] + center[a])
+    ends[:, b], ends[:, c] = center[b], center[c]
+    i = np.arange(n_theta)
+    i1 = (i + 1) % n_theta
+    tris = []
+    for k in range(n_axial):
+        p00, p01, p10, p11 = ring[k, i], ring[k, i1], ring[k + 1, i], ring[k + 1, i1]
+        tris.append(np.concatenate([p00, p01, p11], 1))
+        tris.append(np.concatenate([p00, p11, p10], 1))
+    lo = np.repeat(ends[:1], n_theta, 0)
+    hi = np.repeat(ends[1:], n_theta, 0)
+    tris.append(np.concatenate([lo, ring[0, i1], ring[0, i]], 1))
+    tris.append(np.concatenate([hi, ring[-1, i], ring[-1, i1]], 1))
+    return np.concatenate(tris).astype(np.float32)
+
+
+def tube(r_in, r_out, length, center=(0.0, 0.0, 0.0), axis=0, n_theta=256):
+    """Closed thick-walled cylinder (an annulus extruded along `axis`), the
+    duct wall around a rotor: 8 * n_theta triangles, outward orientation."""
+    a, b, c = _axis_frame(axis)
+    th = 2.0 * np.pi * np.arange(n_theta) / n_theta
+    i = np.arange(n_theta)
+    i1 = (i + 1) % n_theta
+
+    def ring(rad, zz):
+        p = np.zeros((n_theta, 3))
+        p[:, a] = center[a] + zz
+        p[:, b] = center[b] + rad * np.cos(th)
+        p[:, c] = center[c] + rad * np.sin(th)
+        return p.astype(np.float32)
+
+    z0, z1 = -0.5 * length, 0.5 * length
+    o0, o1, n0, n1 = ring(r_out, z0), ring(r_out, z1), ring(r_in, z0), ring(r_in, z1)
+    t = [np.concatenate([o0[i], o0[i1], o1[i1]], 1), np.concatenate([o0[i], o1[i1], o1[i]], 1),      # outer wall
+         np.concatenate([n0[i], n1[i1], n0[i1]], 1), np.concatenate([n0[i], n1[i], n1[i1]], 1),      # inner wall
+         np.concatenate([o1[i], o1[i1], n1[i1]], 1), np.concatenate([o1[i], n1[i1], n1[i]], 1),      # end z1
+         np.concatenate([o0[i], n0[i1], o0[i1]], 1), np.concatenate([o0[i], n0[i], n0[i1]], 1)]      # end z0
+    return np.concatenate(t).astype(np.float32)
+
+
 def write_binary_stl(path, tris):
     tris = np.asarray(tris, np.float32).reshape(-1, 9)
     with open(path, "wb") as fh:

@@ -53,6 +53,10 @@ def load():
         "lbm3_kinetic_energy": (i32, [vp, C.POINTER(f64)]),
         "lbm3_wall_forces": (i32, [vp, i32, C.POINTER(f64), i32, C.POINTER(f64)]),
         "lbm3_reset_wall_forces": (i32, [vp]),
+        "lbm3_build_rotors": (i32, [vp, vp, vp, i32, i32]),
+        "lbm3_set_rotor": (i32, [vp, i32, i32, C.c_float, i32, C.c_float, C.c_float, C.c_float]),
+        "lbm3_rotor_info": (i32, [vp, i32, i32, C.POINTER(f64)]),
+        "lbm3_rotor_forces": (i32, [vp, i32, i32, C.POINTER(f64)]),
         "lbm3_copy_probe": (i32, [i64, i32, C.POINTER(f64)]),
         "lbm3_server_create": (i32, [i64, i32, C.POINTER(vp)]),
         "lbm3_server_destroy": (i32, [vp]),
@@ -208,11 +212,13 @@ class Sim:
         n = max(cnt.value, 0)
         if name in ("rayHitScan", "cellRow"):
             dt = np.int64
+        elif name.startswith("rotor") and name.endswith("Force"):
+            dt = np.float64
         elif name in ("wallForce",):
             dt = np.float64
         elif name == "openRefs":
             dt = np.float64 if self.storage == "f64" else np.float32
-        elif name in _UINT_ARRAYS:
+        elif name in _UINT_ARRAYS or (name.startswith("rotor") and name.endswith("Interp")):
             dt = np.uint32
         else:
             dt = np.int32
@@ -223,11 +229,11 @@ class Sim:
 
     def cell_kinds(self, level=0):
         """Counts of the cells each per-step kernel updates on a level:
-        {"bulk", "wall", "open", "solid"} (open: cells with their own kernel;
+        {"bulk", "wall", "open", "rotor", "solid"} (open, rotor: cells with their own kernel;
         where the open boundary runs as a separate pass they count as bulk)."""
         code = (self.array("cellRow", level) & 0xFFFFFFFF).astype(np.uint32).view(np.int32)
         return {"bulk": int(((code == -1) | (code == -2)).sum()), "wall": int((code >= 0).sum()),
-                "open": int((code <= -4).sum()), "solid": int((code == -3).sum())}
+                "open": int((code <= -8).sum()), "rotor": int((code == -4).sum()), "solid": int((code == -3).sum())}
 
     def kinetic_energy(self):
         v = C.c_double()
@@ -239,6 +245,26 @@ class Sim:
         out = (C.c_double * (6 * bodies))()
         check(load().lbm3_wall_forces(self.h, level, o, bodies, out))
         return np.array(out[:]).reshape(bodies, 6)
+
+    def build_rotors(self, rotors, level_mask=0xFF):
+        """buildRotors on the levels of level_mask; rotors: list of (n, 9) triangle arrays."""
+        tris = [np.ascontiguousarray(t, np.float32).reshape(-1, 9) for t in rotors]
+        allt = np.concatenate(tris) if tris else np.zeros((1, 9), np.float32)
+        counts = np.array([len(t) for t in tris] or [0], np.int32)
+        check(load().lbm3_build_rotors(self.h, allt.ctypes.data, counts.ctypes.data, len(tris), int(level_mask)))
+
+    def set_rotor(self, level, rotor, radians_per_second, axis, origin=(0.0, 0.0, 0.0)):
+        check(load().lbm3_set_rotor(self.h, level, rotor, float(radians_per_second), int(axis), *[float(v) for v in origin]))
+
+    def rotor_info(self, level, rotor):
+        out = (C.c_double * 12)()
+        check(load().lbm3_rotor_info(self.h, level, rotor, out))
+        return np.array(out[:])
+
+    def rotor_forces(self, level, rotor):
+        out = (C.c_double * 6)()
+        check(load().lbm3_rotor_forces(self.h, level, rotor, out))
+        return np.array(out[:])
 
     def reset_wall_forces(self):
         check(load().lbm3_reset_wall_forces(self.h))

@@ -1389,3 +1389,148 @@ int orc_build_grids(orc_sim *Sm, const orc_stl *stls, int nStl, lbm3_bounds Doma
     free(G); free(V);
     return 0;
 }
+
+/* ------------------------------------------------------------------------
+ * Rotors: buildRotors, rotorFunctions.h:315-585
+ * ---------------------------------------------------------------------- */
+
+/* is voxel (i, j, k) inside: a hit pair [kStart, kEnd) of ray (i, j) holds k
+   (rotorFunctions.h:384-404; pairs are read only while both entries exist) */
+static int rotor_voxel(const orc_raymap *R, int X, int i, int j, int k)
+{
+    const int ray = X * j + i;
+    const long long end = R->hitScan[ray + 1];
+    for (long long h = R->hitScan[ray]; h + 1 < end; h += 2) {
+        if (R->rayMap[h] > k) break;
+        if (R->rayMap[h + 1] > k) return 1;
+    }
+    return 0;
+}
+
+static void rotor_free(orc_rotor *Rt)
+{
+    free(Rt->rotorMap); free(Rt->index); free(Rt->interp); free(Rt->g);
+    memset(Rt, 0, sizeof *Rt);
+}
+
+/* one rotor of one level, rotorFunctions.h:322-557 */
+static void build_rotor(orc_rotor *Rt, const orc_stl *S, const lbm3_info *G)
+{
+    lbm3_info V;
+    memset(&V, 0, sizeof V);
+    V.ox = S->Bounds.xMin; V.oy = S->Bounds.yMin; V.oz = S->Bounds.zMin;
+    V.res = G->res * 0.5f;
+    V.cellCountX = (int)((S->Bounds.xMax - S->Bounds.xMin) / V.res) + 2;
+    V.cellCountY = (int)((S->Bounds.yMax - S->Bounds.yMin) / V.res) + 2;
+    V.cellCountZ = (int)((S->Bounds.zMax - S->Bounds.zMin) / V.res) + 2;
+    orc_raymap R;
+    memset(&R, 0, sizeof R);
+    voxelize(&R, S, &V);
+    memset(Rt, 0, sizeof *Rt);
+    Rt->axis = LBM3_AXIS_NONE;
+    Rt->res = V.res;
+    Rt->B.xMin = V.ox - 1.5f * V.res;
+    Rt->B.yMin = V.oy - 1.5f * V.res;
+    Rt->B.zMin = V.oz - 1.5f * V.res;
+    Rt->cx = ((V.cellCountX + 2 + 3) / 4) * 4;
+    Rt->cy = ((V.cellCountY + 2 + 3) / 4) * 4;
+    Rt->cz = ((V.cellCountZ + 2 + 3) / 4) * 4;
+    Rt->B.xMax = Rt->B.xMin + Rt->res * (float)Rt->cx;
+    Rt->B.yMax = Rt->B.yMin + Rt->res * (float)Rt->cy;
+    Rt->B.zMax = Rt->B.zMin + Rt->res * (float)Rt->cz;
+    const int bx = Rt->cx / 4, by = Rt->cy / 4, bz = Rt->cz / 4, bxy = bx * by;
+    const int VX = V.cellCountX, VY = V.cellCountY;
+    Rt->blockCount = bx * by * bz;
+    /* 2) blocks holding any solid voxel of their 6^3 neighbourhood, :358-409 */
+    uint8_t *mark = NEWB(Rt->blockCount);
+    for (int b = 0; b < Rt->blockCount; ++b) {
+        const int kb = b / bxy, jb = (b % bxy) / bx, ib = (b % bxy) % bx;
+        const int i0 = imax(0, ib * 4 - 2), i1 = imin(VX, ib * 4 + 4);
+        const int j0 = imax(0, jb * 4 - 2), j1 = imin(VY, jb * 4 + 4);
+        for (int j = j0; j < j1 && !mark[b]; ++j)
+            for (int i = i0; i < i1 && !mark[b]; ++i)
+                for (int k = kb * 4 - 2; k < kb * 4 + 4; ++k)
+                    if (rotor_voxel(&R, VX, i, j, k)) { mark[b] = 1; break; }
+    }
+    /* 3) compact index of the marked blocks, :411-435 */
+    Rt->rotorMap = NEWI(Rt->blockCount);
+    int nz = 0;
+    for (int b = 0; b < Rt->blockCount; ++b) nz += mark[b];
+    Rt->nonZero = nz;
+    Rt->index = NEWI(nz);
+    for (int b = 0, run = 0; b < Rt->blockCount; ++b) {
+        if (mark[b]) { Rt->rotorMap[b] = run; Rt->index[run++] = b; }
+        else Rt->rotorMap[b] = -1;
+    }
+    /* 4) corner counts of every interpolation cell, :437-547 */
+    Rt->interp = (uint32_t *)calloc((size_t)(nz > 0 ? nz : 1) * 64, sizeof(uint32_t));
+    for (int x = 0; x < nz; ++x) {
+        const int b = Rt->index[x];
+        const int kb = b / bxy, jb = (b % bxy) / bx, ib = (b % bxy) % bx;
+        const int i0 = imax(0, ib * 4 - 2), i1 = imin(VX, ib * 4 + 4);
+        const int j0 = imax(0, jb * 4 - 2), j1 = imin(VY, jb * 4 + 4);
+        uint8_t vm[216];
+        memset(vm, 0, sizeof vm);
+        for (int j = j0; j < j1; ++j)
+            for (int i = i0; i < i1; ++i)
+                for (int k = kb * 4 - 2; k < kb * 4 + 4; ++k)
+                    if (rotor_voxel(&R, VX, i, j, k))
+                        vm[(k - (kb * 4 - 2)) * 36 + (j - (jb * 4 - 2)) * 6 + (i - (ib * 4 - 2))] = 1;
+        uint8_t cnt[125];
+        for (int kc = 0; kc < 5; ++kc)
+            for (int jc = 0; jc < 5; ++jc)
+                for (int ic = 0; ic < 5; ++ic) {
+                    int n = 0;
+                    for (int km = kc; km <= kc + 1; ++km)
+                        for (int jm = jc; jm <= jc + 1; ++jm)
+                            for (int im = ic; im <= ic + 1; ++im) n += vm[km * 36 + jm * 6 + im];
+                    cnt[kc * 25 + jc * 5 + ic] = (uint8_t)n;
+                }
+        for (int ki = 0; ki < 4; ++ki)
+            for (int ji = 0; ji < 4; ++ji)
+                for (int ii = 0; ii < 4; ++ii) {
+                    uint32_t packed = 0u;
+                    int corner = 0;
+                    for (int kc = ki; kc <= ki + 1; ++kc)
+                        for (int jc = ji; jc <= ji + 1; ++jc)
+                            for (int ic = ii; ic <= ii + 1; ++ic)
+                                packed |= (uint32_t)cnt[kc * 25 + jc * 5 + ic] << (4u * (unsigned)corner++);
+                    Rt->interp[x * 64 + ki * 16 + ji * 4 + ii] = packed;
+                }
+    }
+    Rt->g = (double *)calloc((size_t)(nz > 0 ? nz : 1) * 27 * 3, sizeof(double));
+    free(mark);
+    free(R.hitScan);
+    free(R.rayMap);
+}
+
+/* buildGrids' rotor pass, gridBuilderFunctions.h:1612-1619: every level whose
+   bit is set in levelMask (the reference's Info.useRotors) gets all rotors */
+int orc_build_rotors(orc_sim *S, const orc_stl *rotors, int nRotors, int levelMask)
+{
+    if (nRotors < 0 || nRotors > LBM3_MAX_ROTORS) return -1;
+    for (int L = 0; L < S->levels; ++L) {
+        orc_grid *g = &S->g[L];
+        for (int r = 0; r < g->nRotors; ++r) rotor_free(&g->rotors[r]);
+        free(g->rotors);
+        g->rotors = NULL;
+        g->nRotors = 0;
+        g->Info.useRotors = (levelMask >> L) & 1;
+        if (!g->Info.useRotors || nRotors == 0) continue;
+        g->rotors = (orc_rotor *)calloc((size_t)nRotors, sizeof(orc_rotor));
+        g->nRotors = nRotors;
+        for (int r = 0; r < nRotors; ++r) build_rotor(&g->rotors[r], &rotors[r], &g->Info);
+    }
+    return 0;
+}
+
+/* the case's direct writes to grids[level].rotors[r].Info (BruteforcePump.cu:226-229) */
+int orc_set_rotor(orc_sim *S, int level, int rotor, float radiansPerSecond, int axis, float ox, float oy, float oz)
+{
+    if (level < 0 || level >= S->levels || rotor < 0 || rotor >= S->g[level].nRotors) return -1;
+    orc_rotor *R = &S->g[level].rotors[rotor];
+    R->radiansPerSecond = radiansPerSecond;
+    R->axis = axis;
+    R->ox = ox; R->oy = oy; R->oz = oz;
+    return 0;
+}

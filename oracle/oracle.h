@@ -20,11 +20,18 @@
  *   MBBC / restore      tables equal the reference's generated code and
  *                       reproduce the prescribed moments; uniform flow kept
  *   voxelizer/builder   analytic box and sphere, link-code known answer
+ *   interface transfer  reconstruction inverts the stencil values, the
+ *                       interpolant reproduces its polynomial family, uniform
+ *                       flow kept across levels, 2-level creeping TGV decay
+ *                       (tests/test_oracle_interface_pins.py)
+ *   rotors              corner-count packing and trilinear fraction, a box
+ *                       rotor's blocks and counts by geometry, rigid-body
+ *                       forcing inside a fully solid rotor, angle advance
+ *                       (tests/test_oracle_rotor_pins.py)
  * Parity unpinned: the non-reflective characteristic update
  * (updateGrid.h:267-307) beyond the uniform-flow fixed point, the near-wall
  * layer of the moment-based walls and edge normals where ghost layers meet
- * wall faces, and orc_update_interface (the multilevel transfer is not
- * restated yet).
+ * wall faces.
  */
 #ifndef LBM3_ORACLE_H
 #define LBM3_ORACLE_H
@@ -81,6 +88,22 @@ typedef struct orc_openbc {
     void *dRhoRef, *uNormalRef, *dRhoCum, *uNormalCum; /* REAL[count] */
 } orc_openbc;
 
+/* RotorStruct / RotorInfoStruct, types.h:206-215 */
+typedef struct orc_rotor {
+    float res;                    /* half the level spacing */
+    lbm3_bounds B;                /* interpolation-grid bounds */
+    int cx, cy, cz;               /* interpolation cells, multiples of 4 */
+    float ox, oy, oz;             /* a point of the rotation axis */
+    int axis;                     /* LBM3_AXIS_* */
+    float radiansPerSecond;
+    double radiansElapsed;
+    int blockCount, nonZero;
+    int *rotorMap;                /* [blockCount] compact index or -1 */
+    int *index;                   /* [nonZero] block of each compact index */
+    uint32_t *interp;             /* [nonZero * 64] packed corner counts */
+    double *g;                    /* [3][nonZero * 27] forcing accumulated per force cell */
+} orc_rotor;
+
 typedef struct orc_grid {
     lbm3_info Info;
     /* compressed IJK/NBR, types.h:162-164 */
@@ -94,6 +117,7 @@ typedef struct orc_grid {
     int nPeriodic; int *perIndex, *perSource;
     int wallCount; /* cells marked wall by the builder */
     void *f; /* REAL[27][N], index q*N + cell */
+    int nRotors; struct orc_rotor *rotors;
 } orc_grid;
 
 typedef struct orc_sim {
@@ -121,6 +145,14 @@ void orc_update_all_grids(orc_sim *S, int level);
 void orc_update_single_grid(orc_sim *S, int level);
 void orc_update_interface(orc_sim *S, int coarse);
 void orc_sim_free(orc_sim *S);
+
+/* rotors (rotorFunctions.h) */
+int orc_build_rotors(orc_sim *S, const orc_stl *rotors, int nRotors, int levelMask);
+int orc_set_rotor(orc_sim *S, int level, int rotor, float radiansPerSecond, int axis, float ox, float oy, float oz);
+int orc_build_rotors_from_triangles(orc_sim *S, const float *tri, const int *triCounts, int nRotors, int levelMask);
+float orc_rotor_fraction_f32(const orc_sim *S, int L, int r, float angle, float x, float y, float z);
+int orc_rotor_info(const orc_sim *S, int L, int r, double *out12);
+int orc_rotor_forces(const orc_sim *S, int L, int r, double *out6);
 
 #ifdef __cplusplus
 }

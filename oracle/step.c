@@ -134,6 +134,17 @@ void *orc_array(orc_sim *S, int L, const char *name, long long *count, int *elem
 #undef B
         }
     }
+    if (!strncmp(name, "rotor", 5) && name[5] >= '0' && name[5] <= '9') {
+        const int r = name[5] - '0';
+        if (r < g->nRotors) {
+            orc_rotor *R = &g->rotors[r];
+            const char *f = name + 6;
+            if (!strcmp(f, "Map")) { *count = R->blockCount; return R->rotorMap; }
+            if (!strcmp(f, "Index")) { *count = R->nonZero; return R->index; }
+            if (!strcmp(f, "Interp")) { *count = 64LL * R->nonZero; return R->interp; }
+            if (!strcmp(f, "Force")) { *elemSize = 8; *count = 81LL * R->nonZero; return R->g; }
+        }
+    }
     if (!strcmp(name, "rayHitScan")) { *elemSize = 8; *count = S->rayTotal[L].nRays + 1; return S->rayTotal[L].hitScan; }
     if (!strcmp(name, "rayMap")) { *count = S->rayTotal[L].totalHitCount; return S->rayTotal[L].rayMap; }
 #undef A
@@ -196,6 +207,74 @@ int orc_build_from_triangles(orc_sim *S, const float *tri, const int *triCounts,
     return rc;
 }
 
+int orc_build_rotors_from_triangles(orc_sim *S, const float *tri, const int *triCounts, int nRotors, int levelMask)
+{
+    if (nRotors < 0 || nRotors > LBM3_MAX_ROTORS) return -1;
+    orc_stl stl[LBM3_MAX_ROTORS];
+    memset(stl, 0, sizeof stl);
+    int off = 0, rc = 0;
+    for (int r = 0; r < nRotors && !rc; ++r) {
+        if (orc_stl_from_triangles(&stl[r], tri + 9 * (size_t)off, triCounts[r], S->C.resGlobal) != 0) rc = -20;
+        off += triCounts[r];
+    }
+    if (!rc) rc = orc_build_rotors(S, stl, nRotors, levelMask);
+    for (int r = 0; r < nRotors; ++r) orc_stl_free(&stl[r]);
+    return rc;
+}
+
+/* solid fraction of rotor r of level L at a position, FP32 (getRotorFraction) */
+float orc_rotor_fraction_f32(const orc_sim *S, int L, int r, float angle, float x, float y, float z)
+{
+    const orc_rotor *R = &S->g[L].rotors[r];
+    int rm, ijk[3];
+    float w[3], fr = 0.f;
+    if (!rotor_locate_f32(R, cosf(angle), sinf(angle), x, y, z, &rm, ijk, w, &fr)) return 0.f;
+    return fr;
+}
+
+/* RotorInfoStruct of rotor r on level L: res, Bounds (6), cellCountX/Y/Z, radiansElapsed, nonZero blocks */
+int orc_rotor_info(const orc_sim *S, int L, int r, double *out)
+{
+    if (L < 0 || L >= S->levels || r < 0 || r >= S->g[L].nRotors) return -1;
+    const orc_rotor *R = &S->g[L].rotors[r];
+    const double v[12] = {R->res, R->B.xMin, R->B.yMin, R->B.zMin, R->B.xMax, R->B.yMax, R->B.zMax,
+                          R->cx, R->cy, R->cz, R->radiansElapsed, R->nonZero};
+    memcpy(out, v, sizeof v);
+    return 0;
+}
+
+/* rotor force tracker sums, trackerFunctions.h:225-330, in FP64 and before the
+   division by the iteration span and the unit conversion: force, and torque
+   about the rotation point with the force cells' centres as lever arms [mm] */
+int orc_rotor_forces(const orc_sim *S, int L, int r, double *out6)
+{
+    if (L < 0 || L >= S->levels || r < 0 || r >= S->g[L].nRotors) return -1;
+    const orc_rotor *R = &S->g[L].rotors[r];
+    for (int i = 0; i < 6; ++i) out6[i] = 0.0;
+    const float resBlock = R->res * 4.f, resForce = resBlock / 3.f;
+    const int bx = R->cx / 4, bxy = bx * (R->cy / 4);
+    const size_t nf = (size_t)R->nonZero * 27;
+    for (int x = 0; x < R->nonZero; ++x) {
+        const int b = R->index[x];
+        const int kb = b / bxy, jb = (b % bxy) / bx, ib = (b % bxy) % bx;
+        const float x0 = (float)ib * resBlock + R->B.xMin - R->ox + 0.5f * resForce;
+        const float y0 = (float)jb * resBlock + R->B.yMin - R->oy + 0.5f * resForce;
+        const float z0 = (float)kb * resBlock + R->B.zMin - R->oz + 0.5f * resForce;
+        for (int kF = 0; kF < 3; ++kF)
+            for (int jF = 0; jF < 3; ++jF)
+                for (int iF = 0; iF < 3; ++iF) {
+                    const double px = x0 + (float)iF * resForce, py = y0 + (float)jF * resForce, pz = z0 + (float)kF * resForce;
+                    const size_t fi = (size_t)x * 27 + (size_t)(kF * 9 + jF * 3 + iF);
+                    const double gx = R->g[fi], gy = R->g[nf + fi], gz = R->g[2 * nf + fi];
+                    out6[0] += gx; out6[1] += gy; out6[2] += gz;
+                    out6[3] += py * gz - pz * gy;
+                    out6[4] += pz * gx - px * gz;
+                    out6[5] += px * gy - py * gx;
+                }
+    }
+    return 0;
+}
+
 void orc_init_all(orc_sim *S)
 {
     for (int L = 0; L < S->levels; ++L) orc_apply_initial_condition(S, L);
@@ -209,6 +288,11 @@ void orc_sim_free(orc_sim *S)
         free(g->shifter); free(g->ri); free(g->rj); free(g->rk); free(g->rjp); free(g->rkp); free(g->rjkp);
         free(g->wallMap); free(g->wallIndex); free(g->wallData); free(g->linkLength); free(g->gWall);
         free(g->perIndex); free(g->perSource); free(g->f);
+        for (int r = 0; r < g->nRotors; ++r) {
+            orc_rotor *R = &g->rotors[r];
+            free(R->rotorMap); free(R->index); free(R->interp); free(R->g);
+        }
+        free(g->rotors);
         free(g->F2C.index); free(g->F2C.childMap); free(g->C2F.index); free(g->C2F.childMap);
         free(g->C2F.leftoverIndex); free(g->C2F.leftoverParent); free(g->C2F.leftoverNbrI);
         free(g->C2F.leftoverNbrJ); free(g->C2F.leftoverNbrK);

@@ -265,7 +265,101 @@ static void NAME(open_cell)(orc_sim *S, orc_grid *g, orc_openbc *O, int x, int f
     ((REAL *)O->uNormalCum)[x] += uNormal;
 }
 
-/* updateSingleGrid, updateGrid.h:16-386 (no rotors) */
+/* getRotorFraction, rotorFunctions.h:155-208 (projectXYZIntoRotorFrame :8-35,
+   interpolateRotorCube :135-153): the cell's interpolation cell in the rotor
+   frame and its solid fraction; 0 when it lies outside every marked block */
+static int NAME(rotor_locate)(const orc_rotor *R, REAL c, REAL s, REAL x, REAL y, REAL z, int *rmOut, int *ijkOut,
+                              REAL *wOut, REAL *frOut)
+{
+    REAL xr = x, yr = y, zr = z;
+    const REAL dx = x - (REAL)R->ox, dy = y - (REAL)R->oy, dz = z - (REAL)R->oz;
+    if (R->axis == LBM3_AXIS_X) { yr = (REAL)R->oy + c * dy + s * dz; zr = (REAL)R->oz - s * dy + c * dz; }
+    else if (R->axis == LBM3_AXIS_Y) { xr = (REAL)R->ox + c * dx - s * dz; zr = (REAL)R->oz + s * dx + c * dz; }
+    else if (R->axis == LBM3_AXIS_Z) { xr = (REAL)R->ox + c * dx + s * dy; yr = (REAL)R->oy - s * dx + c * dy; }
+    if (xr < (REAL)R->B.xMin || xr >= (REAL)R->B.xMax || yr < (REAL)R->B.yMin || yr >= (REAL)R->B.yMax ||
+        zr < (REAL)R->B.zMin || zr >= (REAL)R->B.zMax) return 0;
+    const REAL xRel = xr - (REAL)R->B.xMin, yRel = yr - (REAL)R->B.yMin, zRel = zr - (REAL)R->B.zMin;
+    const REAL res = (REAL)R->res;
+    const int ii = (int)(xRel / res), ji = (int)(yRel / res), ki = (int)(zRel / res);
+    if (ii >= R->cx || ji >= R->cy || ki >= R->cz) return 0;
+    const int bx = R->cx / 4, by = R->cy / 4;
+    const int block = (ki / 4) * bx * by + (ji / 4) * bx + ii / 4;
+    const int rm = R->rotorMap[block];
+    if (rm < 0) return 0;
+    const uint32_t packed = R->interp[rm * 64 + (ki % 4) * 16 + (ji % 4) * 4 + (ii % 4)];
+    const REAL xw = xRel / res - (REAL)ii, yw = yRel / res - (REAL)ji, zw = zRel / res - (REAL)ki;
+    REAL v[8];
+    for (unsigned q = 0; q < 8; ++q) v[q] = (REAL)((packed >> (4u * q)) & 0xFu);
+    const REAL v00 = v[0] + xw * (v[1] - v[0]), v10 = v[2] + xw * (v[3] - v[2]);
+    const REAL v01 = v[4] + xw * (v[5] - v[4]), v11 = v[6] + xw * (v[7] - v[6]);
+    const REAL v0 = v00 + yw * (v10 - v00), v1 = v01 + yw * (v11 - v01);
+    *frOut = (v0 + zw * (v1 - v0)) * (REAL)0.125f;
+    *rmOut = rm;
+    ijkOut[0] = ii; ijkOut[1] = ji; ijkOut[2] = ki;
+    wOut[0] = xw; wOut[1] = yw; wOut[2] = zw;
+    return 1;
+}
+
+/* processRotor, rotorFunctions.h:210-304 (getRotorForcing :63-100,
+   projectForcingIntoRotorFrame :37-61): adds the rotor's share of the forcing
+   toward rigid rotation to BC, in the compute precision with the rotor frame's
+   cosine and sine of the step given */
+static void NAME(process_rotor)(orc_rotor *R, const lbm3_info *I, REAL c, REAL s, REAL x, REAL y, REAL z, REAL rho,
+                                REAL uxPre, REAL uyPre, REAL uzPre, REAL *cum, int track, NAME(bcr) *BC)
+{
+    int rm, ijk[3];
+    REAL w[3], fr;
+    if (!NAME(rotor_locate)(R, c, s, x, y, z, &rm, ijk, w, &fr)) return;
+    const int ii = ijk[0], ji = ijk[1], ki = ijk[2];
+    const REAL xw = w[0], yw = w[1], zw = w[2];
+    const REAL dx = x - (REAL)R->ox, dy = y - (REAL)R->oy, dz = z - (REAL)R->oz;
+    if (fr == (REAL)0) return;
+    const REAL scale = (REAL)R->radiansPerSecond * (REAL)I->dtPhys / (REAL)I->res;
+    REAL ut[3] = {0, 0, 0};
+    if (R->axis == LBM3_AXIS_X) { ut[1] = -scale * dz; ut[2] = scale * dy; }
+    else if (R->axis == LBM3_AXIS_Y) { ut[0] = scale * dz; ut[2] = -scale * dx; }
+    else if (R->axis == LBM3_AXIS_Z) { ut[0] = -scale * dy; ut[1] = scale * dx; }
+    REAL gx = rho * (ut[0] - uxPre), gy = rho * (ut[1] - uyPre), gz = rho * (ut[2] - uzPre);
+    REAL ff = fr * fr * ((REAL)3 - (REAL)2 * fr);
+    if (*cum + ff > (REAL)1) { ff = (REAL)1 - *cum; *cum = (REAL)1; }
+    else *cum += ff;
+    gx *= ff; gy *= ff; gz *= ff;
+    BC->gx += gx; BC->gy += gy; BC->gz += gz;
+    if (!track) return;
+    if (R->axis == LBM3_AXIS_X) { const REAL t = gy; gy = c * t + s * gz; gz = -s * t + c * gz; }
+    else if (R->axis == LBM3_AXIS_Y) { const REAL t = gx; gx = c * t - s * gz; gz = s * t + c * gz; }
+    else if (R->axis == LBM3_AXIS_Z) { const REAL t = gx; gx = c * t + s * gy; gy = -s * t + c * gy; }
+    int iF = (int)((REAL)0.75f * ((REAL)(ii % 4) + xw)), jF = (int)((REAL)0.75f * ((REAL)(ji % 4) + yw));
+    int kF = (int)((REAL)0.75f * ((REAL)(ki % 4) + zw));
+    iF = iF < 0 ? 0 : (iF > 2 ? 2 : iF); jF = jF < 0 ? 0 : (jF > 2 ? 2 : jF); kF = kF < 0 ? 0 : (kF > 2 ? 2 : kF);
+    const size_t fi = (size_t)rm * 27 + (size_t)(kF * 9 + jF * 3 + iF), nf = (size_t)R->nonZero * 27;
+    /* several cells share a force cell: FP64 sums, atomic under OpenMP */
+#pragma omp atomic
+    R->g[fi] += (double)gx;
+#pragma omp atomic
+    R->g[nf + fi] += (double)gy;
+#pragma omp atomic
+    R->g[2 * nf + fi] += (double)gz;
+}
+
+/* the rotor loop of the cell update, updateGrid.h:98-129 */
+static void NAME(rotors_cell)(orc_grid *g, const REAL *cs, const int *ijk, REAL dRho, REAL ux, REAL uy, REAL uz,
+                              int track, NAME(bcr) *BC)
+{
+    const lbm3_info *I = &g->Info;
+    const REAL x = (REAL)((float)ijk[0] * I->res + I->ox), y = (REAL)((float)ijk[1] * I->res + I->oy),
+               z = (REAL)((float)ijk[2] * I->res + I->oz);
+    const REAL rho = dRho + (REAL)1, rhoInv = (REAL)1 / rho;
+    const REAL uxPre = (ux * rho + BC->gx) * rhoInv, uyPre = (uy * rho + BC->gy) * rhoInv, uzPre = (uz * rho + BC->gz) * rhoInv;
+    REAL cum = 0;
+    for (int r = 0; r < g->nRotors; ++r) {
+        NAME(process_rotor)(&g->rotors[r], I, cs[2 * r], cs[2 * r + 1], x, y, z, rho, uxPre, uyPre, uzPre, &cum, track, BC);
+        if (cum >= (REAL)1) break;
+    }
+    if (cum > (REAL)0) BC->collisionLimiter *= (REAL)1 - (cum < (REAL)0 ? (REAL)0 : (cum > (REAL)1 ? (REAL)1 : cum));
+}
+
+/* updateSingleGrid, updateGrid.h:16-386 */
 void NAME(orc_step_level)(orc_sim *S, int L)
 {
     orc_grid *g = &S->g[L];
@@ -273,6 +367,24 @@ void NAME(orc_step_level)(orc_sim *S, int L)
     const int N = I->cellCount;
     const int flip = I->esotwistFlipper;
     const REAL nu = (REAL)I->nu;
+    /* advance the rotors (updateGrid.h:36-49); the rotor frame's cosine and
+       sine of this step, computed once per rotor in the compute precision */
+    REAL cs[2 * LBM3_MAX_ROTORS];
+    for (int r = 0; r < g->nRotors; ++r) {
+        orc_rotor *R = &g->rotors[r];
+        const double twoPi = 6.283185307179586476925286766559;
+        R->radiansElapsed = fmod(R->radiansElapsed + (double)I->dtPhys * (double)R->radiansPerSecond, twoPi);
+        if (R->radiansElapsed < 0.0) R->radiansElapsed += twoPi;
+        if (R->radiansElapsed >= twoPi) R->radiansElapsed = 0.0;
+        if (sizeof(REAL) == sizeof(float)) {
+            cs[2 * r] = (REAL)cosf((float)R->radiansElapsed);
+            cs[2 * r + 1] = (REAL)sinf((float)R->radiansElapsed);
+        } else {
+            cs[2 * r] = (REAL)cos(R->radiansElapsed);
+            cs[2 * r + 1] = (REAL)sin(R->radiansElapsed);
+        }
+    }
+    const int trackRotor = S->C.trackRotorForce;
 #pragma omp parallel for schedule(static)
     for (int c = 0; c < N; ++c) {
         const int wm = g->wallMap[c];
@@ -295,6 +407,7 @@ void NAME(orc_step_level)(orc_sim *S, int L)
         BC.wallID = wallID;
         BC.dRho = m[0]; BC.ux = m[1]; BC.uy = m[2]; BC.uz = m[3];
         NAME(hook_local)(&S->C, I, ijk[0], ijk[1], ijk[2], &BC);
+        if (g->nRotors > 0) NAME(rotors_cell)(g, cs, ijk, m[0], m[1], m[2], m[3], trackRotor && trackForce, &BC);
         NAME(orc_collide)(f, &BC, nu);
         if (wm < 0) {
             for (int q = 0; q < 27; ++q) { int cc, s; NAME(post_index)(nb, flip, q, &cc, &s); F_AT(g, s, cc) = f[q]; }

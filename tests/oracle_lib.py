@@ -51,6 +51,12 @@ def lib():
         L.orc_moments_batch_f32.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
         L.orc_moments_batch_f64.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
         L.orc_openbc_cell_f32.argtypes = [C.c_int, C.c_int, C.POINTER(T.BC), C.c_void_p]
+        L.orc_build_rotors_from_triangles.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int]
+        L.orc_set_rotor.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_float, C.c_int, C.c_float, C.c_float, C.c_float]
+        L.orc_rotor_fraction_f32.restype = C.c_float
+        L.orc_rotor_fraction_f32.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float]
+        L.orc_rotor_info.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p]
+        L.orc_rotor_forces.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p]
         for t, real in (("f32", C.c_float), ("f64", C.c_double)):
             getattr(L, "orc_if_values_batch_" + t).argtypes = [C.c_void_p, C.c_int, real, C.c_void_p]
             getattr(L, "orc_if_coefficients_" + t).argtypes = [C.c_void_p, C.c_void_p]
@@ -97,14 +103,45 @@ class OracleSim:
             raise KeyError(name)
         if name in ("f", "wallForce") or name.startswith("open") and name[5:] in ("DRhoRef", "UNormalRef", "DRhoCum", "UNormalCum"):
             dt = np.float64 if es.value == 8 else np.float32
-        elif name in ("wallData", "linkLength"):
+        elif name in ("wallData", "linkLength") or (name.startswith("rotor") and name.endswith("Interp")):
             dt = np.uint32
+        elif name.startswith("rotor") and name.endswith("Force"):
+            dt = np.float64
         else:
             dt = _INT_DTYPES[es.value]
         if n.value == 0 or not p:
             return np.zeros(0, dt)
         buf = (C.c_char * (n.value * es.value)).from_address(p)
         return np.frombuffer(buf, dtype=dt).copy()
+
+    def build_rotors(self, rotors, level_mask=0xFF):
+        """buildRotors on every level of level_mask; rotors: list of (n, 9) triangle arrays."""
+        tris = [np.ascontiguousarray(t, np.float32).reshape(-1, 9) for t in rotors]
+        allt = np.concatenate(tris) if tris else np.zeros((1, 9), np.float32)
+        counts = np.array([len(t) for t in tris] or [0], np.int32)
+        rc = lib().orc_build_rotors_from_triangles(self.h, allt.ctypes.data, counts.ctypes.data, len(tris), level_mask)
+        if rc != 0:
+            raise RuntimeError("oracle rotor build failed (%d)" % rc)
+
+    def set_rotor(self, level, rotor, radians_per_second, axis, origin=(0.0, 0.0, 0.0)):
+        rc = lib().orc_set_rotor(self.h, level, rotor, radians_per_second, axis, *origin)
+        if rc != 0:
+            raise RuntimeError("bad rotor")
+
+    def rotor_info(self, level, rotor):
+        out = np.zeros(12)
+        if lib().orc_rotor_info(self.h, level, rotor, out.ctypes.data) != 0:
+            raise KeyError(rotor)
+        return out
+
+    def rotor_forces(self, level, rotor):
+        out = np.zeros(6)
+        if lib().orc_rotor_forces(self.h, level, rotor, out.ctypes.data) != 0:
+            raise KeyError(rotor)
+        return out
+
+    def rotor_fraction(self, level, rotor, angle, x, y, z):
+        return lib().orc_rotor_fraction_f32(self.h, level, rotor, angle, x, y, z)
 
     def open_count(self, level=0):
         return lib().orc_open_count(self.h, level)
