@@ -78,6 +78,59 @@ def measure(storage, steps, warmup, k17=False):
     return out, u
 
 
+def measure_multilevel(storage, steps=200, warmup=20):
+    """Three-level sphere (coarse 128x64x64, two nested factor-2 boxes, D = 24
+    finest cells, seeded inlet velocity): MLUPS of whole coarse steps, in the
+    reference's count (each level's cells times 2^level)."""
+    from lbm3_amd import cases, lib
+    case, dom, tris = cases.multilevel_sphere(128, 64, 64, diameter_fine=24.0, seed=0)
+    sim = lib.Sim(case, storage)
+    t0 = time.time()
+    fu = sim.build(dom, tris)
+    build_s = time.time() - t0
+    sim.init()
+    sim.run(warmup)
+    sim.sync()
+    total_ms = sim.run_timed(steps)[0]
+    out = {"storage": storage, "mlups": round(fu * steps / (total_ms / 1e3) / 1e6, 1),
+           "ms_per_coarse_step": round(total_ms / steps, 4), "fluid_updates_per_coarse_step": fu,
+           "cells_per_level": [sim.info(L).cellCount for L in range(3)], "build_s": round(build_s, 3)}
+    sim.close()
+    return out
+
+
+def measure_rotor(storage, windows=2, window=100, warmup=20):
+    """Rotor in a duct (coarse 192x96x96, two levels, seeded N-blade rotor of
+    ~200k triangles voxelized on the GPU, tip speed 0.08): MLUPS of whole
+    coarse steps and the FP64 rotor torque about the axis, averaged over each
+    window of coarse steps (per fine step of the rotor's level)."""
+    from lbm3_amd import cases, lib
+    case, dom, statics, rotor, ri = cases.rotor_duct(192, 96, 96, seed=0, levels=2)
+    sim = lib.Sim(case, storage)
+    t0 = time.time()
+    fu = sim.build(dom, statics)
+    sim.build_rotors([rotor], 3)
+    for L in range(2):
+        sim.set_rotor(L, 0, ri["omega"], ri["axis"], ri["origin"])
+    build_s = time.time() - t0
+    sim.init()
+    sim.run(warmup)
+    sim.sync()
+    torque, ms = [], 0.0
+    for _ in range(windows):
+        sim.reset_wall_forces()
+        ms += sim.run_timed(window)[0]
+        torque.append(float(sim.rotor_forces(1, 0)[3] / (2 * window)))
+    kinds = sim.cell_kinds(1)
+    out = {"storage": storage, "mlups": round(fu * windows * window / (ms / 1e3) / 1e6, 1),
+           "ms_per_coarse_step": round(ms / (windows * window), 4), "fluid_updates_per_coarse_step": fu,
+           "rotor": {"blades": ri["blades"], "twist_deg": round(ri["twist_deg"], 2), "triangles": ri["triangles"],
+                     "rotor_cells_fine": kinds["rotor"]},
+           "torque_x_per_window": torque, "window_coarse_steps": window, "build_s": round(build_s, 3)}
+    sim.close()
+    return out
+
+
 def cpu_baseline(threads, steps=2):
     """The CPU oracle (C, OpenMP over cells) on the same sphere workload, `steps` time steps."""
     os.environ.setdefault("OMP_NUM_THREADS", str(threads))
@@ -151,6 +204,8 @@ def main():
     ap.add_argument("--serving", choices=["quick", "full", "none"], default="quick",
                     help="serving workload: quick (32 requests, FP32), full (256 requests, "
                          "500-4000 steps, FP32 and FP16), none")
+    ap.add_argument("--no-other-cases", action="store_true",
+                    help="skip the multilevel-sphere and rotor-duct MLUPS lines")
     ap.add_argument("--counters-workload", action="store_true",
                     help="run only the short sequence that the HBM byte counters are collected on")
     args = ap.parse_args()
@@ -197,6 +252,10 @@ def main():
     for run in [main_run] + [v for k, v in variants.items() if k != "k17"]:
         run["roofline"]["traffic"] = traffic_from_profiles(run["storage"])
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_threads)
+    other = None
+    if not args.no_other_cases:
+        other = {"multilevel_sphere": measure_multilevel("f32"), "rotor_duct": measure_rotor("f32")}
+        print("other cases: %s" % other, file=sys.stderr, flush=True)
     serve = None
     if args.serving == "quick":
         serve = serving(32, 1.0, ["f32"])
@@ -231,6 +290,7 @@ def main():
                          "roofline": v["roofline"],
                          "speedup_vs_f32": round(v["mlups"] / main_run["mlups"], 3)} for k, v in variants.items()},
         "accuracy": accuracy,
+        "cases": other,
         "serving": serve,
         "copy_probe_gbs": round(2 * copy_bytes / (copy_ms / 1e3) / 1e9, 1),
         "device": {"name": name, "hbm_bytes": mem, "cus": cus},

@@ -162,6 +162,51 @@ def sphere(nx=256, ny=128, nz=128, diameter=32.0, center=(64.0, 64.0, 64.0), u_i
     return c, bounds(0, 0, 0, nx, ny, nz), [tri]
 
 
+def rotor_duct(nx=192, ny=96, nz=96, seed=0, levels=2, n_theta=2048, n_axial=48, tip_speed=0.08, u_inlet=0.02,
+               nu=0.004):
+    """Rotating rotor in a cylindrical duct on a coarse nx*ny*nz grid of 1 mm
+    cells (axis along x): the duct is a thick tube wall of inner radius 0.42 ny
+    whose outside covers the domain corners; the rotor (N blades, N from the
+    seed in 3..6, twist from the seed in 10..40 deg, hub radius 0.2 R, R = 0.8
+    times the duct radius, ~2 n_theta (n_axial + 1) triangles) sits at mid
+    length.  Velocity inlet at i = 0, non-reflective pressure outlet at
+    i = nx - 1, K15, the tube a no-slip IBB wall; with levels = 2 a box around
+    the rotor is refined once.  The rotor turns about x at the angular
+    velocity that gives tip_speed lattice units (the same on every level under
+    acoustic scaling).
+
+    Returns (case, domain, static triangles, rotor triangles, rotor dict)."""
+    rng = np.random.default_rng(seed)
+    n_blades = int(rng.integers(3, 7))
+    twist = float(rng.uniform(10.0, 40.0))
+    res, dt = f32(1.0), f32(1.0)
+    c = _case(levels, res, dt, f32(1.0))
+    c.nuPhys = float(f32(nu) * (res / f32(1000.0)) * (res / f32(1000.0)))
+    c.trackWallForce = 1
+    c.trackRotorForce = 1
+    c.init.kind = T.INIT_UNIFORM
+    c.init.ux = float(f32(u_inlet))
+    U = float(f32(u_inlet))
+    add_open_rule(c, x=T.FACE_HIGH, y=T.FACE_MID, z=T.FACE_MID, dirichletRho=1, dRho=0.0, openBCID=1, nonReflective=1)
+    add_open_rule(c, x=T.FACE_LOW, y=T.FACE_MID, z=T.FACE_MID, dirichletU=1, ux=U, openBCID=0, nonReflective=0)
+    add_open_rule(c, dirichletU=1, ux=U, openBCID=2, nonReflective=0)
+    add_local_rule(c, limiter=0.0)
+    add_local_rule(c, wall_id=0, velocity=(0.0, 0.0, 0.0))
+    r_duct = 0.42 * ny
+    R = 0.8 * r_duct
+    length = 0.25 * ny
+    cx = 0.5 * nx
+    if levels > 1:
+        h = 0.5 * length + 6.0
+        add_refine_rule(c, 0, (cx - h, -R - 4.0, -R - 4.0, cx + h, R + 4.0, R + 4.0))
+    statics = [geometry.tube(r_duct, 0.75 * ny, nx + 20.0, center=(cx, 0.0, 0.0), axis=0, n_theta=512)]
+    rotor = geometry.rotor(n_blades, R, 0.2, twist, length, (cx, 0.0, 0.0), axis=0, n_theta=n_theta, n_axial=n_axial)
+    omega = float(f32(tip_speed) / (f32(R) * dt / res))
+    info = {"blades": n_blades, "twist_deg": twist, "radius": R, "omega": omega, "axis": T.AXIS_X,
+            "origin": (cx, 0.0, 0.0), "triangles": len(rotor)}
+    return c, bounds(0, -0.5 * ny, -0.5 * nz, nx, 0.5 * ny, 0.5 * nz), statics, rotor, info
+
+
 def periodic_box(n=24, nu=0.05, u=(0.0, 0.0, 0.0), limiter=0.0):
     """Fully periodic cube of n^3 active cells of 1 mm (one ghost layer per
     face), centred on the origin, uniform initial state, lattice viscosity nu."""

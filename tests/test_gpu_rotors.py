@@ -132,3 +132,38 @@ def test_rotor_inside_a_duct_wall_f64():
     fo, fg = o.rotor_forces(0, 0), g.rotor_forces(0, 0)
     assert np.max(np.abs(fg - fo)) < 1e-9 * np.max(np.abs(fo))
     g.close()
+
+
+def test_rotor_duct_two_levels_f64():
+    """The rotor-in-a-duct case at a reduced size (48x24x24 coarse, 2 levels,
+    4-blade twisted rotor): GPU-built grids and rotor arrays bit-exact on both
+    levels, then 20 coarse steps in FP64 against the oracle."""
+    case, dom, statics, rotor, ri = cases.rotor_duct(48, 24, 24, seed=1, levels=2, n_theta=256, n_axial=8)
+    o = OracleSim(case, dom, statics, double=True)
+    g = lib.Sim(case, "f64")
+    g.build(dom, statics)
+    for L in range(2):
+        for name in ("shifter", "wallMap", "wallData", "linkLength"):
+            assert np.array_equal(g.array(name, L), o.array(name, L)), (name, L)
+    o.build_rotors([rotor], 3)
+    g.build_rotors([rotor], 3)
+    for L in range(2):
+        for name in ("rotor0Map", "rotor0Index", "rotor0Interp"):
+            assert np.array_equal(g.array(name, L), o.array(name, L)), (name, L)
+        o.set_rotor(L, 0, ri["omega"], ri["axis"], ri["origin"])
+        g.set_rotor(L, 0, ri["omega"], ri["axis"], ri["origin"])
+    o.init()
+    g.init()
+    o.run(20)
+    g.run(20)
+    g.sync()
+    for L in range(2):
+        mo, mg = o.macros(L), g.macros(L)
+        solid = o.array("wallMap", L) == -3
+        r = rel_max(1.0 + mg[~solid, 0], 1.0 + mo[~solid, 0])
+        u = rel_max(mg[~solid, 1:], mo[~solid, 1:])
+        print("rotor duct level %d f64: rho %.2e u %.2e" % (L, r, u))
+        assert r < 1e-10 and u < 1e-10, L
+    fo, fg = o.rotor_forces(1, 0), g.rotor_forces(1, 0)
+    assert np.max(np.abs(fg - fo)) < 1e-9 * np.max(np.abs(fo))
+    g.close()
