@@ -66,6 +66,7 @@ def measure(storage, steps, warmup, k17=False):
         "boundary_kernels_us": wall_ms / launches * 1e3,
         "kernel_mlups": bulk_cells / avg_kernel_s / 1e6,
         "bulk_cells": bulk_cells, "wall_adjacent": wall_adjacent, "open_cells": kinds["open"],
+        "open_group_cells": kinds["open_group"],
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None},
         "cells": info.cellCount, "walls": walls, "fluid_updates_per_step": fluid_updates,
@@ -284,7 +285,7 @@ def main():
         "boundary_kernels_us": round(main_run["boundary_kernels_us"], 2),
         "kernel_mlups": round(main_run["kernel_mlups"], 1),
         "bulk_cells": main_run["bulk_cells"], "wall_adjacent": main_run["wall_adjacent"],
-        "open_cells": main_run["open_cells"],
+        "open_cells": main_run["open_cells"], "open_group_cells": main_run["open_group_cells"],
         "variants": {k: {"mlups": round(v["mlups"], 1), "ms_per_step": round(v["ms_per_step"], 4),
                          "kernel_us": round(v["kernel_us"], 2), "boundary_kernels_us": round(v["boundary_kernels_us"], 2),
                          "roofline": v["roofline"],

@@ -1,5 +1,6 @@
 // C ABI of the hot path (include/lbm3mi.h): simulation objects, level import,
 // the per-step driver with 2:1 sub-cycling, observables and timing.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -107,12 +108,17 @@ int finish_level_setup(Sim &s, int level)
 
 static int settle_open(Sim &s, int L);
 
+static bool wide_of(const Sim &s, const Level &l)
+{
+    return 27LL * padded_cells(l.N) * (long long)storage_bytes(s.storage) > 0xFFFFFFFFLL;
+}
+
 // Decide which special cells of level L get kernels of their own and rebuild
 // the per-cell records the bulk kernel skips them by:
 // * the cells a rotor can reach at some angle (record -4, k_rotor_cells);
 // * on level 0, the open-boundary cells (record -(8 + open index),
-//   k_open_cells, which applies the boundary condition when it next loads the
-//   cell), unless an open cell is wall-adjacent or reachable by a rotor, or a
+//   k_open_groups over aligned groups of cells around them, which applies the
+//   boundary condition when it next loads the cell), unless an open cell is wall-adjacent or reachable by a rotor, or a
 //   periodic copy or an interface transfer touches one (those passes read or
 //   write whole cells and need the condition applied in the same step).
 static int prepare_open_fusion(Sim &s, int L)
@@ -160,10 +166,37 @@ static int prepare_open_fusion(Sim &s, int L)
         int conflicts = 0;
         HIPCHK(hipMemcpyAsync(&conflicts, flag.p, sizeof(int), hipMemcpyDeviceToHost, s.stream));
         HIPCHK(hipStreamSynchronize(s.stream));
-        fuse = conflicts == 0;
+        // the open-cell kernel uses 32-bit buffer offsets
+        fuse = conflicts == 0 && !wide_of(s, l);
     }
     launch_fast_topo(topo_of(l), l.wallMap.p, fuse ? openOf.p : nullptr, cand, l.cellRow.p, l.rowTab.p, s.stream);
     HIPCHK(hipGetLastError());
+    l.nOpenGroups = 0;
+    if (fuse) {
+        // aligned groups of cells holding open cells, sorted by the boundary
+        // condition (normal and record) of their first open cell, then by group
+        std::vector<int> cell((size_t)l.nOpen);
+        std::vector<uint32_t> meta((size_t)l.nOpen);
+        HIPCHK(hipMemcpyAsync(cell.data(), l.openCell.p, sizeof(int) * cell.size(), hipMemcpyDeviceToHost, s.stream));
+        HIPCHK(hipMemcpyAsync(meta.data(), l.openMeta.p, sizeof(uint32_t) * meta.size(), hipMemcpyDeviceToHost, s.stream));
+        HIPCHK(hipStreamSynchronize(s.stream));
+        std::vector<std::pair<int, int>> first((size_t)(l.N / LBM3_OPEN_GROUP + 1), {-1, 0});   // {first open cell, key}
+        for (int x = 0; x < l.nOpen; ++x) {
+            const int g = cell[(size_t)x] / LBM3_OPEN_GROUP;
+            if (first[(size_t)g].first < 0 || cell[(size_t)x] < first[(size_t)g].first)
+                first[(size_t)g] = {cell[(size_t)x], int(meta[(size_t)x] & 0x1FFFu)};
+        }
+        std::vector<std::pair<int, int>> order;
+        for (size_t g = 0; g < first.size(); ++g)
+            if (first[g].first >= 0) order.push_back({first[g].second, (int)g});
+        std::sort(order.begin(), order.end());
+        std::vector<int> groups;
+        for (const auto &o : order) groups.push_back(o.second);
+        l.nOpenGroups = (int)groups.size();
+        if (upload(l.openGroups, groups.data(), (long long)groups.size(), s.stream)) return LBM3_E_HIP;
+        launch_mark_groups(l.openGroups.p, l.nOpenGroups, LBM3_OPEN_GROUP, l.N, l.cellRow.p, s.stream);
+        HIPCHK(hipGetLastError());
+    }
     if (cand) {
         // the cells with record -4, in cell order (coalesced rows)
         std::vector<int2> rec((size_t)l.N);
@@ -255,7 +288,7 @@ template <class S> static void step_level(Sim &s, int level)
         // open cells have their own kernel, which first applies the previous
         // step's boundary condition when it is still pending
         const OpenArgs<S> o = open_args<S>(s, l, l.info.iterationsFinished - 1);
-        a.open = {o.cell, o.count, l.openSettled ? 0 : 1, o.meta, o.records, o.dRhoRef, o.uNormalRef, o.dRhoCum,
+        a.open = {l.openGroups.p, l.nOpenGroups, l.openSettled ? 0 : 1, o.meta, o.records, o.dRhoRef, o.uNormalRef, o.dRhoCum,
                   o.uNormalCum, o.iterationsFinished};
     }
     if (timed || (l.W == 0 && a.open.count == 0 && a.rotorCellCount == 0)) {

@@ -88,12 +88,16 @@ struct LevelGeom {          // what the hooks need about the level
     int cellCountX, cellCountY, cellCountZ;
 };
 
-// Open-boundary cells updated by their own kernel (k_open_cells): each step
-// completes the populations the previous step left unknown (the deferred
-// boundary condition), then collides and streams the cell.  count == 0: the
-// level's open cells go through the bulk kernel and the separate pass.
+// Open-boundary cells updated by their own kernel (k_open_groups): each step
+// it completes the populations the previous step left unknown (the deferred
+// boundary condition), then collides and streams the cell.  The kernel takes
+// whole aligned groups of LBM3_OPEN_GROUP cells around the open cells (the
+// other cells of a group are plain fluid cells, record -5), so that a face
+// cell at the end of a row loads and stores full lines together with its row
+// neighbours.  count == 0: the level's open cells go through the bulk kernel
+// and the separate pass.
 template <class T> struct FusedOpen {
-    const int *cell;              // [count] open-list cells, sorted by BCID then cell
+    const int *groups;            // [count] groups, sorted by the boundary condition of their first open cell
     int count;
     int pending;                  // the previous step's boundary condition is still to apply
     const uint32_t *meta;         // per open-list entry: normal index, record, track flag

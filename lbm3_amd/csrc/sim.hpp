@@ -73,6 +73,8 @@ struct Level {
     bool openFusionChecked = false;  // decided for the current lists
     bool fusedOpen = false;          // the level uses it
     bool openSettled = true;         // no boundary condition pending (after init, upload or settle)
+    DBuf<int> openGroups;            // aligned cell groups holding open cells (k_open_groups)
+    int nOpenGroups = 0;
     LocalBC bulk{};
     DBuf<LocalBC> wallBC;            // [32]
     DBuf<unsigned char> f;           // [27][N] storage type

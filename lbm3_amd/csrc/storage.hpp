@@ -36,9 +36,18 @@ namespace lbm3 {
 #endif
 constexpr int LBM3_TILE = LBM3_TILE_CELLS;
 
+// cells per open-boundary group (k_open_groups).  Measured on the 256x128x128
+// sphere (FP32 whole step): 4 cells 206.8 us, 8 cells 205.2 us, 16 cells
+// 210.9 us, 32 cells 214.2 us, 64 cells 231.4 us, one open cell per lane
+// 208.9 us -- small groups keep the x-face rows' other cells in the bulk
+// kernel, large groups share fewer lines with it
+#ifndef LBM3_OPEN_GROUP
+#define LBM3_OPEN_GROUP 8
+#endif
+
 __host__ __device__ __forceinline__ size_t tile_base(int cell)
 {
-    return (size_t)(cell / LBM3_TILE) * (27 * LBM3_TILE) + (size_t)(cell % LBM3_TILE);
+    return (size_t)((unsigned)cell / LBM3_TILE) * (27 * LBM3_TILE) + (size_t)((unsigned)cell % LBM3_TILE);
 }
 __host__ __device__ __forceinline__ size_t pop_index(int plane, int cell)
 {
@@ -82,18 +91,52 @@ struct Codec {
     float iw[27];
 };
 
+// Raw buffer access for levels below 4 GiB: a wave-uniform buffer resource for
+// the level array, a per-lane 32-bit byte offset and a wave-uniform byte offset
+// per slot plane, so that the 54 population accesses of a cell need no vector
+// address arithmetic (buffer_load/store ... offen with an SGPR soffset).
+// Word 3 of the resource is the raw-buffer format of gfx9-class GPUs.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t pop_rsrc(const void *base, long long bytes)
+{
+    const int num = bytes >= 0xFFFFFFFFLL ? int(0xFFFFFFFFu) : int((unsigned)bytes);
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, num, 0x00020000);
+}
+
 template <class S> struct StoreTraits;
 
 template <> struct StoreTraits<float> {
     using compute = float;
     static __device__ __forceinline__ float load(const float *p, int, const Codec &) { return ld(p); }
     static __device__ __forceinline__ void store(float *p, float v, int, const Codec &) { st(p, v); }
+    static __device__ __forceinline__ float
+    bload(__amdgpu_buffer_rsrc_t r, uint32_t voff, int soff, int q, const Codec &c)
+    {
+        (void)q; (void)c;
+        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, soff, 0));
+    }
+    static __device__ __forceinline__ void bstore(__amdgpu_buffer_rsrc_t r, uint32_t voff, int soff, float v, int q, const Codec &c)
+    {
+        (void)q; (void)c;
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int, v), r, (int)voff, soff, 0);
+    }
 };
 
 template <> struct StoreTraits<double> {
     using compute = double;
     static __device__ __forceinline__ double load(const double *p, int, const Codec &) { return *p; }
     static __device__ __forceinline__ void store(double *p, double v, int, const Codec &) { *p = v; }
+    static __device__ __forceinline__ double
+    bload(__amdgpu_buffer_rsrc_t r, uint32_t voff, int soff, int q, const Codec &c)
+    {
+        (void)q; (void)c;
+        return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, soff, 0));
+    }
+    static __device__ __forceinline__ void bstore(__amdgpu_buffer_rsrc_t r, uint32_t voff, int soff, double v, int q, const Codec &c)
+    {
+        (void)q; (void)c;
+        typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, (int)voff, soff, 0);
+    }
 };
 
 template <> struct StoreTraits<__half> {
@@ -106,6 +149,17 @@ template <> struct StoreTraits<__half> {
     static __device__ __forceinline__ void store(__half *p, float v, int q, const Codec &c)
     {
         st(reinterpret_cast<unsigned short *>(p), __half_as_ushort(__float2half_rn((v - c.off[q]) * c.iw[q])));
+    }
+    static __device__ __forceinline__ float
+    bload(__amdgpu_buffer_rsrc_t r, uint32_t voff, int soff, int q, const Codec &c)
+    {
+        const unsigned short bits = (unsigned short)__builtin_amdgcn_raw_buffer_load_b16(r, (int)voff, soff, 0);
+        return fmaf(__half2float(__ushort_as_half(bits)), c.w[q], c.off[q]);
+    }
+    static __device__ __forceinline__ void bstore(__amdgpu_buffer_rsrc_t r, uint32_t voff, int soff, float v, int q, const Codec &c)
+    {
+        const unsigned short bits = __half_as_ushort(__float2half_rn((v - c.off[q]) * c.iw[q]));
+        __builtin_amdgcn_raw_buffer_store_b16((short)bits, r, (int)voff, soff, 0);
     }
 };
 
@@ -122,6 +176,18 @@ template <> struct StoreTraits<__hip_bfloat16> {
         unsigned int u = __float_as_uint((v - c.off[q]) * c.iw[q]);
         u += 0x7FFFu + ((u >> 16) & 1u);
         st(reinterpret_cast<unsigned short *>(p), static_cast<unsigned short>(u >> 16));
+    }
+    static __device__ __forceinline__ float
+    bload(__amdgpu_buffer_rsrc_t r, uint32_t voff, int soff, int q, const Codec &c)
+    {
+        const unsigned short bits = (unsigned short)__builtin_amdgcn_raw_buffer_load_b16(r, (int)voff, soff, 0);
+        return fmaf(__uint_as_float(static_cast<unsigned int>(bits) << 16), c.w[q], c.off[q]);
+    }
+    static __device__ __forceinline__ void bstore(__amdgpu_buffer_rsrc_t r, uint32_t voff, int soff, float v, int q, const Codec &c)
+    {
+        unsigned int u = __float_as_uint((v - c.off[q]) * c.iw[q]);
+        u += 0x7FFFu + ((u >> 16) & 1u);
+        __builtin_amdgcn_raw_buffer_store_b16((short)(unsigned short)(u >> 16), r, (int)voff, soff, 0);
     }
 };
 

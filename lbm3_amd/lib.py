@@ -229,11 +229,13 @@ class Sim:
 
     def cell_kinds(self, level=0):
         """Counts of the cells each per-step kernel updates on a level:
-        {"bulk", "wall", "open", "rotor", "solid"} (open, rotor: cells with their own kernel;
+        {"bulk", "wall", "open", "rotor", "open_group", "solid"} (open, rotor and the
+        fluid cells grouped with open cells have kernels of their own;
         where the open boundary runs as a separate pass they count as bulk)."""
         code = (self.array("cellRow", level) & 0xFFFFFFFF).astype(np.uint32).view(np.int32)
         return {"bulk": int(((code == -1) | (code == -2)).sum()), "wall": int((code >= 0).sum()),
-                "open": int((code <= -8).sum()), "rotor": int((code == -4).sum()), "solid": int((code == -3).sum())}
+                "open": int((code <= -8).sum()), "rotor": int((code == -4).sum()), "open_group": int((code == -5).sum()),
+                "solid": int((code == -3).sum())}
 
     def kinetic_energy(self):
         v = C.c_double()
