@@ -51,28 +51,24 @@ __host__ __device__ inline Relax<T> make_relax(T nu, T nuMult, T lambda, T gx, T
     return r;
 }
 
-// reciprocal: the hardware estimate refined by one Newton step in FP32, exact division in FP64
+// reciprocal: the hardware estimate (about 1 ulp) refined by one Newton step
+// on the residual e = 1 - x r (one FMA, no rounding next to 1), so the result
+// is correctly rounded but for rare near-halfway cases and its error has no
+// preferred sign.  The form
+// r (2 - x r) rounds 2 - x r next to 1 and leaves a bias of a few 1e-8 that
+// the collision turns into a loss of momentum every step (0.3% extra energy
+// decay on a 128^3 Taylor-Green vortex).  Exact division in FP64.
 __host__ __device__ __forceinline__ float fast_rcp(float x)
 {
 #if defined(__HIP_DEVICE_COMPILE__)
     const float r = __builtin_amdgcn_rcpf(x);
-    return r * (2.f - x * r);
+    const float e = __builtin_fmaf(-x, r, 1.f);
+    return __builtin_fmaf(r, e, r);
 #else
     return 1.f / x;
 #endif
 }
 __host__ __device__ __forceinline__ double fast_rcp(double x) { return 1.0 / x; }
-
-// hardware reciprocal estimate (about 1 ulp) for the limiter's rational function
-__host__ __device__ __forceinline__ float rough_rcp(float x)
-{
-#if defined(__HIP_DEVICE_COMPILE__)
-    return __builtin_amdgcn_rcpf(x);
-#else
-    return 1.f / x;
-#endif
-}
-__host__ __device__ __forceinline__ double rough_rcp(double x) { return 1.0 / x; }
 
 // forward: (f-, f0, f+) about velocity u -> (k0, k1, k2); K is the weight moment of the chain
 template <class T>
@@ -262,7 +258,7 @@ __host__ __device__ __forceinline__ void collide(T (&f)[27], const Relax<T> &R)
         // limited relaxation: the rate tends to 1 where the cumulant is large compared to lambda
         auto limited = [&](T c, T w) {
             const T a = c < T(0) ? -c : c;
-            const T wl = w + (T(1) - w) * a * rough_rcp(lambda + a);
+            const T wl = w + (T(1) - w) * a * fast_rcp(lambda + a);
             return (T(1) - wl) * c;
         };
         const T p1 = limited(k120 + k102, R.w3), p2 = limited(k210 + k012, R.w3), p3 = limited(k201 + k021, R.w3);

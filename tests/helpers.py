@@ -1,4 +1,7 @@
-"""Shared helpers: move an oracle-built level into the GPU import layout, error norms."""
+"""Shared helpers: move an oracle-built level into the GPU import layout, error
+norms, the reference-held Taylor-Green kinetic-energy curves."""
+import os
+
 import numpy as np
 
 
@@ -33,3 +36,35 @@ def rel_l2(a, b):
     a = np.asarray(a, np.float64)
     b = np.asarray(b, np.float64)
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+# ---------------------------------------------------------------- Taylor-Green kinetic-energy curves
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+# fixture name -> (active coarse cells per axis, grid levels, iterations between samples)
+# (taylorGreenVortexMain.cu:8-13,39-40)
+TGV_CURVES = {"coarse": (128, 1, 10), "quadratic": (128, 2, 10), "fine": (256, 1, 20)}
+
+
+def tgv_curve(name):
+    """(t*, E*) of a reference-held Taylor-Green kinetic-energy curve."""
+    d = np.loadtxt(os.path.join(GOLDEN, "tgv_%s_kinetic_energy.txt" % name), delimiter=";", skiprows=1)
+    return d[:, 0], d[:, 1]
+
+
+def tgv_reference_case(name):
+    """The reference's Taylor-Green run for a curve: Re = 1600, Ma = 0.1 (lattice
+    u0 = Ma / sqrt(3) in FP32), NASA pressure perturbation, periodic, default
+    collision (K17, limiter 0.05).  Returns (case, domain, triangles, sample
+    iterations, energy scale) with E* = kinetic_energy() * scale."""
+    from lbm3_amd import cases
+    f32 = np.float32
+    n, levels, period = TGV_CURVES[name]
+    u0 = float(f32(0.1) * f32(0.5773502691896258))
+    case, dom, tris = cases.taylor_green(n, u0=u0, reynolds=1600.0, pressure=True, levels=levels)
+    dt = f32(case.dtPhysGlobal)
+    iterations = int(f32(20.0) / dt + f32(0.5))        # ITERATION_COUNT, taylorGreenVortexMain.cu:34-35
+    samples = list(range(0, iterations + 1, period))
+    if samples[-1] != iterations:
+        samples.append(iterations)
+    return case, dom, tris, np.array(samples), 1.0 / (n ** 3 * u0 * u0)

@@ -1,0 +1,52 @@
+"""The oracle against the reference's own Taylor-Green output: the kinetic-energy
+curves the reference holds (cases/taylorGreenVortex/results/*_kineticEnergy.txt,
+copied as data fixtures under tests/golden/) were written by the reference's
+FP32 solver on the same case (Re = 1600, Ma = 0.1, taylorGreenVortexMain.cu).
+
+The first samples pin the FP32 oracle's collision (K17 with limiter),
+streaming, periodic ghosts, initial condition and, on two levels, the
+interface transfer and the level weighting of the energy: the energy lost in
+10 to 30 steps matches the reference's to 3e-8, i.e. within the rounding of the
+curve's 9-digit FP32 values (a 1% viscosity error, K15 in place of K17 or the
+pressure term left out of the initial condition each move it by 2.5e-7 or more
+within 30 steps)."""
+import numpy as np
+import pytest
+
+from tests.helpers import TGV_CURVES, tgv_curve, tgv_reference_case
+from tests.oracle_lib import OracleSim
+
+
+@pytest.mark.parametrize("name", sorted(TGV_CURVES))
+def test_case_reproduces_the_reference_time_axis(name):
+    """Time step, iteration count and sampling of our case construction give the
+    curve's t* column: t* = iteration * dt (taylorGreenVortexMain.cu:21-40)."""
+    case, _, _, samples, _ = tgv_reference_case(name)
+    t_ref, e_ref = tgv_curve(name)
+    assert len(samples) == len(t_ref) == 707
+    t = samples * float(np.float32(case.dtPhysGlobal))
+    assert np.max(np.abs(t - t_ref) / np.maximum(t_ref, 1.0)) < 1e-8
+    assert abs(e_ref[0] - 0.125) < 2e-7
+    assert np.all(np.diff(e_ref) < 0)                       # the energy only decays
+
+
+def _energy_run(name, samples):
+    case, dom, tris, it, scale = tgv_reference_case(name)
+    o = OracleSim(case, dom, tris, double=False)
+    o.init()
+    e = [o.kinetic_energy() * scale]
+    for k in range(1, samples + 1):
+        o.run(int(it[k] - it[k - 1]))
+        e.append(o.kinetic_energy() * scale)
+    return np.array(e)
+
+
+@pytest.mark.parametrize("name,samples", [("coarse", 3), ("quadratic", 2)])
+def test_oracle_energy_loss_matches_the_reference_curve(name, samples):
+    e = _energy_run(name, samples)
+    _, e_ref = tgv_curve(name)
+    e_ref = e_ref[:samples + 1]
+    print(name, ["%.9g" % x for x in e], ["%.9g" % x for x in e_ref])
+    assert abs(e[0] - e_ref[0]) < 2e-8
+    loss, loss_ref = e[0] - e[1:], e_ref[0] - e_ref[1:]
+    assert np.max(np.abs(loss - loss_ref)) < 3e-8
