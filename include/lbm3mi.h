@@ -154,6 +154,18 @@ int lbm3_kinetic_energy(lbm3_sim *s, double *out);
  * `origin` (trackerFunctions.h:356-474, FP64): out[body*6 + {Fx,Fy,Fz,Tx,Ty,Tz}] */
 int lbm3_wall_forces(lbm3_sim *s, int level, const double *origin, int nBodies, double *out);
 int lbm3_reset_wall_forces(lbm3_sim *s);   /* wall and rotor force accumulators */
+/* Open-boundary flow statistics of updateTracker (trackerFunctions.h:127-223)
+ * for boundary openBCID over every level that has it, from the per-cell
+ * accumulators time-averaged over the steps since the last
+ * lbm3_reset_open_boundary_stats (the case must track open boundaries):
+ * out7 = {tracked area [m^2], area-averaged normal velocity [m/s], mass flow
+ * [kg/s], momentum thrust [N], area-averaged pressure [Pa], pressure power [W],
+ * normal kinetic power [W]}.  Normal velocities are along the outer normal.
+ * FP64 throughout (the reference reduces in FP32). */
+int lbm3_open_boundary_stats(lbm3_sim *s, int openBCID, double *out7);
+/* zero the open-boundary accumulators and every level's iterationsSinceTrackerReset
+ * (the reset at the end of updateTracker, trackerFunctions.h:476-486) */
+int lbm3_reset_open_boundary_stats(lbm3_sim *s);
 
 /* applyCollision (applyCollision.h:17) on n independent cells, 27 shifted
  * populations each (cell-major), on the device. */

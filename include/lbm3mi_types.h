@@ -73,10 +73,22 @@ typedef struct lbm3_bc {
 #define LBM3_FACE_HIGH 4
 #define LBM3_FACE_ANY 7
 
+/* A radial profile about a line: the factor max(0, 1 - r^2 / radius^2), r the
+ * distance of the cell centre from the line through `origin` along `axis`,
+ * evaluated in FP32 as the pipe cases write it (r^2 = dx*dx + dy*dy,
+ * poiseuilleFlowMain.cu:54-55,66-72).  radius <= 0 (a zeroed struct): no
+ * profile, factor 1. */
+typedef struct lbm3_profile {
+    int axis;               /* LBM3_AXIS_X, _Y or _Z */
+    float origin[3];        /* a point of the line [mm] */
+    float radius;           /* [mm]; <= 0: no profile */
+} lbm3_profile;
+
 typedef struct lbm3_open_rule {
     int levelMask;          /* bit L: applies on level L; 0 means every level */
     int faceMask[3];        /* per axis x, y, z: LBM3_FACE_* bits a boundary cell must match */
     lbm3_bc bc;             /* the BCStruct the hook returns for matching cells */
+    lbm3_profile profile;   /* bc.ux/uy/uz are scaled by the profile factor of the cell */
 } lbm3_open_rule;
 
 /* getLocalBC: rules are applied in order to the BCStruct that the kernel has
@@ -110,6 +122,7 @@ typedef struct lbm3_init {
     float u0;                /* Taylor-Green amplitude, lattice units */
     float length;            /* Taylor-Green reference length [mm]: X = x / length */
     int pressure;            /* Taylor-Green: add the NASA pressure perturbation */
+    lbm3_profile profile;    /* uniform state: ux/uy/uz scaled by the profile factor of the cell */
 } lbm3_init;
 
 #define LBM3_REFINE_SET 0    /* marker := inside (any SET box of the level) */
@@ -120,6 +133,7 @@ typedef struct lbm3_refine_rule {
     int level;
     int mode;
     float xMin, yMin, zMin, xMax, yMax, zMax; /* open box, physical [mm] */
+    lbm3_profile cylinder;  /* radius > 0: also r^2 < radius^2 about that line */
 } lbm3_refine_rule;
 
 typedef struct lbm3_case {

@@ -69,11 +69,23 @@ def add_local_rule(c, wall_id=-2, level_mask=0, **kw):
     c.nLocalRules += 1
 
 
-def add_refine_rule(c, level, box, mode=T.REFINE_SET):
+def add_refine_rule(c, level, box, mode=T.REFINE_SET, cylinder=None):
+    """Refinement box (open, physical mm); cylinder=(axis, origin, radius)
+    further limits it to r < radius about that line."""
     r = c.refineRules[c.nRefineRules]
     r.level, r.mode = level, mode
     r.xMin, r.yMin, r.zMin, r.xMax, r.yMax, r.zMax = [float(f32(v)) for v in box]
+    if cylinder is not None:
+        set_profile(r.cylinder, *cylinder)
     c.nRefineRules += 1
+
+
+def set_profile(p, axis, origin, radius):
+    """Fill an lbm3_profile: max(0, 1 - r^2 / radius^2) about the line through origin along axis."""
+    p.axis = axis
+    for a in range(3):
+        p.origin[a] = float(f32(origin[a]))
+    p.radius = float(f32(radius))
 
 
 def bounds(xmin, ymin, zmin, xmax, ymax, zmax):
@@ -160,6 +172,47 @@ def sphere(nx=256, ny=128, nz=128, diameter=32.0, center=(64.0, 64.0, 64.0), u_i
     c.storeRef[0] = U                     # 16-bit stores: encode the deviation from the free stream
     tri = geometry.uv_sphere(diameter / 2.0, center, segments[0], segments[1])
     return c, bounds(0, 0, 0, nx, ny, nz), [tri]
+
+
+def pipe(triangles, length=100.0, res=0.2, diameter=20.0, reynolds=1000.0, u_mean=0.03, inlet_non_reflective=True,
+         profile_init=False, levels=1, nu_phys=1.5e-5, rho_phys=1.225):
+    """Flow through the 20 mm bore of the reference's pipe STL
+    (pipeD20L200STL.STL: a 25 mm square block with a 200 mm bore along z),
+    cut to `length` mm about z = 0, as the reference's pipe cases set it up
+    (nonReflectiveTestMain.cu:1-105: Re 1000, mean inlet velocity 0.03, fluid at
+    rest at the start, 0.2 mm cells; poiseuilleFlowMain.cu:1-95: Re 100, mean
+    inlet velocity 0.01, Poiseuille profile in the whole bore at the start):
+    parabolic velocity inlet at k = 0 (centre-line velocity 2 u_mean, open
+    boundary 0, non-reflective or strict Dirichlet), pressure outlet dRho = 0 at
+    k = nz - 1 (open boundary 1), K15 everywhere, the bore a no-slip wall
+    without interpolated bounceback, open-boundary flow tracked.  With
+    levels = 2 the core r < D/4, |z| < length/4 is refined once
+    (poiseuilleFlowMain.cu:32-46).  Returns (case, domain, triangles)."""
+    R = f32(0.5) * f32(diameter)
+    u_phys = f32(reynolds) * f32(nu_phys) / (f32(diameter) / f32(1000.0))
+    dt = (f32(u_mean) / u_phys) * (f32(res) / f32(1000.0))
+    c = _case(levels, res, dt, nu_phys, rho_phys)
+    c.trackWallForce = 0
+    c.trackOpenBoundaries = 1
+    U = float(f32(2.0) * f32(u_mean))
+    add_open_rule(c, z=T.FACE_LOW, dirichletU=1, ux=0.0, uy=0.0, uz=U, nonReflective=int(bool(inlet_non_reflective)),
+                  openBCID=0)
+    set_profile(c.openRules[c.nOpenRules - 1].profile, T.AXIS_Z, (0.0, 0.0, 0.0), R)
+    add_open_rule(c, z=T.FACE_HIGH, dirichletRho=1, dRho=0.0, nonReflective=0, openBCID=1)
+    add_local_rule(c, limiter=0.0, use_ibb=0)
+    add_local_rule(c, wall_id=0, velocity=(0.0, 0.0, 0.0))
+    c.init.kind = T.INIT_UNIFORM
+    if profile_init:
+        c.init.uz = U
+        set_profile(c.init.profile, T.AXIS_Z, (0.0, 0.0, 0.0), R)
+    if levels > 1:
+        zr = f32(length) * f32(0.25)
+        add_refine_rule(c, 0, (-1e30, -1e30, -zr, 1e30, 1e30, zr), cylinder=(T.AXIS_Z, (0.0, 0.0, 0.0), f32(0.5) * R))
+    tri = np.ascontiguousarray(triangles, np.float32).reshape(-1, 9)
+    v = tri.reshape(-1, 3)
+    lo, hi = v.min(axis=0), v.max(axis=0)
+    half = f32(length) * f32(0.5)
+    return c, bounds(lo[0], lo[1], -half, hi[0], hi[1], half), [tri]
 
 
 def rotor_duct(nx=192, ny=96, nz=96, seed=0, levels=2, n_theta=2048, n_axial=48, tip_speed=0.08, u_inlet=0.02,

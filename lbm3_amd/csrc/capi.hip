@@ -93,6 +93,18 @@ int finish_level_setup(Sim &s, int level)
     std::vector<OpenBCRecord> rec(s.C.nOpenRules + 1);
     for (int r = 0; r <= s.C.nOpenRules; ++r) rec[r] = open_record(open_bc(s.C, r));
     if (upload(l.openRecords, rec.data(), (long long)rec.size(), s.stream)) return LBM3_E_HIP;
+    OpenProfiles prof{};
+    bool profiled = false;
+    for (int r = 0; r < s.C.nOpenRules; ++r) {
+        prof.p[r] = s.C.openRules[r].profile;
+        profiled = profiled || prof.p[r].radius > 0.f;
+    }
+    l.openScale.release();
+    if (profiled && l.nOpen > 0) {
+        HIPCHK(l.openScale.alloc(l.nOpen));
+        launch_open_scale(topo_of(l), l.geom, l.nOpen, l.openCell.p, l.openMeta.p, prof, l.openScale.p, s.stream);
+        HIPCHK(hipGetLastError());
+    }
     HIPCHK(l.gWall.alloc(3LL * (l.W > 0 ? l.W : 1)));
     HIPCHK(hipMemsetAsync(l.gWall.p, 0, sizeof(double) * (size_t)l.gWall.n, s.stream));
     HIPCHK(l.openRefs.alloc(2LL * (l.nOpen > 0 ? l.nOpen : 1) * (long long)compute_bytes(s.storage)));
@@ -289,7 +301,7 @@ template <class S> static void step_level(Sim &s, int level)
         // step's boundary condition when it is still pending
         const OpenArgs<S> o = open_args<S>(s, l, l.info.iterationsFinished - 1);
         a.open = {l.openGroups.p, l.nOpenGroups, l.openSettled ? 0 : 1, o.meta, o.records, o.dRhoRef, o.uNormalRef, o.dRhoCum,
-                  o.uNormalCum, o.iterationsFinished};
+                  o.uNormalCum, o.iterationsFinished, o.scale};
     }
     if (timed || (l.W == 0 && a.open.count == 0 && a.rotorCellCount == 0)) {
         // in order on one stream, so that each kernel's events bracket it alone
@@ -351,6 +363,7 @@ template <class S> static int init_level(Sim &s, int level, const float *fieldsD
     d.kind = fieldsDev ? 2 : s.C.init.kind;
     d.dRho = s.C.init.dRho; d.ux = s.C.init.ux; d.uy = s.C.init.uy; d.uz = s.C.init.uz;
     d.u0 = s.C.init.u0; d.length = s.C.init.length; d.pressure = s.C.init.pressure;
+    d.profile = s.C.init.profile;
     d.fields = fieldsDev;
     const bool flip = l.info.esotwistFlipper != 0;
     l.openSettled = true;
@@ -868,6 +881,73 @@ int lbm3_reset_wall_forces(lbm3_sim *h)
         if (l.gWall.p) HIPCHK(hipMemsetAsync(l.gWall.p, 0, sizeof(double) * (size_t)l.gWall.n, h->s.stream));
         for (Level::Rotor &r : l.rotors)
             if (r.g.p) HIPCHK(hipMemsetAsync(r.g.p, 0, sizeof(double) * (size_t)r.g.n, h->s.stream));
+    }
+    return 0;
+}
+
+int lbm3_open_boundary_stats(lbm3_sim *h, int openBCID, double *out)
+{
+    if (!h || !out) return set_error(LBM3_E_ARG, "null argument");
+    Sim &s = h->s;
+    if (!s.C.trackOpenBoundaries) return set_error(LBM3_E_ARG, "the case does not track open boundaries");
+    double area = 0.0, sum[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    bool found = false;
+    for (int level = 0; level < s.levels; ++level) {
+        if (const int rc = check_level(s, level)) return rc;
+        Level &l = s.L[level];
+        if (openBCID < 0 || openBCID >= l.nOpenIDs) continue;
+        found = true;
+        if (const int rc = settle_open(s, level)) return rc;
+        if (l.info.iterationsSinceTrackerReset <= 0) return set_error(LBM3_E_ARG, "no time step since the last reset");
+        int start = 0;
+        for (int b = 0; b < openBCID; ++b) start += l.openCount[b];
+        const int count = l.openCount[openBCID];
+        const long long nb = (count + 255) / 256;
+        if (s.partial.n < 7 * nb) HIPCHK(s.partial.alloc(7 * nb));
+        const long long stride = l.nOpen > 0 ? l.nOpen : 1;
+        const int blocks = launch_open_stats(l.openCum.p, l.openCum.p + stride, l.openMeta.p, start, count,
+                                             1.0 / double(l.info.iterationsSinceTrackerReset), s.partial.p, s.stream);
+        HIPCHK(hipGetLastError());
+        std::vector<double> part(7 * (size_t)blocks);
+        if (blocks > 0)
+            HIPCHK(hipMemcpyAsync(part.data(), s.partial.p, sizeof(double) * part.size(), hipMemcpyDeviceToHost, s.stream));
+        HIPCHK(hipStreamSynchronize(s.stream));
+        double v[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        for (int b = 0; b < blocks; ++b)
+            for (int k = 0; k < 7; ++k) v[k] += part[7 * (size_t)b + k];
+        // physical units (trackerFunctions.h:172-192, cellFunctions.h:288-294)
+        const double dx = double(l.info.res) / 1000.0, cellArea = dx * dx;
+        const double vel = dx / double(l.info.dtPhys), rho = double(s.C.rhoPhys);
+        const double pres = rho * vel * vel / 3.0;       // dRho -> pressure: rho_phys c_s^2
+        sum[0] += v[0] * cellArea * vel;
+        sum[1] += v[1] * cellArea * vel * rho;
+        sum[2] += v[2] * cellArea * vel * vel * rho;
+        sum[3] += v[3] * pres * cellArea;
+        sum[4] += v[4] * pres * cellArea * vel;
+        sum[5] += v[5] * cellArea * vel * vel * vel * rho;
+        area += v[6] * cellArea;
+    }
+    if (!found) return set_error(LBM3_E_ARG, "no such open boundary");
+    out[0] = area;
+    out[1] = sum[0] / area;
+    out[2] = sum[1];
+    out[3] = sum[2];
+    out[4] = sum[3] / area;
+    out[5] = sum[4];
+    out[6] = sum[5];
+    return 0;
+}
+
+int lbm3_reset_open_boundary_stats(lbm3_sim *h)
+{
+    if (!h) return set_error(LBM3_E_ARG, "null sim");
+    Sim &s = h->s;
+    for (int level = 0; level < s.levels; ++level) {
+        if (const int rc = check_level(s, level)) return rc;
+        if (const int rc = settle_open(s, level)) return rc;   // the last step's condition belongs to the old window
+        Level &l = s.L[level];
+        if (l.openCum.p) HIPCHK(hipMemsetAsync(l.openCum.p, 0, sizeof(double) * (size_t)l.openCum.n, s.stream));
+        l.info.iterationsSinceTrackerReset = 0;
     }
     return 0;
 }

@@ -105,6 +105,7 @@ template <class T> struct FusedOpen {
     T *dRhoRef, *uNormalRef;
     double *dRhoCum, *uNormalCum;
     int iter;                     // iteration the pending boundary condition belongs to
+    const float *scale;           // per open-list entry: velocity profile factor, or null (none)
 };
 
 // One rotor of a level (RotorStruct, types.h:206-219) as the kernels read it;
@@ -171,6 +172,7 @@ template <class S> struct OpenArgs {
     double *dRhoCum, *uNormalCum; // flow accumulators or null
     int iterationsFinished;
     Codec cd;
+    const float *scale;           // per entry: velocity profile factor of its rule, or null (none)
 };
 
 // Blocks are dispatched round-robin over the 8 XCDs.  With LBM3_XCD_REMAP=1

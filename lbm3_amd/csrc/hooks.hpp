@@ -20,6 +20,43 @@ __host__ __device__ inline int face_class(int idx, int count)
     return idx == 0 ? LBM3_FACE_LOW : (idx == count - 1 ? LBM3_FACE_HIGH : LBM3_FACE_MID);
 }
 
+// getXYZFromIJKCellIndex (cellFunctions.h:103-108), never contracted
+__host__ __device__ inline void cell_xyz(const LevelGeom &g, const int (&ijk)[3], float &x, float &y, float &z)
+{
+#pragma clang fp contract(off)
+    x = float(ijk[0]) * g.res + g.ox;
+    y = float(ijk[1]) * g.res + g.oy;
+    z = float(ijk[2]) * g.res + g.oz;
+}
+
+// radial profile factor max(0, 1 - r^2 / radius^2) at (x, y, z), 1 without a
+// profile; never contracted, so host, device and the oracle round alike
+__host__ __device__ inline float profile_factor(const lbm3_profile &P, float x, float y, float z)
+{
+#pragma clang fp contract(off)
+    if (!(P.radius > 0.f)) return 1.f;
+    const float dx = x - P.origin[0], dy = y - P.origin[1], dz = z - P.origin[2];
+    float r2;
+    if (P.axis == LBM3_AXIS_X) r2 = dy * dy + dz * dz;
+    else if (P.axis == LBM3_AXIS_Y) r2 = dx * dx + dz * dz;
+    else r2 = dx * dx + dy * dy;
+    const float v = 1.f - r2 / (P.radius * P.radius);
+    return 0.f < v ? v : 0.f;
+}
+
+// inside a refinement rule's cylinder (always, without one)
+__host__ __device__ inline bool in_cylinder(const lbm3_profile &P, float x, float y, float z)
+{
+#pragma clang fp contract(off)
+    if (!(P.radius > 0.f)) return true;
+    const float dx = x - P.origin[0], dy = y - P.origin[1], dz = z - P.origin[2];
+    float r2;
+    if (P.axis == LBM3_AXIS_X) r2 = dy * dy + dz * dz;
+    else if (P.axis == LBM3_AXIS_Y) r2 = dx * dx + dz * dz;
+    else r2 = dx * dx + dy * dy;
+    return r2 < P.radius * P.radius;
+}
+
 // getOpenBC: index of the first matching rule, nOpenRules for the default record
 __host__ __device__ inline int open_rule(const lbm3_case &c, int level, int X, int Y, int Z, int i, int j, int k)
 {
@@ -77,14 +114,16 @@ __host__ __device__ inline bool refine_hook(const lbm3_case &c, int level, float
     for (int r = 0; r < c.nRefineRules; ++r) {
         const lbm3_refine_rule &R = c.refineRules[r];
         if (R.level != level) continue;
-        const bool in = x > R.xMin && x < R.xMax && y > R.yMin && y < R.yMax && z > R.zMin && z < R.zMax;
+        const bool in = x > R.xMin && x < R.xMax && y > R.yMin && y < R.yMax && z > R.zMin && z < R.zMax &&
+                        in_cylinder(R.cylinder, x, y, z);
         if (R.mode == LBM3_REFINE_SET) { haveSet = true; inSet = inSet || in; }
     }
     if (haveSet) marker = inSet;
     for (int r = 0; r < c.nRefineRules; ++r) {
         const lbm3_refine_rule &R = c.refineRules[r];
         if (R.level != level) continue;
-        const bool in = x > R.xMin && x < R.xMax && y > R.yMin && y < R.yMax && z > R.zMin && z < R.zMax;
+        const bool in = x > R.xMin && x < R.xMax && y > R.yMin && y < R.yMax && z > R.zMin && z < R.zMax &&
+                        in_cylinder(R.cylinder, x, y, z);
         if (R.mode == LBM3_REFINE_ADD && in) marker = true;
         if (R.mode == LBM3_REFINE_REMOVE && in) marker = false;
     }

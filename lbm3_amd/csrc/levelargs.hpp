@@ -47,6 +47,7 @@ template <class S> inline OpenArgs<S> open_args(Sim &s, Level &l, int iter)
     a.nb = l.openNb.p;
     a.meta = l.openMeta.p;
     a.records = l.openRecords.p;
+    a.scale = l.openScale.p;
     T *refs = reinterpret_cast<T *>(l.openRefs.p);
     a.dRhoRef = refs;
     a.uNormalRef = refs + (l.nOpen > 0 ? l.nOpen : 1);

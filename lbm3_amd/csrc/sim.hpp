@@ -61,6 +61,7 @@ struct Level {
     DBuf<int> openCell, openSource;
     DBuf<int4> openNb;               // [2][nOpen] neighbour cells of each open cell
     DBuf<uint32_t> openMeta;
+    DBuf<float> openScale;           // per open entry: velocity profile factor (null: no rule has a profile)
     DBuf<OpenBCRecord> openRecords;
     DBuf<unsigned char> openRefs;    // 2 arrays of nOpen compute-type values
     DBuf<double> openCum;            // 2 arrays of nOpen

@@ -21,6 +21,7 @@
 // synchronisation.  The esotwist parity is a template parameter, which makes
 // every plane offset a compile-time multiple of N.
 #include "step.hpp"
+#include "hooks.hpp"
 
 namespace lbm3 {
 
@@ -98,7 +99,7 @@ __device__ __forceinline__ void bounce_back(T (&f)[27], const T (&pre)[27], uint
 // condition belongs to.
 template <int NI, class T>
 __device__ __forceinline__ void open_fill(T (&f)[27], int x, uint32_t meta, const OpenBCRecord *records, T *dRhoRef,
-                                          T *uNormalRef, double *dRhoCum, double *uNormalCum, int iter)
+                                          T *uNormalRef, double *dRhoCum, double *uNormalCum, int iter, const float *scale)
 {
     constexpr int nx = LBM3_BT_NORMAL[NI][0], ny = LBM3_BT_NORMAL[NI][1], nz = LBM3_BT_NORMAL[NI][2];
     // open_cell leaves the known populations as full (unshifted) values: keep
@@ -110,7 +111,11 @@ __device__ __forceinline__ void open_fill(T (&f)[27], int x, uint32_t meta, cons
         known[q] = f[q];
         if (unknown_dir(nx, ny, nz, q)) f[q] = T(0);
     }
-    const OpenBCRecord R = records[(meta >> 5) & 255u];
+    OpenBCRecord R = records[(meta >> 5) & 255u];
+    if (scale) {                   // the rule's velocity profile at this cell
+        const float sc = scale[x];
+        R.ux *= sc; R.uy *= sc; R.uz *= sc;
+    }
     const T dRef = dRhoRef[x], uRef = uNormalRef[x];
     T dRho, un;
     open_cell<NI>(f, R, dRef, uRef, dRho, un);
@@ -134,10 +139,11 @@ __device__ __forceinline__ void open_fill(T (&f)[27], int x, uint32_t meta, cons
 
 template <class T>
 __device__ __forceinline__ void open_dispatch(T (&f)[27], int x, uint32_t meta, const OpenBCRecord *records, T *dRhoRef,
-                                              T *uNormalRef, double *dRhoCum, double *uNormalCum, int iter)
+                                              T *uNormalRef, double *dRhoCum, double *uNormalCum, int iter,
+                                              const float *scale)
 {
     switch (meta & 31u) {
-#define OPEN_CASE(n) case n: open_fill<n, T>(f, x, meta, records, dRhoRef, uNormalRef, dRhoCum, uNormalCum, iter); break;
+#define OPEN_CASE(n) case n: open_fill<n, T>(f, x, meta, records, dRhoRef, uNormalRef, dRhoCum, uNormalCum, iter, scale); break;
         OPEN_CASE(0) OPEN_CASE(1) OPEN_CASE(2) OPEN_CASE(3) OPEN_CASE(4) OPEN_CASE(5) OPEN_CASE(6)
         OPEN_CASE(7) OPEN_CASE(8) OPEN_CASE(9) OPEN_CASE(10) OPEN_CASE(11) OPEN_CASE(12) OPEN_CASE(13)
         OPEN_CASE(14) OPEN_CASE(15) OPEN_CASE(16) OPEN_CASE(17) OPEN_CASE(18) OPEN_CASE(19) OPEN_CASE(20)
@@ -342,7 +348,7 @@ __global__ __launch_bounds__(256) void k_open_groups(StepArgs<S> a)
     if (code <= -8 && a.open.pending) {
         const int x = -(code + 8);
         open_dispatch<T>(f, x, a.open.meta[x], a.open.records, a.open.dRhoRef, a.open.uNormalRef, a.open.dRhoCum,
-                         a.open.uNormalCum, a.open.iter);
+                         a.open.uNormalCum, a.open.iter, a.open.scale);
     }
     collide<T, BULK>(f, relax_of<T>(a.bulk));
 #pragma unroll
@@ -568,7 +574,8 @@ __device__ __forceinline__ void open_one(const OpenArgs<S> &a, int x, const int 
     for (int q = 0; q < 27; ++q)
         f[q] = unknown_dir(nx, ny, nz, q) ? T(0)
                                           : StoreTraits<S>::load(a.f + pop_index(pre_plane(q, FLIP), nb[pre_nb(q)]), q, a.cd);
-    open_fill<NI, T>(f, x, meta, a.records, a.dRhoRef, a.uNormalRef, a.dRhoCum, a.uNormalCum, a.iterationsFinished);
+    open_fill<NI, T>(f, x, meta, a.records, a.dRhoRef, a.uNormalRef, a.dRhoCum, a.uNormalCum, a.iterationsFinished,
+                     a.scale);
 #pragma unroll
     for (int q = 0; q < 27; ++q)
         if (unknown_dir(nx, ny, nz, q))
@@ -636,6 +643,11 @@ __global__ __launch_bounds__(256) void k_init(S *f, Topology topo, InitDesc d, L
     } else if (d.kind == 2) {
         dRho = d.fields[4 * (size_t)cell]; ux = d.fields[4 * (size_t)cell + 1];
         uy = d.fields[4 * (size_t)cell + 2]; uz = d.fields[4 * (size_t)cell + 3];
+    } else if (d.profile.radius > 0.f) {
+        float x, y, z;
+        cell_xyz(g, ijk, x, y, z);
+        const float sc = profile_factor(d.profile, x, y, z);
+        ux *= sc; uy *= sc; uz *= sc;
     }
     T v[27];
 #pragma unroll
@@ -725,6 +737,43 @@ __global__ __launch_bounds__(256) void k_sum_partials(const double *partial, int
     for (int i = threadIdx.x; i < n; i += 256) v += partial[i];
     const double s = block_sum(v, lds);
     if (threadIdx.x == 0) *out = s;
+}
+
+// open-boundary flow statistics (trackerFunctions.h:145-170): per block, the
+// sums over the tracked entries [start, start + count) of the open list of
+// uNormal, rho uNormal, rho uNormal |uNormal|, dRho, dRho uNormal,
+// rho uNormal^3 / 2 (time averages of the accumulators, lattice units) and the
+// tracked-entry count; partial[block * 7 + k]
+__global__ __launch_bounds__(256) void k_open_stats(const double *dRhoCum, const double *uNormalCum, const uint32_t *meta,
+                                                    int start, int count, double invSpan, double *partial)
+{
+    __shared__ double lds[7 * 4];
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    double v[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    if (i < count && ((meta[start + i] >> 13) & 1u)) {
+        const double dRho = dRhoCum[start + i] * invSpan, un = uNormalCum[start + i] * invSpan;
+        const double rho = 1.0 + dRho;
+        v[0] = un;
+        v[1] = rho * un;
+        v[2] = rho * un * fabs(un);
+        v[3] = dRho;
+        v[4] = dRho * un;
+        v[5] = 0.5 * rho * un * un * un;
+        v[6] = 1.0;
+    }
+#pragma unroll
+    for (int k = 0; k < 7; ++k) {
+        const double sum = block_sum(v[k], lds + 4 * k);
+        if (threadIdx.x == 0) partial[7 * (size_t)blockIdx.x + k] = sum;
+    }
+}
+
+int launch_open_stats(const double *dRhoCum, const double *uNormalCum, const uint32_t *meta, int start, int count,
+                      double invSpan, double *partial, hipStream_t s)
+{
+    const int blocks = (count + 255) / 256;
+    if (blocks > 0) k_open_stats<<<blocks, 256, 0, s>>>(dRhoCum, uNormalCum, meta, start, count, invSpan, partial);
+    return blocks;
 }
 
 // per-wall-body sums of the accumulated force and its torque about `origin`;
@@ -911,6 +960,27 @@ void launch_open_cells(const StepArgs<S> &a, bool flip, hipStream_t s)
     const bool k17 = relax_of<typename StoreTraits<S>::compute>(a.bulk).k17 != 0;
     if (flip) { if (k17) hipLaunchKernelGGL((k_open_groups<S, true, 1, G>), g, b, 0, s, a); else hipLaunchKernelGGL((k_open_groups<S, true, 2, G>), g, b, 0, s, a); }
     else { if (k17) hipLaunchKernelGGL((k_open_groups<S, false, 1, G>), g, b, 0, s, a); else hipLaunchKernelGGL((k_open_groups<S, false, 2, G>), g, b, 0, s, a); }
+}
+
+// velocity profile factor of each open-list entry: its rule's radial profile
+// at the cell centre (getOpenBC of the pipe cases, poiseuilleFlowMain.cu:58-75)
+__global__ __launch_bounds__(256) void k_open_scale(Topology t, LevelGeom g, int count, const int *cell,
+                                                     const uint32_t *meta, OpenProfiles P, float *scale)
+{
+    const int x = blockIdx.x * 256 + threadIdx.x;
+    if (x >= count) return;
+    int nb[8], ijk[3];
+    decode(t, cell[x], nb, ijk);
+    float px, py, pz;
+    cell_xyz(g, ijk, px, py, pz);
+    scale[x] = profile_factor(P.p[(meta[x] >> 5) & 255u], px, py, pz);
+}
+
+void launch_open_scale(const Topology &t, const LevelGeom &g, int count, const int *cell, const uint32_t *meta,
+                       const OpenProfiles &P, float *scale, hipStream_t s)
+{
+    if (count <= 0) return;
+    k_open_scale<<<(count + 255) / 256, 256, 0, s>>>(t, g, count, cell, meta, P, scale);
 }
 
 void launch_open_nb(const Topology &t, int count, const int *cell, int4 *out, hipStream_t s)

@@ -12,6 +12,12 @@ struct InitDesc {
     float u0, length;
     int pressure;
     const float *fields;      // kind 2: [N][4] (dRho, ux, uy, uz), device
+    lbm3_profile profile;     // kind 0: velocity scaled by the profile factor of the cell
+};
+
+// the radial profile of every open rule, and none for the default record
+struct OpenProfiles {
+    lbm3_profile p[LBM3_MAX_RULES + 1];
 };
 
 // bulk cells (all but wall-adjacent and solid) and wall-adjacent cells; together one collide-stream step
@@ -45,6 +51,10 @@ void launch_wall_force(const double *gWall, int W, const uint32_t *wallData, con
                        const LevelGeom &g, const double origin[3], int nBodies, double *partial, hipStream_t s);
 template <class T> void launch_collide_batch(T *f, int n, const Relax<T> &R, hipStream_t s);
 void launch_open_nb(const Topology &t, int count, const int *cell, int4 *out, hipStream_t s);
+int launch_open_stats(const double *dRhoCum, const double *uNormalCum, const uint32_t *meta, int start, int count,
+                      double invSpan, double *partial, hipStream_t s);
+void launch_open_scale(const Topology &t, const LevelGeom &g, int count, const int *cell, const uint32_t *meta,
+                       const OpenProfiles &P, float *scale, hipStream_t s);
 void launch_fast_topo(const Topology &t, const int *wallMap, const int *openOf, const uint8_t *rotorCand, int2 *cellRow,
                       int4 *rows, hipStream_t s);
 void launch_scatter_index(const int *cell, int count, int *openOf, hipStream_t s);

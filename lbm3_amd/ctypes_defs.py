@@ -42,8 +42,13 @@ def default_bc(**kw):
     return b
 
 
+class Profile(C.Structure):
+    """max(0, 1 - r^2 / radius^2) about the line through origin along axis; radius <= 0: none."""
+    _fields_ = [("axis", C.c_int), ("origin", C.c_float * 3), ("radius", C.c_float)]
+
+
 class OpenRule(C.Structure):
-    _fields_ = [("levelMask", C.c_int), ("faceMask", C.c_int * 3), ("bc", BC)]
+    _fields_ = [("levelMask", C.c_int), ("faceMask", C.c_int * 3), ("bc", BC), ("profile", Profile)]
 
 
 class LocalRule(C.Structure):
@@ -56,12 +61,12 @@ class LocalRule(C.Structure):
 
 class Init(C.Structure):
     _fields_ = [("kind", C.c_int), ("dRho", C.c_float), ("ux", C.c_float), ("uy", C.c_float), ("uz", C.c_float),
-                ("u0", C.c_float), ("length", C.c_float), ("pressure", C.c_int)]
+                ("u0", C.c_float), ("length", C.c_float), ("pressure", C.c_int), ("profile", Profile)]
 
 
 class RefineRule(C.Structure):
     _fields_ = [("level", C.c_int), ("mode", C.c_int)] + \
-               [(n, C.c_float) for n in ("xMin", "yMin", "zMin", "xMax", "yMax", "zMax")]
+               [(n, C.c_float) for n in ("xMin", "yMin", "zMin", "xMax", "yMax", "zMax")] + [("cylinder", Profile)]
 
 
 MAX_ROTORS = 8

@@ -53,6 +53,8 @@ def load():
         "lbm3_kinetic_energy": (i32, [vp, C.POINTER(f64)]),
         "lbm3_wall_forces": (i32, [vp, i32, C.POINTER(f64), i32, C.POINTER(f64)]),
         "lbm3_reset_wall_forces": (i32, [vp]),
+        "lbm3_open_boundary_stats": (i32, [vp, i32, C.POINTER(f64)]),
+        "lbm3_reset_open_boundary_stats": (i32, [vp]),
         "lbm3_build_rotors": (i32, [vp, vp, vp, i32, i32]),
         "lbm3_set_rotor": (i32, [vp, i32, i32, C.c_float, i32, C.c_float, C.c_float, C.c_float]),
         "lbm3_rotor_info": (i32, [vp, i32, i32, C.POINTER(f64)]),
@@ -247,6 +249,19 @@ class Sim:
         out = (C.c_double * (6 * bodies))()
         check(load().lbm3_wall_forces(self.h, level, o, bodies, out))
         return np.array(out[:]).reshape(bodies, 6)
+
+    OPEN_STATS = ("area", "normal_velocity", "mass_flow", "momentum_thrust", "pressure", "pressure_power",
+                  "normal_kinetic_power")
+
+    def open_boundary_stats(self, open_bc_id):
+        """updateTracker's open-boundary statistics of one boundary since the last
+        reset, physical units (area m^2, velocity m/s, pressure Pa, ...)."""
+        out = (C.c_double * 7)()
+        check(load().lbm3_open_boundary_stats(self.h, open_bc_id, out))
+        return dict(zip(self.OPEN_STATS, out[:]))
+
+    def reset_open_boundary_stats(self):
+        check(load().lbm3_reset_open_boundary_stats(self.h))
 
     def build_rotors(self, rotors, level_mask=0xFF):
         """buildRotors on the levels of level_mask; rotors: list of (n, 9) triangle arrays."""

@@ -33,6 +33,32 @@ static inline void xyz_of(const lbm3_info *I, int i, int j, int k, float *x, flo
     *z = (float)k * I->res + I->oz;
 }
 
+/* radial profile max(0, 1 - r^2 / radius^2) of a pipe case's hooks
+   (poiseuilleFlowMain.cu:54-55,66-72); radius <= 0: 1 */
+float orc_profile_factor(const lbm3_profile *P, float x, float y, float z)
+{
+    if (!(P->radius > 0.f)) return 1.f;
+    const float dx = x - P->origin[0], dy = y - P->origin[1], dz = z - P->origin[2];
+    float r2;
+    if (P->axis == LBM3_AXIS_X) r2 = dy * dy + dz * dz;
+    else if (P->axis == LBM3_AXIS_Y) r2 = dx * dx + dz * dz;
+    else r2 = dx * dx + dy * dy;
+    const float v = 1.f - r2 / (P->radius * P->radius);
+    return 0.f < v ? v : 0.f;
+}
+
+/* the refinement cylinder of poiseuilleFlowMain.cu:38-45 (r^2 < radius^2) */
+static int in_cylinder(const lbm3_profile *P, float x, float y, float z)
+{
+    if (!(P->radius > 0.f)) return 1;
+    const float dx = x - P->origin[0], dy = y - P->origin[1], dz = z - P->origin[2];
+    float r2;
+    if (P->axis == LBM3_AXIS_X) r2 = dy * dy + dz * dz;
+    else if (P->axis == LBM3_AXIS_Y) r2 = dx * dx + dz * dz;
+    else r2 = dx * dx + dy * dy;
+    return r2 < P->radius * P->radius;
+}
+
 static inline int face_class(int idx, int count)
 {
     if (idx == 0) return LBM3_FACE_LOW;
@@ -48,7 +74,16 @@ lbm3_bc orc_hook_open_bc(const lbm3_case *C, const lbm3_info *I, int i, int j, i
     for (int r = 0; r < C->nOpenRules; ++r) {
         const lbm3_open_rule *R = &C->openRules[r];
         if (R->levelMask && !(R->levelMask & (1 << level))) continue;
-        if ((R->faceMask[0] & fc[0]) && (R->faceMask[1] & fc[1]) && (R->faceMask[2] & fc[2])) return R->bc;
+        if ((R->faceMask[0] & fc[0]) && (R->faceMask[1] & fc[1]) && (R->faceMask[2] & fc[2])) {
+            lbm3_bc b = R->bc;
+            if (R->profile.radius > 0.f) {
+                float x, y, z;
+                xyz_of(I, i, j, k, &x, &y, &z);
+                const float s = orc_profile_factor(&R->profile, x, y, z);
+                b.ux *= s; b.uy *= s; b.uz *= s;
+            }
+            return b;
+        }
     }
     return C->openDefault;
 }
@@ -63,14 +98,16 @@ static void orc_hook_refine(const lbm3_case *C, const lbm3_info *I, int i, int j
     for (int r = 0; r < C->nRefineRules; ++r) {
         const lbm3_refine_rule *R = &C->refineRules[r];
         if (R->level != level) continue;
-        const int in = x > R->xMin && x < R->xMax && y > R->yMin && y < R->yMax && z > R->zMin && z < R->zMax;
+        const int in = x > R->xMin && x < R->xMax && y > R->yMin && y < R->yMax && z > R->zMin && z < R->zMax &&
+                       in_cylinder(&R->cylinder, x, y, z);
         if (R->mode == LBM3_REFINE_SET) { haveSet = 1; inSet |= in; }
     }
     if (haveSet) *marker = (uint8_t)inSet;
     for (int r = 0; r < C->nRefineRules; ++r) {
         const lbm3_refine_rule *R = &C->refineRules[r];
         if (R->level != level) continue;
-        const int in = x > R->xMin && x < R->xMax && y > R->yMin && y < R->yMax && z > R->zMin && z < R->zMax;
+        const int in = x > R->xMin && x < R->xMax && y > R->yMin && y < R->yMax && z > R->zMin && z < R->zMax &&
+                       in_cylinder(&R->cylinder, x, y, z);
         if (R->mode == LBM3_REFINE_ADD && in) *marker = 1;
         if (R->mode == LBM3_REFINE_REMOVE && in) *marker = 0;
     }

@@ -153,6 +153,9 @@ int orc_build_rotors_from_triangles(orc_sim *S, const float *tri, const int *tri
 float orc_rotor_fraction_f32(const orc_sim *S, int L, int r, float angle, float x, float y, float z);
 int orc_rotor_info(const orc_sim *S, int L, int r, double *out12);
 int orc_rotor_forces(const orc_sim *S, int L, int r, double *out6);
+/* open-boundary flow statistics and their reset (trackerFunctions.h:127-223,476-486) */
+int orc_open_boundary_stats(const orc_sim *S, int openBCID, double *out7);
+void orc_reset_open_boundary_stats(orc_sim *S);
 
 #ifdef __cplusplus
 }

@@ -9,6 +9,7 @@
 #include "../lbm3_amd/csrc/generated/boundary_tables.h"
 
 lbm3_bc orc_hook_open_bc(const lbm3_case *C, const lbm3_info *I, int i, int j, int k);
+float orc_profile_factor(const lbm3_profile *P, float x, float y, float z);
 
 /* the per-cell numerics are compiled in numerics.c; re-include them here
    (static inline helpers + the bcr type) under the same names */
@@ -186,6 +187,81 @@ double orc_kinetic_energy(const orc_sim *S)
         total += sum * scale;
     }
     return total;
+}
+
+/* open-boundary flow statistics of updateTracker, trackerFunctions.h:127-223.
+   The FP32 oracle rounds every operation to FP32 as the reference does (R32),
+   the FP64 oracle keeps FP64; the cell sums are FP64 in both (the reference's
+   tree reduction is FP32).  out7 = {area, normal velocity, mass flow, momentum
+   thrust, pressure, pressure power, normal kinetic power}; returns -1 when no
+   level has that boundary. */
+#define R32(v) (dbl ? (double)(v) : (double)(float)(v))
+int orc_open_boundary_stats(const orc_sim *S, int openBCID, double *out7)
+{
+    const int dbl = S->isDouble;
+    double aream2 = 0.0, res[6] = {0, 0, 0, 0, 0, 0};
+    int found = 0;
+    for (int L = 0; L < S->levels; ++L) {
+        const orc_grid *g = &S->g[L];
+        if (openBCID >= g->nOpen) continue;
+        found = 1;
+        const orc_openbc *O = &g->open[openBCID];
+        const lbm3_info *I = &g->Info;
+        const double invIterationSpan = R32(1.0 / R32((double)I->iterationsSinceTrackerReset));
+        double acc[6] = {0, 0, 0, 0, 0, 0};
+        for (int x = 0; x < O->count; ++x) {
+            const double dc = dbl ? ((const double *)O->dRhoCum)[x] : ((const float *)O->dRhoCum)[x];
+            const double uc = dbl ? ((const double *)O->uNormalCum)[x] : ((const float *)O->uNormalCum)[x];
+            const double dRho = R32(dc * invIterationSpan);
+            const double uNormal = R32(uc * invIterationSpan);
+            const double rho = R32(1.0 + dRho);
+            acc[0] += uNormal;
+            acc[1] += R32(rho * uNormal);
+            acc[2] += R32(R32(rho * uNormal) * fabs(uNormal));
+            acc[3] += dRho;
+            acc[4] += R32(dRho * uNormal);
+            acc[5] += R32(R32(R32(R32(0.5 * rho) * uNormal) * uNormal) * uNormal);
+        }
+        double r[6];
+        for (int k = 0; k < 6; ++k) r[k] = R32(acc[k]);
+        const double rhoPhys = S->C.rhoPhys;
+        const double dx = R32((double)I->res / 1000.0);
+        const double cellAream2 = R32(dx * dx);
+        const double vel = R32(dx / (double)I->dtPhys);
+        const double cs = R32(R32(R32(0.57735026918962576) * dx) / (double)I->dtPhys);
+        r[0] = R32(r[0] * R32(cellAream2 * vel));
+        r[1] = R32(r[1] * R32(R32(cellAream2 * vel) * rhoPhys));
+        r[2] = R32(r[2] * R32(R32(R32(cellAream2 * vel) * vel) * rhoPhys));
+        r[3] = R32(R32(R32(R32(r[3] * rhoPhys) * cs) * cs) * cellAream2);   /* convertToPhysicalPressure */
+        r[4] = R32(R32(R32(R32(r[4] * rhoPhys) * cs) * cs) * R32(cellAream2 * vel));
+        r[5] = R32(r[5] * R32(R32(R32(R32(cellAream2 * vel) * vel) * vel) * rhoPhys));
+        aream2 = R32(aream2 + R32((double)O->trackFlowCount * cellAream2));
+        for (int k = 0; k < 6; ++k) res[k] = R32(res[k] + r[k]);
+    }
+    if (!found) return -1;
+    out7[0] = aream2;
+    out7[1] = R32(res[0] / aream2);
+    out7[2] = res[1];
+    out7[3] = res[2];
+    out7[4] = R32(res[3] / aream2);
+    out7[5] = res[4];
+    out7[6] = res[5];
+    return 0;
+}
+#undef R32
+
+/* the accumulator reset at the end of updateTracker, trackerFunctions.h:476-486 */
+void orc_reset_open_boundary_stats(orc_sim *S)
+{
+    for (int L = 0; L < S->levels; ++L) {
+        orc_grid *g = &S->g[L];
+        g->Info.iterationsSinceTrackerReset = 0;
+        const size_t rs = S->isDouble ? 8 : 4;
+        for (int b = 0; b < g->nOpen; ++b) {
+            memset(g->open[b].dRhoCum, 0, rs * (size_t)g->open[b].count);
+            memset(g->open[b].uNormalCum, 0, rs * (size_t)g->open[b].count);
+        }
+    }
 }
 
 /* advance the whole hierarchy n coarse steps */

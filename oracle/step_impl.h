@@ -82,6 +82,11 @@ static void NAME(hook_init)(const lbm3_case *C, const lbm3_info *I, int i, int j
         BC->dRho = In->pressure ? (3.f * u0 * u0 / 16.f) * (cosf(2.f * X) + cosf(2.f * Y)) * (2.f + cosf(2.f * Z)) : 0.f;
     } else {
         BC->dRho = In->dRho; BC->ux = In->ux; BC->uy = In->uy; BC->uz = In->uz;
+        if (In->profile.radius > 0.f) {
+            const float x = (float)i * I->res + I->ox, y = (float)j * I->res + I->oy, z = (float)k * I->res + I->oz;
+            const float s = orc_profile_factor(&In->profile, x, y, z);
+            BC->ux = (float)BC->ux * s; BC->uy = (float)BC->uy * s; BC->uz = (float)BC->uz * s;
+        }
     }
 }
 

@@ -68,3 +68,66 @@ def tgv_reference_case(name):
     if samples[-1] != iterations:
         samples.append(iterations)
     return case, dom, tris, np.array(samples), 1.0 / (n ** 3 * u0 * u0)
+
+
+# ---------------------------------------------------------------- cell positions, pipe inlet histories
+def cell_ijk(sim, level=0):
+    """(i, j, k) of every cell from the compressed IJK arrays (cellFunctions.h:21-94);
+    `sim` is a lib.Sim or an OracleSim."""
+    sh = sim.array("shifter", level).astype(np.int64)
+    ri, rj, rk = (sim.array(n, level).astype(np.int64) for n in ("rowI", "rowJ", "rowK"))
+    cell = np.arange(len(sh))
+    head = np.where(sh >= 0, cell, cell + sh)
+    row = sh[head]
+    back = cell - head
+    return np.stack([ri[row] + back, rj[row], rk[row]], axis=1)
+
+
+def open_cells(sim, bcid, level=0):
+    """Cells of open boundary `bcid` (the oracle names them open<b>Index, the
+    library concatenates every boundary in openIndex)."""
+    if hasattr(sim, "open_count"):
+        return sim.array("open%dIndex" % bcid, level)
+    counts = sim.array("openCount", level)
+    start = int(np.sum(counts[:bcid]))
+    return sim.array("openIndex", level)[start:start + int(counts[bcid])]
+
+
+def pipe_inlet_history(name):
+    """Rows (iteration, inlet velocity m/s, inlet pressure Pa, relative L2 error of
+    the inlet velocity profile) of a reference-held pipe inlet history."""
+    return np.loadtxt(os.path.join(GOLDEN, "pipe_inlet_%s.txt" % name), delimiter=";", skiprows=1)
+
+
+def pipe_reference_case(name):
+    """The reference's non-reflective pipe test (nonReflectiveTestMain.cu) with the
+    characteristic non-reflective inlet or the strict Dirichlet inlet."""
+    from lbm3_amd import cases, geometry
+    tri = geometry.read_binary_stl(os.path.join(GOLDEN, "pipeD20L200STL.STL"))
+    return cases.pipe(tri, inlet_non_reflective=(name == "characteristic"))
+
+
+def pipe_inlet_profile_error(sim, case):
+    """getInletProfileError (nonReflectiveTestMain.cu:114-196): relative L2 error of
+    uz against the prescribed profile over the counted cells of open boundary 0
+    on the k = 0 face, from the current macroscopic fields."""
+    f32 = np.float32
+    m = sim.macros(0)
+    cells = open_cells(sim, 0)
+    wm = sim.array("wallMap", 0)[cells]
+    wd = sim.array("wallData", 0)
+    keep = (wm != -3) & (wm != -2)
+    keep &= ~((wm >= 0) & ((wd[np.maximum(wm, 0)] & 1) != 0))
+    ijk = cell_ijk(sim, 0)[cells]
+    keep &= ijk[:, 2] == 0
+    info = sim.info(0)
+    x = ijk[:, 0].astype(f32) * f32(info.res) + f32(info.ox)
+    y = ijk[:, 1].astype(f32) * f32(info.res) + f32(info.oy)
+    rule = case.openRules[0]
+    R = f32(rule.profile.radius)
+    r2 = x * x + y * y
+    target = f32(rule.bc.uz) * np.maximum(f32(0.0), f32(1.0) - r2 / (R * R))
+    err = m[cells, 3].astype(f32) - target
+    e2 = np.sum((err[keep].astype(np.float64)) ** 2)
+    t2 = np.sum((target[keep].astype(np.float64)) ** 2)
+    return float(np.sqrt(e2 / t2))

@@ -57,6 +57,8 @@ def lib():
         L.orc_rotor_fraction_f32.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float]
         L.orc_rotor_info.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p]
         L.orc_rotor_forces.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p]
+        L.orc_open_boundary_stats.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
+        L.orc_reset_open_boundary_stats.argtypes = [C.c_void_p]
         for t, real in (("f32", C.c_float), ("f64", C.c_double)):
             getattr(L, "orc_if_values_batch_" + t).argtypes = [C.c_void_p, C.c_int, real, C.c_void_p]
             getattr(L, "orc_if_coefficients_" + t).argtypes = [C.c_void_p, C.c_void_p]
@@ -142,6 +144,17 @@ class OracleSim:
 
     def rotor_fraction(self, level, rotor, angle, x, y, z):
         return lib().orc_rotor_fraction_f32(self.h, level, rotor, angle, x, y, z)
+
+    def open_boundary_stats(self, open_bc_id):
+        """updateTracker's open-boundary statistics (same keys as lib.Sim)."""
+        out = np.zeros(7)
+        if lib().orc_open_boundary_stats(self.h, open_bc_id, out.ctypes.data) != 0:
+            raise KeyError(open_bc_id)
+        from lbm3_amd.lib import Sim
+        return dict(zip(Sim.OPEN_STATS, out.tolist()))
+
+    def reset_open_boundary_stats(self):
+        lib().orc_reset_open_boundary_stats(self.h)
 
     def open_count(self, level=0):
         return lib().orc_open_count(self.h, level)

@@ -1,0 +1,92 @@
+"""The GPU path against the reference's own non-reflective pipe test over its
+whole run (nonReflectiveTestMain.cu: 124 x 124 x 500 cells of 0.2 mm built
+from the reference's pipe STL, Re 1000, parabolic velocity inlet from rest,
+pressure outlet, K15, no-slip wall without interpolated bounceback; FP32
+storage).  Every iteration the open-boundary statistics
+(lbm3_open_boundary_stats, updateTracker's open-boundary part) are read and
+reset as the reference's case does; they are compared with the reference-held
+inlet histories (cases/nonReflectiveTest/{characteristic,dirichlet}.txt, every
+iteration up to 2000, every 100th after that, copied under tests/golden/):
+area-averaged inlet velocity within 5e-6 relative, inlet pressure within
+3e-4 Pa (it swings between -10 and 17 Pa), through the start-up transient
+and the acoustic waves the characteristic inlet lets out, to the steady
+state; the relative L2 error of the inlet velocity profile within 1e-3 of
+the reference's at the first iterations and at iteration 1000."""
+import numpy as np
+import pytest
+
+from lbm3_amd import lib
+from tests.helpers import pipe_inlet_history, pipe_inlet_profile_error, pipe_reference_case
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("name,steps", [("characteristic", 50000), ("dirichlet", 5000)])
+def test_inlet_history_matches_the_reference(name, steps):
+    case, dom, tris = pipe_reference_case(name)
+    rows = {int(r[0]): r for r in pipe_inlet_history(name)}
+    s = lib.Sim(case, "f32")
+    s.build(dom, tris)
+    s.init()
+    worst_u = worst_p = 0.0
+    for it in range(1, steps + 1):
+        s.run(1)
+        st = s.open_boundary_stats(0)
+        s.reset_open_boundary_stats()
+        r = rows.get(it)
+        if r is None:
+            continue
+        worst_u = max(worst_u, abs(st["normal_velocity"] - r[1]) / abs(r[1]))
+        worst_p = max(worst_p, abs(st["pressure"] - r[2]))
+        if it in (1, 2, 3, 1000) and r[3] > 1e-4:
+            e = pipe_inlet_profile_error(s, case)
+            assert abs(e - r[3]) <= 1e-3 * r[3], (it, e, r[3])
+    s.close()
+    print("%s, %d steps: inlet velocity max rel dev %.2e, pressure max dev %.2e Pa" % (name, steps, worst_u, worst_p))
+    assert worst_u < 5e-6 and worst_p < 3e-4
+
+
+def _pipe(levels):
+    from lbm3_amd import cases, geometry
+    from tests.helpers import GOLDEN
+    import os
+    tri = geometry.read_binary_stl(os.path.join(GOLDEN, "pipeD20L200STL.STL"))
+    # the reference's Poiseuille case (poiseuilleFlowMain.cu): Re 100, mean inlet
+    # velocity 0.01, Poiseuille profile at the start; 0.4 mm cells, a 20 mm piece
+    return cases.pipe(tri, length=20.0, res=0.4, reynolds=100.0, u_mean=0.01, profile_init=True, levels=levels)
+
+
+@pytest.mark.parametrize("levels", [1, 2])
+def test_pipe_with_profiles_matches_the_oracle_f64(levels):
+    """Velocity profiles at the inlet and in the initial state, the refinement
+    cylinder (two levels) and the open-boundary statistics, GPU against the
+    oracle in FP64: grid arrays bit-exact, fields within 1e-10 after 20 coarse
+    steps, every statistic of both boundaries within 1e-9 over two windows
+    (the accumulators reset in between)."""
+    from tests.oracle_lib import OracleSim
+    from tests.helpers import rel_max
+    case, dom, tris = _pipe(levels)
+    o = OracleSim(case, dom, tris, double=True)
+    g = lib.Sim(case, "f64")
+    g.build(dom, tris)
+    for L in range(levels):
+        for name in ("shifter", "wallMap", "wallData"):
+            assert np.array_equal(g.array(name, L), o.array(name, L)), (name, L)
+    o.init()
+    g.init()
+    for window in (7, 13):
+        o.run(window)
+        g.run(window)
+        for b in (0, 1):
+            so, sg = o.open_boundary_stats(b), g.open_boundary_stats(b)
+            for k in lib.Sim.OPEN_STATS:
+                assert abs(sg[k] - so[k]) <= 1e-9 * max(abs(so[k]), 1e-12), (window, b, k, sg[k], so[k])
+        o.reset_open_boundary_stats()
+        g.reset_open_boundary_stats()
+    g.sync()
+    for L in range(levels):
+        mo, mg = o.macros(L), g.macros(L)
+        solid = o.array("wallMap", L) == -3
+        assert rel_max(1.0 + mg[~solid, 0], 1.0 + mo[~solid, 0]) < 1e-10
+        assert rel_max(mg[~solid, 1:], mo[~solid, 1:]) < 1e-10
+    g.close()

@@ -1,0 +1,61 @@
+"""The oracle against the reference's own non-reflective pipe test
+(nonReflectiveTestMain.cu: the reference's 20 mm pipe STL, 124 x 124 x 500
+cells of 0.2 mm, Re 1000, parabolic velocity inlet from rest, pressure outlet,
+K15, no-slip wall without interpolated bounceback).  The reference holds the
+inlet history of its run for the characteristic non-reflective inlet and for
+the strict Dirichlet inlet (cases/nonReflectiveTest/{characteristic,
+dirichlet}.txt, copied under tests/golden/): the area-averaged inlet velocity
+and pressure of updateTracker (trackerFunctions.h:127-223) and the relative L2
+error of the inlet velocity profile, every iteration.
+
+The first iterations pin the oracle's pipe geometry (voxelizer and builder on
+the STL), the open boundaries (the characteristic update of
+updateGrid.h:267-307, MBBC, restore), the no-slip wall and the tracker sums:
+the oracle reproduces the printed 9-digit values to 5e-7 (its FP64 sums against
+the reference's FP32 tree sums).  The strict Dirichlet inlet holds the
+velocity at -0.749436736 m/s from the first step, the characteristic inlet
+lets it fall to -0.47: a dropped or mis-signed term of the characteristic
+update moves the first value far beyond the bar."""
+import numpy as np
+import pytest
+
+from tests.helpers import pipe_inlet_history, pipe_inlet_profile_error, pipe_reference_case
+from tests.oracle_lib import OracleSim
+
+
+@pytest.mark.parametrize("name,steps", [("characteristic", 20), ("dirichlet", 5)])
+def test_oracle_inlet_history_matches_the_reference(name, steps):
+    case, dom, tris = pipe_reference_case(name)
+    o = OracleSim(case, dom, tris, double=False)
+    info = o.info()
+    assert (info.cellCountX, info.cellCountY, info.cellCountZ) == (124, 124, 500)
+    o.init()
+    ref = pipe_inlet_history(name)
+    for it in range(1, steps + 1):
+        o.run(1)
+        st = o.open_boundary_stats(0)
+        o.reset_open_boundary_stats()
+        r = ref[it - 1]
+        assert r[0] == it
+        assert abs(st["normal_velocity"] - r[1]) <= 5e-7 * abs(r[1]), (it, st["normal_velocity"], r[1])
+        assert abs(st["pressure"] - r[2]) <= 5e-7 * abs(r[2]) + 1e-6, (it, st["pressure"], r[2])
+        if it <= 3:
+            # the strict inlet's profile error, ~2e-6, is FP32 rounding of uz
+            # itself: there only its size is compared (5e-8)
+            e = pipe_inlet_profile_error(o, case)
+            assert abs(e - r[3]) <= 1e-6 * r[3] + 5e-8, (it, e, r[3])
+
+
+def test_inlet_area_is_the_counted_bore_cells():
+    """Tracked inlet area = (number of inlet cells that are fluid and not
+    ghosts) x (0.2 mm)^2, within 1% of the bore's pi (10 mm)^2."""
+    case, dom, tris = pipe_reference_case("dirichlet")
+    o = OracleSim(case, dom, tris, double=False)
+    o.init()
+    o.run(1)
+    st = o.open_boundary_stats(0)
+    n = o.open_count(0)
+    assert n >= 1
+    counted = st["area"] / (0.2e-3 ** 2)
+    assert abs(counted - round(counted)) < 1e-3
+    assert abs(st["area"] / (np.pi * 1e-4) - 1.0) < 0.01
