@@ -215,6 +215,38 @@ def pipe(triangles, length=100.0, res=0.2, diameter=20.0, reynolds=1000.0, u_mea
     return c, bounds(lo[0], lo[1], -half, hi[0], hi[1], half), [tri]
 
 
+def couette(r_inner=12.0, r_outer=20.0, length=8.0, reynolds=10.0, u_wall=0.02, n_theta=512):
+    """Circular Couette flow, the set-up of the reference's torque test
+    (torqueTestMain.cu:1-112) at a small size: a solid inner cylinder (wall 1)
+    turning at the angular velocity that gives it the surface speed u_wall
+    (lattice units), inside a fixed outer tube (wall 0) that also covers the
+    domain corners; periodic along the axis z, fluid at rest at the start, K15,
+    interpolated bounceback on both walls.  Lengths in cells of 1 mm;
+    Re = u_wall (r_outer - r_inner) / nu.  Both meshes extend beyond the
+    periodic ends.  Returns (case, domain, [outer tube, inner cylinder], info)
+    with info: omega (rad/s), nu (lattice), the radii and the length."""
+    res = f32(1.0)
+    nu = f32(u_wall) * f32(r_outer - r_inner) / f32(reynolds)
+    dt = f32(1.0)
+    c = _case(1, res, dt, f32(1.0))
+    c.nuPhys = float(nu * (res / f32(1000.0)) * (res / f32(1000.0)) / dt)
+    c.trackWallForce = 1
+    c.init.kind = T.INIT_UNIFORM
+    omega = float(f32(u_wall) / f32(r_inner) / dt)     # rad/s, lattice speed u_wall at r_inner
+    add_open_rule(c, z=T.FACE_LOW | T.FACE_HIGH, periodic=1)
+    add_local_rule(c, limiter=0.0)
+    add_local_rule(c, wall_id=0, velocity=(0.0, 0.0, 0.0))
+    add_local_rule(c, wall_id=1, omega=(0.0, 0.0, omega), origin=(0.0, 0.0, 0.0))
+    h = float(r_outer) + 3.0
+    half = 0.5 * float(length)
+    eps = 1e-4
+    dom = bounds(-h, -h, -half - 1.0 - eps, h, h, half + 1.0 + eps)
+    outer = geometry.tube(r_outer, 1.5 * h, length + 8.0, axis=2, n_theta=n_theta)
+    inner = geometry.rotor(1, r_inner, 1.0, 0.0, length + 8.0, axis=2, n_theta=n_theta, n_axial=1)
+    info = {"omega": omega, "nu": float(nu), "r_inner": float(r_inner), "r_outer": float(r_outer), "length": float(length)}
+    return c, dom, [outer, inner], info
+
+
 def rotor_duct(nx=192, ny=96, nz=96, seed=0, levels=2, n_theta=2048, n_axial=48, tip_speed=0.08, u_inlet=0.02,
                nu=0.004):
     """Rotating rotor in a cylindrical duct on a coarse nx*ny*nz grid of 1 mm

@@ -13,6 +13,11 @@ nothing of the reference is run.
   L2 error of the inlet velocity profile; nonReflectiveTestMain.cu:208-233):
   the characteristic non-reflective inlet and the strict Dirichlet inlet.
   Rows 1-2000 are kept whole, then every 100th row up to 50000.
+* Centre-line pressure of the Poiseuille pipe case after 20000 iterations
+  (cases/poiseuilleFlow/results/INTERFACE/{COARSE,FINE}_centerlinePressure.txt,
+  "z [mm];pressure [Pa]" along the pipe axis from the x = 0 section cut,
+  poiseuilleFlowMain.cu:158-168, exportCenterlinePressure.py): uniform grids
+  of 0.4 mm and 0.2 mm cells.
 * The pipe mesh both pipe cases read (pipeD20L200STL.STL, binary STL,
   376 triangles).
 
@@ -24,6 +29,7 @@ import sys
 
 TGV = ("COARSE", "FINE", "QUADRATIC")
 INLET = ("characteristic", "dirichlet")
+POISEUILLE = ("COARSE", "FINE")
 
 
 def _rows(path):
@@ -52,6 +58,12 @@ def main(root):
         with open(os.path.join(here, "pipe_inlet_%s.txt" % name), "w") as f:
             f.write(header + "\n" + "\n".join(keep) + "\n")
         print(name, len(keep), "of", len(body), "rows")
+    for name in POISEUILLE:
+        header, body = _rows(os.path.join(root, "cases", "poiseuilleFlow", "results", "INTERFACE",
+                                          "%s_centerlinePressure.txt" % name))
+        with open(os.path.join(here, "poiseuille_%s_centerline_pressure.txt" % name.lower()), "w") as f:
+            f.write(header + "\n" + "\n".join(body) + "\n")
+        print(name, len(body), "rows")
     shutil.copyfile(os.path.join(root, "cases", "nonReflectiveTest", "pipeD20L200STL.STL"),
                     os.path.join(here, "pipeD20L200STL.STL"))
 

@@ -131,3 +131,47 @@ def pipe_inlet_profile_error(sim, case):
     e2 = np.sum((err[keep].astype(np.float64)) ** 2)
     t2 = np.sum((target[keep].astype(np.float64)) ** 2)
     return float(np.sqrt(e2 / t2))
+
+
+def wall_force_sums(sim, bodies):
+    """Per-body sums of the accumulated wall force (on the fluid) and its torque
+    about the origin, from the oracle's arrays: (bodies, 6) as lib.Sim.wall_forces
+    (lever arms: the wall-adjacent cell centres in mm)."""
+    info = sim.info()
+    idx = sim.array("wallIndex")
+    W = len(idx)
+    g = sim.array("wallForce").reshape(3, -1)[:, :W].astype(np.float64)
+    ijk = cell_ijk(sim)[idx]
+    f32 = np.float32
+    p = [(ijk[:, a].astype(f32) * f32(info.res) + f32(o)).astype(np.float64) for a, o in enumerate((info.ox, info.oy, info.oz))]
+    body = (sim.array("wallData").astype(np.int64) >> 27) & 31
+    t = [p[1] * g[2] - p[2] * g[1], p[2] * g[0] - p[0] * g[2], p[0] * g[1] - p[1] * g[0]]
+    out = np.zeros((bodies, 6))
+    for b in range(bodies):
+        sel = body == b
+        out[b] = [g[0][sel].sum(), g[1][sel].sum(), g[2][sel].sum(), t[0][sel].sum(), t[1][sel].sum(), t[2][sel].sum()]
+    return out
+
+
+def centreline_pressure(sim, case):
+    """The pipe cases' centre-line pressure on a one-level grid, as
+    exportSectionCutPlotZY at x = 0 (exportSectionCutPlot.h:11-330) and
+    exportCenterlinePressure.py sample it: the section plane i = (int)((0 - ox)
+    / res + 0.5), the two middle rows j = (ny - 1) / 2 and ny / 2 averaged, every
+    k, dRho converted to Pa by convertToPhysicalPressure (cellFunctions.h:288-294).
+    Returns (z [mm], p [Pa])."""
+    f32 = np.float32
+    info = sim.info(0)
+    ox, res = f32(info.ox), f32(info.res)
+    icut = int((f32(0.0) - ox) / res + f32(0.5))
+    ny, nz = info.cellCountY, info.cellCountZ
+    ijk = cell_ijk(sim, 0)
+    sel = np.where(ijk[:, 0] == icut)[0]
+    where = {(int(ijk[c, 1]), int(ijk[c, 2])): c for c in sel}
+    m = sim.macros(0)
+    cs = f32(0.57735026918962576) * (res / f32(1000.0)) / f32(info.dtPhys)
+    p = np.zeros(nz)
+    for k in range(nz):
+        v = [f32(f32(m[where[(j, k)], 0]) * f32(case.rhoPhys) * cs * cs) for j in ((ny - 1) // 2, ny // 2)]
+        p[k] = np.mean(np.array(v, np.float32))
+    return float(info.oz) + np.arange(nz) * float(res), p
