@@ -63,6 +63,16 @@ template <class T> static int upload(DBuf<T> &b, const T *host, long long n, hip
     return 0;
 }
 
+// the bulk kernel's skip masks and block rows, from the current cell records
+static int refresh_bulk_mask(Sim &s, Level &l)
+{
+    const long long blocks = (l.N + 255) / 256;
+    if (l.tileRow.n < blocks) HIPCHK(l.tileRow.alloc(blocks > 0 ? blocks : 1));
+    if (l.bulkSkip.n < 4 * blocks) HIPCHK(l.bulkSkip.alloc(blocks > 0 ? 4 * blocks : 4));
+    launch_bulk_mask(l.cellRow.p, l.N, l.bulkSkip.p, l.tileRow.p, s.stream);
+    return check_hip(hipGetLastError(), "bulk mask");
+}
+
 // derived per-level tables: local BC, open-BC records, force accumulators, populations
 int finish_level_setup(Sim &s, int level)
 {
@@ -81,6 +91,7 @@ int finish_level_setup(Sim &s, int level)
     HIPCHK(l.cellRow.alloc(l.N > 0 ? l.N : 1));
     HIPCHK(l.rowTab.alloc(l.nRows > 0 ? l.nRows : 1));
     launch_fast_topo(topo_of(l), l.wallMap.p, nullptr, nullptr, l.cellRow.p, l.rowTab.p, s.stream);
+    if (const int rc = refresh_bulk_mask(s, l)) return rc;
     l.rotors.clear();              // rotors belong to the grid they were built on
     l.rotorDev.release();
     l.rotorCellCount = 0;
@@ -220,6 +231,7 @@ static int prepare_open_fusion(Sim &s, int L)
         l.rotorCellCount = (int)cells.size();
         if (upload(l.rotorCells, cells.data(), (long long)cells.size(), s.stream)) return LBM3_E_HIP;
     }
+    if (const int rc = refresh_bulk_mask(s, l)) return rc;
     HIPCHK(hipStreamSynchronize(s.stream));
     l.fusedOpen = fuse;
     return 0;
@@ -487,16 +499,20 @@ static int validate_level(const lbm3_level_arrays &a)
     if (!a.shifter || !a.rowI || !a.rowJ || !a.rowK || !a.rowJPlus || !a.rowKPlus || !a.rowJKPlus || !a.wallMap)
         return bad("missing topology array");
     if (W > 0 && (!a.wallIndex || !a.wallData || !a.linkLength)) return bad("missing wall arrays");
+    long long nextRow = 0;      // rows are numbered in the order of their first cells (the bulk kernel relies on it)
     for (long long c = 0; c < N; ++c) {
         const int sh = a.shifter[c];
         if (sh >= 0) {
             if (sh >= R) return bad("shifter row out of range");
+            if (sh != nextRow) return bad("rows not numbered in the order of their first cells");
+            ++nextRow;
         } else if (c + sh < 0 || a.shifter[c + sh] < 0 || a.shifter[c + sh] >= R) {
             return bad("shifter offset does not reach a row head");
         }
         const int wm = a.wallMap[c];
         if (wm < -3 || wm >= W) return bad("wallMap out of range");
     }
+    if (nextRow != R) return bad("a row without a first cell");
     for (long long r = 0; r < R; ++r)
         if (a.rowJPlus[r] < 0 || a.rowJPlus[r] >= N || a.rowKPlus[r] < 0 || a.rowKPlus[r] >= N || a.rowJKPlus[r] < 0 ||
             a.rowJKPlus[r] >= N)

@@ -40,10 +40,16 @@ __device__ __forceinline__ void decode(const Topology &t, int cell, int (&nb)[8]
 // record per cell {wallMap, row} and one 16-byte record per row {first cell of
 // the row, first cell of the j+ row, of the k+ row, of the jk+ row}.  The
 // neighbour cells of a cell are then two dependent loads away instead of four.
+// The bulk kernel reads neither per-cell record: a bit per cell says whether
+// another kernel (or none) updates it, and the row of every 256th cell, with
+// the rows' first cells, gives the row of the others.
 struct FastTopo {
     int N;
     const int2 *cellRow;   // [N]
     const int4 *rows;      // [nRows]
+    const uint64_t *skip;  // [N / 64 rounded up to 4] bit c % 64 of word c / 64: not a bulk cell; null: read cellRow
+    const int *tileRow;    // [N / 256 rounded up] row of cell 256 t
+    int nRows;
 };
 
 __device__ __forceinline__ void decode_fast(const FastTopo &t, int cell, int row, int (&nb)[8])

@@ -61,6 +61,8 @@ struct Level {
     DBuf<int> openCell, openSource;
     DBuf<int4> openNb;               // [2][nOpen] neighbour cells of each open cell
     DBuf<uint32_t> openMeta;
+    DBuf<uint64_t> bulkSkip;         // per 64 cells: bit set where the bulk kernel leaves the cell to others
+    DBuf<int> tileRow;               // per 256 cells: row of the first cell
     DBuf<float> openScale;           // per open entry: velocity profile factor (null: no rule has a profile)
     DBuf<OpenBCRecord> openRecords;
     DBuf<unsigned char> openRefs;    // 2 arrays of nOpen compute-type values

@@ -271,9 +271,21 @@ template <class S, bool FLIP, int BULK, class Off>
 __device__ __forceinline__ void bulk_cell(const StepArgs<S> &a, int cell)
 {
     using T = typename StoreTraits<S>::compute;
-    const long long rec = __builtin_nontemporal_load(reinterpret_cast<const long long *>(a.fast.cellRow) + cell);
-    const int wm = int(rec), row = int(rec >> 32);
-    if (wm >= 0 || wm <= -3) return;   // handled by another kernel, or solid
+    int row;
+    if (a.fast.skip) {
+        // a wave's 64 cells are 64-aligned and a block's 256: one mask word per
+        // wave and one row per block, both scalar loads
+        const uint64_t m = a.fast.skip[__builtin_amdgcn_readfirstlane(cell >> 6)];
+        if ((m >> (cell & 63)) & 1ull) return;   // handled by another kernel, or solid
+        int r = a.fast.tileRow[__builtin_amdgcn_readfirstlane(cell >> 8)];
+        while (r + 1 < a.fast.nRows && a.fast.rows[r + 1].x <= cell) ++r;
+        row = r;
+    } else {
+        const long long rec = __builtin_nontemporal_load(reinterpret_cast<const long long *>(a.fast.cellRow) + cell);
+        const int wm = int(rec);
+        row = int(rec >> 32);
+        if (wm >= 0 || wm <= -3) return;   // handled by another kernel, or solid
+    }
     int nb[8];
     decode_fast(a.fast, cell, row, nb);
     // byte offset of each neighbour's slot run in its tile; the plane adds a
@@ -365,6 +377,30 @@ __global__ __launch_bounds__(256) void k_mark_groups(const int *groups, int coun
     if (cell >= N) return;
     const int code = cellRow[cell].x;
     if (code == -1 || code == -2) cellRow[cell].x = -5;
+}
+
+// the bulk kernel's view of the cell records: per 64 cells a mask of the cells
+// it skips (record >= 0 or <= -3, and the padding past N), per 256 cells the
+// row of the first one
+__global__ __launch_bounds__(256) void k_bulk_mask(const int2 *cellRow, int N, uint64_t *skip, int *tileRow)
+{
+    const int cell = blockIdx.x * 256 + threadIdx.x;
+    bool sk = true;
+    int row = 0;
+    if (cell < N) {
+        const int2 r = cellRow[cell];
+        sk = r.x >= 0 || r.x <= -3;
+        row = r.y;
+    }
+    const unsigned long long m = __ballot(sk);
+    if ((threadIdx.x & 63) == 0) skip[cell >> 6] = m;
+    if (threadIdx.x == 0) tileRow[blockIdx.x] = row;
+}
+
+void launch_bulk_mask(const int2 *cellRow, int N, uint64_t *skip, int *tileRow, hipStream_t s)
+{
+    if (N <= 0) return;
+    k_bulk_mask<<<(N + 255) / 256, 256, 0, s>>>(cellRow, N, skip, tileRow);
 }
 
 void launch_mark_groups(const int *groups, int count, int G, int N, int2 *cellRow, hipStream_t s)
