@@ -267,6 +267,13 @@ __device__ __forceinline__ void rotors_apply(const StepArgs<S> &a, int cell, T d
 // the level on the host (1: K17, 2: K15).  Topology record >= 0: wall-adjacent,
 // -3: solid, -4: a cell the rotors can reach, -5: a fluid cell in a group of
 // open cells, <= -8: open cell (each with its own kernel).
+// 16-bit storage in the bulk kernel: 0 global loads and stores, 1 raw buffer
+// loads and stores, 2 buffer loads and global stores.  Measured on the
+// 256x128x128 sphere (FP16 bulk kernel): 93.4 us, 97.6 us, 94.4 us.
+#ifndef LBM3_BUF16
+#define LBM3_BUF16 0
+#endif
+
 template <class S, bool FLIP, int BULK, class Off>
 __device__ __forceinline__ void bulk_cell(const StepArgs<S> &a, int cell)
 {
@@ -295,17 +302,22 @@ __device__ __forceinline__ void bulk_cell(const StepArgs<S> &a, int cell)
     for (int k = 0; k < 8; ++k)
         off[k] = (Off(uint32_t(nb[k]) / LBM3_TILE) * Off(27 * LBM3_TILE) + Off(uint32_t(nb[k]) % LBM3_TILE)) * Off(sizeof(S));
     T f[27];
-    if constexpr (sizeof(Off) == 4 && sizeof(S) >= 4) {
-        // raw buffer accesses: the plane offset is a scalar operand (the 16-bit
-        // stores measured 3% faster with global accesses)
+    if constexpr (sizeof(Off) == 4 && (sizeof(S) >= 4 || LBM3_BUF16 > 0)) {
+        // raw buffer accesses: the plane offset is a scalar operand
         const __amdgpu_buffer_rsrc_t r = pop_rsrc(a.f, 27LL * padded_cells(a.fast.N) * (long long)sizeof(S));
 #pragma unroll
         for (int q = 0; q < 27; ++q)
             f[q] = StoreTraits<S>::bload(r, off[pre_nb(q)], pre_plane(q, FLIP) * LBM3_TILE * int(sizeof(S)), q, a.cd);
         collide<T, BULK>(f, relax_of<T>(a.bulk));
+        if constexpr (sizeof(S) >= 4 || LBM3_BUF16 == 1) {
 #pragma unroll
-        for (int q = 0; q < 27; ++q)
-            StoreTraits<S>::bstore(r, off[post_nb(q)], post_plane(q, FLIP) * LBM3_TILE * int(sizeof(S)), f[q], q, a.cd);
+            for (int q = 0; q < 27; ++q)
+                StoreTraits<S>::bstore(r, off[post_nb(q)], post_plane(q, FLIP) * LBM3_TILE * int(sizeof(S)), f[q], q, a.cd);
+        } else {
+#pragma unroll
+            for (int q = 0; q < 27; ++q)
+                StoreTraits<S>::store(at(a.f + post_plane(q, FLIP) * LBM3_TILE, off[post_nb(q)]), f[q], q, a.cd);
+        }
     } else {
 #pragma unroll
         for (int q = 0; q < 27; ++q)
