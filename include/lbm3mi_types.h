@@ -157,7 +157,20 @@ typedef struct lbm3_case {
      * near that state keeps the full 16-bit precision for its deviation.  Zero gives
      * the plain shifted populations (f - w) / w.  Ignored by the FP32/FP64 stores. */
     float storeRef[3];
+    /* Non-reflective scheme of the open boundaries (no counterpart field in the
+     * reference, which swaps its updateGrid.h for another file):
+     * LBM3_OPEN_CHARACTERISTIC (updateGrid.h:267-307: the target corrected along
+     * the outgoing characteristic, reference state an exponential moving
+     * average) or LBM3_OPEN_IMPEDANCE (nonReflectiveTest/
+     * updateGrid_SCHLAFFER_IMPEDANCE_SCHEME.h:250-331: the target clamped to
+     * within rhoReflectionTolerance of the non-reflecting density, reference
+     * state the last completed one). */
+    int openScheme;
+    float rhoReflectionTolerance;   /* RHO_REFLECTION_TOLERANCE (nonReflectiveTestMain.cu:1); >= 1 disables */
 } lbm3_case;
+
+#define LBM3_OPEN_CHARACTERISTIC 0
+#define LBM3_OPEN_IMPEDANCE 1
 
 /* Storage precisions of the population array (shifted f - w). */
 #define LBM3_STORE_F32 0

@@ -175,7 +175,7 @@ def sphere(nx=256, ny=128, nz=128, diameter=32.0, center=(64.0, 64.0, 64.0), u_i
 
 
 def pipe(triangles, length=100.0, res=0.2, diameter=20.0, reynolds=1000.0, u_mean=0.03, inlet_non_reflective=True,
-         profile_init=False, levels=1, nu_phys=1.5e-5, rho_phys=1.225):
+         profile_init=False, levels=1, nu_phys=1.5e-5, rho_phys=1.225, impedance_tolerance=None):
     """Flow through the 20 mm bore of the reference's pipe STL
     (pipeD20L200STL.STL: a 25 mm square block with a 200 mm bore along z),
     cut to `length` mm about z = 0, as the reference's pipe cases set it up
@@ -187,13 +187,19 @@ def pipe(triangles, length=100.0, res=0.2, diameter=20.0, reynolds=1000.0, u_mea
     k = nz - 1 (open boundary 1), K15 everywhere, the bore a no-slip wall
     without interpolated bounceback, open-boundary flow tracked.  With
     levels = 2 the core r < D/4, |z| < length/4 is refined once
-    (poiseuilleFlowMain.cu:32-46).  Returns (case, domain, triangles)."""
+    (poiseuilleFlowMain.cu:32-46).  impedance_tolerance selects the impedance
+    non-reflective scheme with that density tolerance
+    (updateGrid_SCHLAFFER_IMPEDANCE_SCHEME.h, RHO_REFLECTION_TOLERANCE) in place
+    of the characteristic one.  Returns (case, domain, triangles)."""
     R = f32(0.5) * f32(diameter)
     u_phys = f32(reynolds) * f32(nu_phys) / (f32(diameter) / f32(1000.0))
     dt = (f32(u_mean) / u_phys) * (f32(res) / f32(1000.0))
     c = _case(levels, res, dt, nu_phys, rho_phys)
     c.trackWallForce = 0
     c.trackOpenBoundaries = 1
+    if impedance_tolerance is not None:
+        c.openScheme = T.OPEN_IMPEDANCE
+        c.rhoReflectionTolerance = float(f32(impedance_tolerance))
     U = float(f32(2.0) * f32(u_mean))
     add_open_rule(c, z=T.FACE_LOW, dirichletU=1, ux=0.0, uy=0.0, uz=U, nonReflective=int(bool(inlet_non_reflective)),
                   openBCID=0)

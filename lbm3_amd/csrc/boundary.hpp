@@ -138,7 +138,15 @@ struct OpenBCRecord {      // a resolved getOpenBC result
     float dRho, ux, uy, uz;
     float beta;
     int dirichletU, dirichletRho, nonReflective;
+    int impedance;         // the case's non-reflective scheme is LBM3_OPEN_IMPEDANCE
+    float tolerance;       // its rhoReflectionTolerance
 };
+
+// std::clamp(v, lo, hi)
+template <class T> __host__ __device__ __forceinline__ T clamp_to(T v, T lo, T hi)
+{
+    return v < lo ? lo : (hi < v ? hi : v);
+}
 
 // One open-boundary cell, normal table index N.  f holds the known
 // pre-collision populations (shifted); unknown entries are overwritten.
@@ -160,7 +168,39 @@ __host__ __device__ __forceinline__ void open_cell(T (&f)[27], const OpenBCRecor
     T dRho = T(R.dRho), ux = T(R.ux), uy = T(R.uy), uz = T(R.uz);
     const bool nonRefl = face && R.nonReflective;
     constexpr T cs = T(0.57735026918962576);
-    if (R.dirichletRho) {
+    constexpr int axis = nx ? 0 : (ny ? 1 : 2);
+    constexpr T sgn = T(nx + ny + nz);
+    if (R.impedance) {
+        // impedance scheme (updateGrid_SCHLAFFER_IMPEDANCE_SCHEME.h:250-311): the
+        // reference state is the previous completed one; the target is clamped
+        // to within the tolerance of the non-reflecting density
+        const T tol = T(R.tolerance);
+        const bool useNR = nonRefl && tol < T(1);
+        T dRhoNR = T(0);
+        if (useNR) {
+            const T cs2 = cs * cs;
+            const T bracket = uNormalRef + cs + T(1);
+            const T t = uNormalRef - dRhoZ;
+            const T D = fma(bracket, bracket, cs2);
+            const T B = fma(bracket, cs + t, -cs2 * dRhoRef);
+            const T H = fma(-t, T(2) * cs + t, T(2) * cs2 * dRhoRef);
+            const T disc = fma(D, H, B * B);
+            dRhoNR = H / (B + sqrt(disc));
+        }
+        if (R.dirichletRho) {
+            if (useNR) dRho = clamp_to(dRho, dRhoNR - tol, dRhoNR + tol);
+            restore_u<N>(f, dRho + T(1), ux, uy, uz);
+        } else if (R.dirichletU) {
+            if (useNR) {
+                const T uMin = (dRhoZ - dRhoNR - tol) / (dRhoNR + T(1) + tol);
+                const T uMax = (dRhoZ - dRhoNR + tol) / (dRhoNR + T(1) - tol);
+                T uvec[3] = {ux, uy, uz};
+                uvec[axis] = sgn > T(0) ? clamp_to(uvec[axis], uMin, uMax) : clamp_to(uvec[axis], -uMax, -uMin);
+                ux = uvec[0]; uy = uvec[1]; uz = uvec[2];
+            }
+            dRho = restore_rho<N>(f, ux, uy, uz) - T(1);
+        }
+    } else if (R.dirichletRho) {
         if (nonRefl) {
             const T rhoT = dRho + T(1);
             const T rt = (dRhoZ - dRho) / rhoT;
@@ -171,8 +211,6 @@ __host__ __device__ __forceinline__ void open_cell(T (&f)[27], const OpenBCRecor
         restore_u<N>(f, dRho + T(1), ux, uy, uz);
     } else if (R.dirichletU) {
         if (nonRefl) {
-            constexpr int axis = nx ? 0 : (ny ? 1 : 2);
-            constexpr T sgn = T(nx + ny + nz);
             T uvec[3] = {ux, uy, uz};
             const T target = sgn * uvec[axis];
             const T rt = (dRhoZ - dRhoRef) / (dRhoRef + T(1));

@@ -102,7 +102,11 @@ int finish_level_setup(Sim &s, int level)
     launch_open_nb(topo_of(l), l.nOpen, l.openCell.p, l.openNb.p, s.stream);
     HIPCHK(hipGetLastError());
     std::vector<OpenBCRecord> rec(s.C.nOpenRules + 1);
-    for (int r = 0; r <= s.C.nOpenRules; ++r) rec[r] = open_record(open_bc(s.C, r));
+    for (int r = 0; r <= s.C.nOpenRules; ++r) {
+        rec[r] = open_record(open_bc(s.C, r));
+        rec[r].impedance = s.C.openScheme == LBM3_OPEN_IMPEDANCE;
+        rec[r].tolerance = s.C.rhoReflectionTolerance;
+    }
     if (upload(l.openRecords, rec.data(), (long long)rec.size(), s.stream)) return LBM3_E_HIP;
     OpenProfiles prof{};
     bool profiled = false;

@@ -76,7 +76,7 @@ __host__ __device__ inline const lbm3_bc &open_bc(const lbm3_case &c, int rule)
 
 inline OpenBCRecord open_record(const lbm3_bc &b)
 {
-    OpenBCRecord r;
+    OpenBCRecord r{};
     r.dRho = b.dRho; r.ux = b.ux; r.uy = b.uy; r.uz = b.uz;
     r.beta = b.nonReflectiveBeta;
     r.dirichletU = b.dirichletU; r.dirichletRho = b.dirichletRho; r.nonReflective = b.nonReflective;

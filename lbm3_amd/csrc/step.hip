@@ -122,14 +122,20 @@ __device__ __forceinline__ void open_fill(T (&f)[27], int x, uint32_t meta, cons
 #pragma unroll
     for (int q = 0; q < 27; ++q)
         if (!unknown_dir(nx, ny, nz, q)) f[q] = known[q];
-    // exponential moving average of the reference state
-    const T beta = T(R.beta);
-    T alpha = beta / (T(iter) + beta + T(10));
-    int every = 1;
-    while (alpha < T(1.e-5f)) { alpha *= T(3); every *= 3; }
-    if (iter % every == 0) {
-        dRhoRef[x] = fma(alpha, dRho - dRef, dRef);
-        uNormalRef[x] = fma(alpha, un - uRef, uRef);
+    if (R.impedance) {
+        // the impedance scheme keeps the completed state for the next step
+        dRhoRef[x] = dRho;
+        uNormalRef[x] = un;
+    } else {
+        // exponential moving average of the reference state
+        const T beta = T(R.beta);
+        T alpha = beta / (T(iter) + beta + T(10));
+        int every = 1;
+        while (alpha < T(1.e-5f)) { alpha *= T(3); every *= 3; }
+        if (iter % every == 0) {
+            dRhoRef[x] = fma(alpha, dRho - dRef, dRef);
+            uNormalRef[x] = fma(alpha, un - uRef, uRef);
+        }
     }
     if (dRhoCum && ((meta >> 13) & 1u)) {
         dRhoCum[x] += double(dRho);

@@ -69,6 +69,7 @@ class RefineRule(C.Structure):
                [(n, C.c_float) for n in ("xMin", "yMin", "zMin", "xMax", "yMax", "zMax")] + [("cylinder", Profile)]
 
 
+OPEN_CHARACTERISTIC, OPEN_IMPEDANCE = 0, 1
 MAX_ROTORS = 8
 AXIS_NONE, AXIS_X, AXIS_Y, AXIS_Z = -1, 0, 1, 2
 
@@ -81,4 +82,4 @@ class Case(C.Structure):
                 ("nOpenRules", C.c_int), ("openRules", OpenRule * MAX_RULES), ("openDefault", BC),
                 ("nLocalRules", C.c_int), ("localRules", LocalRule * MAX_RULES),
                 ("nRefineRules", C.c_int), ("refineRules", RefineRule * MAX_RULES),
-                ("storeRef", C.c_float * 3)]
+                ("storeRef", C.c_float * 3), ("openScheme", C.c_int), ("rhoReflectionTolerance", C.c_float)]

@@ -213,7 +213,41 @@ static void NAME(open_cell)(orc_sim *S, orc_grid *g, orc_openbc *O, int x, int f
     const int code = (nx + 5) * 100 + (ny + 5) * 10 + (nz + 5);
     while (LBM3_BT_NORMAL_CODE[n] != code) ++n;
     const REAL cs = (REAL)ORC_INVSQRT3;
-    if (BC.dirichletRho) {
+    if (S->C.openScheme == LBM3_OPEN_IMPEDANCE) {
+        /* the impedance scheme, cases/nonReflectiveTest/
+           updateGrid_SCHLAFFER_IMPEDANCE_SCHEME.h:250-311: dRhoRef and uNormalRef
+           hold the previous completed state */
+        const REAL tol = (REAL)S->C.rhoReflectionTolerance;
+        const int useNR = BC.nonReflective && tol < (REAL)1;
+        REAL dRhoNR = 0;
+        if (useNR) {
+            const REAL cs2 = cs * cs;
+            const REAL bracket = uNormalRef + cs + (REAL)1;
+            const REAL t = uNormalRef - dRhoZ;
+            const REAL D = ORC_FMA(bracket, bracket, cs2);
+            const REAL B = ORC_FMA(bracket, cs + t, -cs2 * dRhoRef);
+            const REAL H = ORC_FMA(-t, (REAL)2 * cs + t, (REAL)2 * cs2 * dRhoRef);
+            const REAL disc = ORC_FMA(D, H, B * B);
+            dRhoNR = H / (B + ORC_SQRT(disc));
+        }
+        if (BC.dirichletRho) {
+            if (useNR) {
+                const REAL lo = dRhoNR - tol, hi = dRhoNR + tol;
+                BC.dRho = BC.dRho < lo ? lo : (hi < BC.dRho ? hi : BC.dRho);
+            }
+            NAME(orc_restore_u)(n, &BC, f);
+        } else if (BC.dirichletU) {
+            if (useNR) {
+                const REAL uMin = (dRhoZ - dRhoNR - tol) / (dRhoNR + (REAL)1 + tol);
+                const REAL uMax = (dRhoZ - dRhoNR + tol) / (dRhoNR + (REAL)1 - tol);
+                REAL *u = nx ? &BC.ux : (ny ? &BC.uy : &BC.uz);
+                const int pos = nx + ny + nz > 0;
+                const REAL lo = pos ? uMin : -uMax, hi = pos ? uMax : -uMin;
+                *u = *u < lo ? lo : (hi < *u ? hi : *u);
+            }
+            NAME(orc_restore_rho)(n, &BC, f);
+        }
+    } else if (BC.dirichletRho) {
         if (BC.nonReflective) {
             const REAL dRhoTarget = BC.dRho;
             const REAL rhoTarget = dRhoTarget + (REAL)1;
@@ -257,13 +291,19 @@ static void NAME(open_cell)(orc_sim *S, orc_grid *g, orc_openbc *O, int x, int f
         F_AT(g, slotIx[q], cellIx[q]) = f[q];
     }
     const REAL uNormal = (REAL)nx * BC.ux + (REAL)ny * BC.uy + (REAL)nz * BC.uz;
-    const REAL beta = BC.nonReflectiveBeta;
-    REAL alpha = beta / ((REAL)I->iterationsFinished + beta + (REAL)10);
-    int every = 1;
-    while (alpha < (REAL)1.e-5f) { alpha *= (REAL)3; every *= 3; }
-    if (I->iterationsFinished % every == 0) {
-        ((REAL *)O->dRhoRef)[x] = ORC_FMA(alpha, BC.dRho - dRhoRef, dRhoRef);
-        ((REAL *)O->uNormalRef)[x] = ORC_FMA(alpha, uNormal - uNormalRef, uNormalRef);
+    if (S->C.openScheme == LBM3_OPEN_IMPEDANCE) {
+        /* the completed state for the next step (:326-331) */
+        ((REAL *)O->dRhoRef)[x] = BC.dRho;
+        ((REAL *)O->uNormalRef)[x] = uNormal;
+    } else {
+        const REAL beta = BC.nonReflectiveBeta;
+        REAL alpha = beta / ((REAL)I->iterationsFinished + beta + (REAL)10);
+        int every = 1;
+        while (alpha < (REAL)1.e-5f) { alpha *= (REAL)3; every *= 3; }
+        if (I->iterationsFinished % every == 0) {
+            ((REAL *)O->dRhoRef)[x] = ORC_FMA(alpha, BC.dRho - dRhoRef, dRhoRef);
+            ((REAL *)O->uNormalRef)[x] = ORC_FMA(alpha, uNormal - uNormalRef, uNormalRef);
+        }
     }
     if (!S->C.trackOpenBoundaries || !trackFlow) return;
     ((REAL *)O->dRhoCum)[x] += BC.dRho;

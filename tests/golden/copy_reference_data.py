@@ -8,10 +8,12 @@ nothing of the reference is run.
   COARSE 128^3 uniform, FINE 256^3 uniform, QUADRATIC 128^3 with the central
   cube refined once; Re = 1600, Ma = 0.1, 0 <= t* <= 20.
 * Inlet histories of the non-reflective pipe test (cases/nonReflectiveTest/
-  {characteristic,dirichlet}.txt, one row per iteration: iteration, area-
-  averaged inlet velocity [m/s], area-averaged inlet pressure [Pa], relative
-  L2 error of the inlet velocity profile; nonReflectiveTestMain.cu:208-233):
-  the characteristic non-reflective inlet and the strict Dirichlet inlet.
+  {characteristic,dirichlet,impedance2e-5,impedance2e-6,impedance2e-7}.txt,
+  one row per iteration: iteration, area-averaged inlet velocity [m/s],
+  area-averaged inlet pressure [Pa], relative L2 error of the inlet velocity
+  profile; nonReflectiveTestMain.cu:208-233): the characteristic
+  non-reflective inlet, the strict Dirichlet inlet and the impedance scheme
+  (updateGrid_SCHLAFFER_IMPEDANCE_SCHEME.h) at three density tolerances.
   Rows 1-2000 are kept whole, then every 100th row up to 50000.
 * Centre-line pressure of the Poiseuille pipe case after 20000 iterations
   (cases/poiseuilleFlow/results/INTERFACE/{COARSE,FINE}_centerlinePressure.txt,
@@ -28,7 +30,7 @@ import shutil
 import sys
 
 TGV = ("COARSE", "FINE", "QUADRATIC")
-INLET = ("characteristic", "dirichlet")
+INLET = ("characteristic", "dirichlet", "impedance2e-5", "impedance2e-6", "impedance2e-7")
 POISEUILLE = ("COARSE", "FINE")
 
 

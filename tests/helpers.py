@@ -101,10 +101,13 @@ def pipe_inlet_history(name):
 
 def pipe_reference_case(name):
     """The reference's non-reflective pipe test (nonReflectiveTestMain.cu) with the
-    characteristic non-reflective inlet or the strict Dirichlet inlet."""
+    characteristic non-reflective inlet ("characteristic"), the strict Dirichlet
+    inlet ("dirichlet") or the impedance scheme at a density tolerance
+    ("impedance2e-6")."""
     from lbm3_amd import cases, geometry
     tri = geometry.read_binary_stl(os.path.join(GOLDEN, "pipeD20L200STL.STL"))
-    return cases.pipe(tri, inlet_non_reflective=(name == "characteristic"))
+    tol = float(name[len("impedance"):]) if name.startswith("impedance") else None
+    return cases.pipe(tri, inlet_non_reflective=(name != "dirichlet"), impedance_tolerance=tol)
 
 
 def pipe_inlet_profile_error(sim, case):

@@ -59,3 +59,27 @@ def test_inlet_area_is_the_counted_bore_cells():
     counted = st["area"] / (0.2e-3 ** 2)
     assert abs(counted - round(counted)) < 1e-3
     assert abs(st["area"] / (np.pi * 1e-4) - 1.0) < 0.01
+
+
+# impedance scheme (updateGrid_SCHLAFFER_IMPEDANCE_SCHEME.h): the inlet velocity
+# is clamped to a window of the density tolerance around the non-reflecting
+# state, so FP32 rounding (the reference contracts into FMAs, the oracle does
+# not) moves the small early values by up to 1e-5 of the window's largest
+# value at tolerance 2e-5 and 5e-5 at 2e-6 (measured 1.7e-6 and 1.3e-5)
+@pytest.mark.parametrize("name,bar", [("impedance2e-5", 1e-5), ("impedance2e-6", 5e-5)])
+def test_oracle_impedance_inlet_history_matches_the_reference(name, bar):
+    case, dom, tris = pipe_reference_case(name)
+    o = OracleSim(case, dom, tris, double=False)
+    o.init()
+    ref = pipe_inlet_history(name)[:20]
+    got = []
+    for it in range(1, 21):
+        o.run(1)
+        st = o.open_boundary_stats(0)
+        o.reset_open_boundary_stats()
+        got.append((st["normal_velocity"], st["pressure"]))
+    got = np.array(got)
+    assert abs(got[0, 0] - ref[0, 1]) <= 1e-6 * abs(ref[0, 1])          # the first step exactly
+    for k, col in ((0, 1), (1, 2)):
+        dev = np.max(np.abs(got[:, k] - ref[:, col])) / np.max(np.abs(ref[:, col]))
+        assert dev < bar, (name, k, dev)
