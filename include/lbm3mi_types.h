@@ -167,7 +167,16 @@ typedef struct lbm3_case {
      * state the last completed one). */
     int openScheme;
     float rhoReflectionTolerance;   /* RHO_REFLECTION_TOLERANCE (nonReflectiveTestMain.cu:1); >= 1 disables */
+    /* Coarse <-> fine interface transfer (no counterpart field in the reference,
+     * which swaps its updateInterface.h for another file): LBM3_INTERFACE_QUADRATIC
+     * (updateInterface.h, Geier's quadratic velocity interpolation) or
+     * LBM3_INTERFACE_LINEAR (INTERFACE_VERSIONS_BACKUP/updateInterfaceLINEAR.h:
+     * affine fits, volume-averaged density and momentum). */
+    int interfaceScheme;
 } lbm3_case;
+
+#define LBM3_INTERFACE_QUADRATIC 0
+#define LBM3_INTERFACE_LINEAR 1
 
 #define LBM3_OPEN_CHARACTERISTIC 0
 #define LBM3_OPEN_IMPEDANCE 1

@@ -1,5 +1,7 @@
 // Multilevel coarse <-> fine interface transfer for gfx950 (reference
-// include/updateInterface.h:155-890), run after every pair of fine steps.
+// include/updateInterface.h:155-890, or with LINEAR the affine variant of
+// include/INTERFACE_VERSIONS_BACKUP/updateInterfaceLINEAR.h), run after every
+// pair of fine steps.
 //
 //  k_fine_to_coarse  one coarse cell per group of 8 lanes: lane n loads fine
 //                    stencil corner n, the group shares the corner values in
@@ -56,7 +58,7 @@ template <class T> struct IfArgs {
 
 // F->C: the coarse cell at the centre of a 2x2x2 fine stencil
 // (updateInterface.h:155-360)
-template <class S, bool FLIPC, bool FLIPF>
+template <class S, bool FLIPC, bool FLIPF, bool LINEAR>
 __global__ __launch_bounds__(256) void k_fine_to_coarse(IfArgs<typename StoreTraits<S>::compute> a)
 {
     using T = typename StoreTraits<S>::compute;
@@ -77,17 +79,21 @@ __global__ __launch_bounds__(256) void k_fine_to_coarse(IfArgs<typename StoreTra
     CellValues<T> cv[8];
 #pragma unroll
     for (int k = 0; k < 8; ++k) cv[k] = lds[base + k];
-    const Interpolant<T> P = interpolant(cv);
-    T d, u[3], g[3][3];
-    evaluate(P, T(0), T(0), T(0), d, u, g);
-    // density: the plain mean of the eight fine cells (updateInterface.h:235-237)
-    T mean = 0;
-#pragma unroll
-    for (int k = 0; k < 8; ++k) mean += cv[k].d;
-    d = mean * T(0.125);
     T f[27];
-    reconstruct(f, d, u, g[0][1] + g[1][0], g[1][2] + g[2][1], g[0][2] + g[2][0], g[0][0] - g[1][1], g[0][0] - g[2][2],
-                T(2) / a.w1c);
+    if constexpr (LINEAR) {
+        linear_fine_to_coarse(f, cv, a.w1c);
+    } else {
+        const Interpolant<T> P = interpolant(cv);
+        T d, u[3], g[3][3];
+        evaluate(P, T(0), T(0), T(0), d, u, g);
+        // density: the plain mean of the eight fine cells (updateInterface.h:235-237)
+        T mean = 0;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) mean += cv[k].d;
+        d = mean * T(0.125);
+        reconstruct(f, d, u, g[0][1] + g[1][0], g[1][2] + g[2][1], g[0][2] + g[2][0], g[0][0] - g[1][1], g[0][0] - g[2][2],
+                    T(2) / a.w1c);
+    }
     const int cell = a.index[item];
     int nb[8];
     decode_fast(a.coarse, cell, a.coarse.cellRow[cell].y, nb);
@@ -100,7 +106,7 @@ __global__ __launch_bounds__(256) void k_fine_to_coarse(IfArgs<typename StoreTra
 
 // C->F: the eight fine cells inside a 2x2x2 coarse block, at +-1/4
 // (updateInterface.h:362-687)
-template <class S, bool FLIPC, bool FLIPF>
+template <class S, bool FLIPC, bool FLIPF, bool LINEAR>
 __global__ __launch_bounds__(256) void k_coarse_to_fine(IfArgs<typename StoreTraits<S>::compute> a)
 {
     using T = typename StoreTraits<S>::compute;
@@ -121,13 +127,17 @@ __global__ __launch_bounds__(256) void k_coarse_to_fine(IfArgs<typename StoreTra
     CellValues<T> cv[8];
 #pragma unroll
     for (int k = 0; k < 8; ++k) cv[k] = lds[base + k];
-    const Interpolant<T> P = interpolant(cv);
     const T x = (n & 1) ? T(0.25) : T(-0.25), y = (n & 2) ? T(0.25) : T(-0.25), z = (n & 4) ? T(0.25) : T(-0.25);
-    T d, u[3], g[3][3];
-    evaluate(P, x, y, z, d, u, g);
     T f[27];
-    reconstruct(f, d, u, g[0][1] + g[1][0], g[1][2] + g[2][1], g[0][2] + g[2][0], g[0][0] - g[1][1], g[0][0] - g[2][2],
-                T(0.5) / a.w1f);
+    if constexpr (LINEAR) {
+        linear_coarse_to_fine(f, cv, a.w1f, x, y, z);
+    } else {
+        const Interpolant<T> P = interpolant(cv);
+        T d, u[3], g[3][3];
+        evaluate(P, x, y, z, d, u, g);
+        reconstruct(f, d, u, g[0][1] + g[1][0], g[1][2] + g[2][1], g[0][2] + g[2][0], g[0][0] - g[1][1], g[0][0] - g[2][2],
+                    T(0.5) / a.w1f);
+    }
     const int cell = stencil_corner(a.fine, a.child[item], n);
     int nb[8];
     decode_fast(a.fine, cell, a.fine.cellRow[cell].y, nb);
@@ -205,16 +215,18 @@ __global__ __launch_bounds__(256) void k_leftovers(IfArgs<typename StoreTraits<S
 inline unsigned groups_of_8(long long items) { return (unsigned)((8 * items + 255) / 256); }
 
 template <class S, bool FC, bool FF> void launch_all(const IfArgs<typename StoreTraits<S>::compute> &base, const Level &c,
-                                                      hipStream_t s)
+                                                      bool linear, hipStream_t s)
 {
     IfArgs<typename StoreTraits<S>::compute> a = base;
     if (c.F2C.interfaceCount > 0) {
         a.count = c.F2C.interfaceCount; a.index = c.F2C.index.p; a.child = c.F2C.childMap.p;
-        hipLaunchKernelGGL((k_fine_to_coarse<S, FC, FF>), dim3(groups_of_8(a.count)), dim3(256), 0, s, a);
+        if (linear) hipLaunchKernelGGL((k_fine_to_coarse<S, FC, FF, true>), dim3(groups_of_8(a.count)), dim3(256), 0, s, a);
+        else hipLaunchKernelGGL((k_fine_to_coarse<S, FC, FF, false>), dim3(groups_of_8(a.count)), dim3(256), 0, s, a);
     }
     if (c.C2F.interfaceCount > 0) {
         a.count = c.C2F.interfaceCount; a.index = c.C2F.index.p; a.child = c.C2F.childMap.p;
-        hipLaunchKernelGGL((k_coarse_to_fine<S, FC, FF>), dim3(groups_of_8(a.count)), dim3(256), 0, s, a);
+        if (linear) hipLaunchKernelGGL((k_coarse_to_fine<S, FC, FF, true>), dim3(groups_of_8(a.count)), dim3(256), 0, s, a);
+        else hipLaunchKernelGGL((k_coarse_to_fine<S, FC, FF, false>), dim3(groups_of_8(a.count)), dim3(256), 0, s, a);
     }
     if (c.C2F.leftoverCount > 0) {
         a.count = c.C2F.leftoverCount;
@@ -243,8 +255,9 @@ template <class S> void interface_typed(Sim &s, int coarse)
     a.topoC = Topology{c.N, c.shifter.p, c.ri.p, c.rj.p, c.rk.p, c.rjp.p, c.rkp.p, c.rjkp.p};
     a.cd = s.codec;
     const bool fc = c.info.esotwistFlipper != 0, ff = f.info.esotwistFlipper != 0;
-    if (fc) { if (ff) launch_all<S, true, true>(a, c, s.stream); else launch_all<S, true, false>(a, c, s.stream); }
-    else { if (ff) launch_all<S, false, true>(a, c, s.stream); else launch_all<S, false, false>(a, c, s.stream); }
+    const bool lin = s.C.interfaceScheme == LBM3_INTERFACE_LINEAR;
+    if (fc) { if (ff) launch_all<S, true, true>(a, c, lin, s.stream); else launch_all<S, true, false>(a, c, lin, s.stream); }
+    else { if (ff) launch_all<S, false, true>(a, c, lin, s.stream); else launch_all<S, false, false>(a, c, lin, s.stream); }
 }
 
 } // namespace

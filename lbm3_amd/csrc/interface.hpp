@@ -150,15 +150,10 @@ __host__ __device__ __forceinline__ void evaluate(const Interpolant<T> &P, T x, 
 // sigma / omega1 (updateInterface.h:334-344, :645-661); third and higher
 // cumulants at equilibrium (updateInterface.h:29-153)
 template <class T>
-__host__ __device__ __forceinline__ void reconstruct(T (&f)[27], T dRho, const T (&u)[3], T sxy, T syz, T sxz, T nxy, T nxz,
-                                                     T sigmaOverOmega)
+__host__ __device__ __forceinline__ void reconstruct_k(T (&f)[27], T dRho, const T (&u)[3], T k011, T k101, T k110, T mxy,
+                                                       T mxz)
 {
-    const T rho = T(1) + dRho;
     const T third = T(1) / T(3);
-    const T shear = -third * sigmaOverOmega * rho;
-    const T k011 = shear * syz, k101 = shear * sxz, k110 = shear * sxy;
-    const T normal = -T(2) / T(3) * sigmaOverOmega * rho;
-    const T mxy = normal * nxy, mxz = normal * nxz;
     const T k200 = third * (mxy + mxz + dRho);
     const T k020 = third * (-T(2) * mxy + mxz + dRho);
     const T k002 = third * (mxy - T(2) * mxz + dRho);
@@ -167,6 +162,92 @@ __host__ __device__ __forceinline__ void reconstruct(T (&f)[27], T dRho, const T
                               T(0), k110, T(0), k011, T(0), T(0), T(0), T(0), T(0),
                               k020, T(0), k4, T(0), T(0), T(0), k4, T(0), k6};
     from_central_moments(f, M, u[0], u[1], u[2]);
+}
+
+template <class T>
+__host__ __device__ __forceinline__ void reconstruct(T (&f)[27], T dRho, const T (&u)[3], T sxy, T syz, T sxz, T nxy, T nxz,
+                                                     T sigmaOverOmega)
+{
+    const T rho = T(1) + dRho;
+    const T third = T(1) / T(3);
+    const T shear = -third * sigmaOverOmega * rho;
+    const T normal = -T(2) / T(3) * sigmaOverOmega * rho;
+    reconstruct_k(f, dRho, u, shear * syz, shear * sxz, shear * sxy, normal * nxy, normal * nxz);
+}
+
+// ---- the linear interface (INTERFACE_VERSIONS_BACKUP/updateInterfaceLINEAR.h)
+
+// affine fit c0 + cx x + cy y + cz z of the eight corner values (:155-180)
+template <class T> struct Affine {
+    T c0, cx, cy, cz;
+    __host__ __device__ T at(T x, T y, T z) const { return c0 + cx * x + cy * y + cz * z; }
+};
+
+template <class T> __host__ __device__ __forceinline__ Affine<T> affine_fit(const T (&v)[8])
+{
+    Affine<T> p{T(0), T(0), T(0), T(0)};
+#pragma unroll
+    for (int n = 0; n < 8; ++n) {
+        p.c0 += T(0.125) * v[n];
+        p.cx += ((n & 1) ? T(0.25) : T(-0.25)) * v[n];
+        p.cy += ((n & 2) ? T(0.25) : T(-0.25)) * v[n];
+        p.cz += ((n & 4) ? T(0.25) : T(-0.25)) * v[n];
+    }
+    return p;
+}
+
+// F->C: density and momentum averaged over the eight fine cells, the shear
+// moments from their density-weighted strain (:265-299); sigma = 2
+template <class T> __host__ __device__ __forceinline__ void linear_fine_to_coarse(T (&f)[27], const CellValues<T> (&cv)[8],
+                                                                                 T w1c)
+{
+    T dRho = 0, j[3] = {0, 0, 0}, qxy = 0, qyz = 0, qxz = 0, qxy2 = 0, qxz2 = 0;
+#pragma unroll
+    for (int n = 0; n < 8; ++n) {
+        const T mw = T(0.125) * (T(1) + cv[n].d);
+        dRho += T(0.125) * cv[n].d;
+        j[0] += mw * cv[n].u[0]; j[1] += mw * cv[n].u[1]; j[2] += mw * cv[n].u[2];
+        qxy += mw * cv[n].sxy; qyz += mw * cv[n].syz; qxz += mw * cv[n].sxz;
+        qxy2 += mw * cv[n].nxy; qxz2 += mw * cv[n].nxz;
+    }
+    const T rho = T(1) + dRho;
+    const T u[3] = {j[0] / rho, j[1] / rho, j[2] / rho};
+    const T ss = T(2) / w1c;
+    const T third = -(T(1) / T(3)) * ss, twoThirds = -(T(2) / T(3)) * ss;
+    reconstruct_k(f, dRho, u, third * qyz, third * qxz, third * qxy, twoThirds * qxy2, twoThirds * qxz2);
+}
+
+// C->F at (x, y, z) = +-1/4: affine density, velocity and density-weighted
+// strain, the velocity intercepts corrected so the eight targets keep the
+// block's momentum (:405-482); sigma = 1/2
+template <class T> __host__ __device__ __forceinline__ void linear_coarse_to_fine(T (&f)[27], const CellValues<T> (&cv)[8],
+                                                                                 T w1f, T x, T y, T z)
+{
+    T D[8], U[3][8], QXY[8], QYZ[8], QXZ[8], QXY2[8], QXZ2[8];
+    T jm[3] = {0, 0, 0};
+#pragma unroll
+    for (int n = 0; n < 8; ++n) {
+        const T rhoS = T(1) + cv[n].d;
+        const T mw = T(0.125) * rhoS;
+        D[n] = cv[n].d;
+        U[0][n] = cv[n].u[0]; U[1][n] = cv[n].u[1]; U[2][n] = cv[n].u[2];
+        jm[0] += mw * cv[n].u[0]; jm[1] += mw * cv[n].u[1]; jm[2] += mw * cv[n].u[2];
+        QXY[n] = rhoS * cv[n].sxy; QYZ[n] = rhoS * cv[n].syz; QXZ[n] = rhoS * cv[n].sxz;
+        QXY2[n] = rhoS * cv[n].nxy; QXZ2[n] = rhoS * cv[n].nxz;
+    }
+    const Affine<T> d = affine_fit(D);
+    Affine<T> a[3] = {affine_fit(U[0]), affine_fit(U[1]), affine_fit(U[2])};
+    const T rhoMean = T(1) + d.c0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+        a[c].c0 = (jm[c] - (d.cx * a[c].cx + d.cy * a[c].cy + d.cz * a[c].cz) / T(16)) / rhoMean;
+    const T dRho = d.at(x, y, z);
+    const T u[3] = {a[0].at(x, y, z), a[1].at(x, y, z), a[2].at(x, y, z)};
+    const T ss = T(0.5) / w1f;
+    const T third = -(T(1) / T(3)) * ss, twoThirds = -(T(2) / T(3)) * ss;
+    reconstruct_k(f, dRho, u, third * affine_fit(QYZ).at(x, y, z), third * affine_fit(QXZ).at(x, y, z),
+                  third * affine_fit(QXY).at(x, y, z), twoThirds * affine_fit(QXY2).at(x, y, z),
+                  twoThirds * affine_fit(QXZ2).at(x, y, z));
 }
 
 } // namespace lbm3

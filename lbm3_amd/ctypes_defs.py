@@ -70,6 +70,7 @@ class RefineRule(C.Structure):
 
 
 OPEN_CHARACTERISTIC, OPEN_IMPEDANCE = 0, 1
+INTERFACE_QUADRATIC, INTERFACE_LINEAR = 0, 1
 MAX_ROTORS = 8
 AXIS_NONE, AXIS_X, AXIS_Y, AXIS_Z = -1, 0, 1, 2
 
@@ -82,4 +83,5 @@ class Case(C.Structure):
                 ("nOpenRules", C.c_int), ("openRules", OpenRule * MAX_RULES), ("openDefault", BC),
                 ("nLocalRules", C.c_int), ("localRules", LocalRule * MAX_RULES),
                 ("nRefineRules", C.c_int), ("refineRules", RefineRule * MAX_RULES),
-                ("storeRef", C.c_float * 3), ("openScheme", C.c_int), ("rhoReflectionTolerance", C.c_float)]
+                ("storeRef", C.c_float * 3), ("openScheme", C.c_int), ("rhoReflectionTolerance", C.c_float),
+                ("interfaceScheme", C.c_int)]

@@ -326,6 +326,116 @@ static void NAME(if_coarse_to_fine)(orc_grid *gc, orc_grid *gf)
     }
 }
 
+/* ---- the linear interface, include/INTERFACE_VERSIONS_BACKUP/updateInterfaceLINEAR.h */
+
+/* fitInterfaceLinear (:155-180): affine fit {c0, cx, cy, cz} of the eight corner values */
+static void NAME(if_affine)(const REAL *v, REAL *p)
+{
+    p[0] = p[1] = p[2] = p[3] = 0;
+    for (int i = 0; i < 8; ++i) {
+        p[0] += (REAL)0.125 * v[i];
+        p[1] += ((i & 1) ? (REAL)0.25 : -(REAL)0.25) * v[i];
+        p[2] += ((i & 2) ? (REAL)0.25 : -(REAL)0.25) * v[i];
+        p[3] += ((i & 4) ? (REAL)0.25 : -(REAL)0.25) * v[i];
+    }
+}
+
+static inline REAL NAME(if_affine_at)(const REAL *p, REAL x, REAL y, REAL z) { return p[0] + p[1] * x + p[2] * y + p[3] * z; }
+
+static void NAME(if_reconstruct_m)(REAL *f, REAL dRho, REAL ux, REAL uy, REAL uz, REAL k011, REAL k101, REAL k110,
+                                   REAL mxxMyy, REAL mxxMzz)
+{
+    const REAL k200 = RC(1, 3) * (mxxMyy + mxxMzz + dRho);
+    const REAL k020 = RC(1, 3) * (-(REAL)2 * mxxMyy + mxxMzz + dRho);
+    const REAL k002 = RC(1, 3) * (mxxMyy - (REAL)2 * mxxMzz + dRho);
+    NAME(if_reconstruct)(f, dRho, ux, uy, uz, k011, k101, k110, k200, k020, k002);
+}
+
+/* fine to coarse, :265-299 (the rest as updateInterface.h) */
+static void NAME(if_fine_to_coarse_linear)(orc_grid *gc, orc_grid *gf)
+{
+    const REAL w1c = NAME(omega1)(gc), w1f = NAME(omega1)(gf);
+    const orc_interface *IF = &gc->F2C;
+#pragma omp parallel for schedule(static)
+    for (int x = 0; x < IF->interfaceCount; ++x) {
+        int st[8];
+        NAME(if_stencil)(gf, IF->childMap[x], st);
+        NAME(ifv) s[8];
+        for (int n = 0; n < 8; ++n) NAME(if_cell)(gf, st[n], w1f, &s[n]);
+        REAL dRho = 0, jx = 0, jy = 0, jz = 0, qxy = 0, qyz = 0, qxz = 0, qxxMyy = 0, qxxMzz = 0;
+        for (int i = 0; i < 8; ++i) {
+            const REAL rhoSource = (REAL)1 + s[i].d;
+            const REAL massWeight = (REAL)0.125 * rhoSource;
+            dRho += (REAL)0.125 * s[i].d;
+            jx += massWeight * s[i].x;
+            jy += massWeight * s[i].y;
+            jz += massWeight * s[i].z;
+            qxy += massWeight * s[i].xy;
+            qyz += massWeight * s[i].yz;
+            qxz += massWeight * s[i].xz;
+            qxxMyy += massWeight * s[i].my;
+            qxxMzz += massWeight * s[i].mz;
+        }
+        const REAL rho = (REAL)1 + dRho;
+        const REAL stressScale = (REAL)2 / w1c;
+        REAL f[27];
+        NAME(if_reconstruct_m)(f, dRho, jx / rho, jy / rho, jz / rho, -RC(1, 3) * stressScale * qyz,
+                               -RC(1, 3) * stressScale * qxz, -RC(1, 3) * stressScale * qxy,
+                               -RC(2, 3) * stressScale * qxxMyy, -RC(2, 3) * stressScale * qxxMzz);
+        NAME(if_write)(gc, IF->index[x], f);
+    }
+}
+
+/* coarse to fine, full blocks, :405-482 */
+static void NAME(if_coarse_to_fine_linear)(orc_grid *gc, orc_grid *gf)
+{
+    const REAL w1c = NAME(omega1)(gc), w1f = NAME(omega1)(gf);
+    const orc_interface *IF = &gc->C2F;
+#pragma omp parallel for schedule(static)
+    for (int x = 0; x < IF->interfaceCount; ++x) {
+        int st[8], tg[8];
+        NAME(if_stencil)(gc, IF->index[x], st);
+        NAME(ifv) s[8];
+        for (int n = 0; n < 8; ++n) NAME(if_cell)(gc, st[n], w1c, &s[n]);
+        REAL D[8], UX[8], UY[8], UZ[8], QXY[8], QYZ[8], QXZ[8], QMY[8], QMZ[8];
+        REAL jx = 0, jy = 0, jz = 0;
+        for (int i = 0; i < 8; ++i) {
+            const REAL rhoSource = (REAL)1 + s[i].d;
+            const REAL massWeight = (REAL)0.125 * rhoSource;
+            D[i] = s[i].d; UX[i] = s[i].x; UY[i] = s[i].y; UZ[i] = s[i].z;
+            jx += massWeight * s[i].x;
+            jy += massWeight * s[i].y;
+            jz += massWeight * s[i].z;
+            QXY[i] = rhoSource * s[i].xy; QYZ[i] = rhoSource * s[i].yz; QXZ[i] = rhoSource * s[i].xz;
+            QMY[i] = rhoSource * s[i].my; QMZ[i] = rhoSource * s[i].mz;
+        }
+        REAL d[4], a[4], b[4], c[4], qxy[4], qyz[4], qxz[4], qmy[4], qmz[4];
+        NAME(if_affine)(D, d); NAME(if_affine)(UX, a); NAME(if_affine)(UY, b); NAME(if_affine)(UZ, c);
+        NAME(if_affine)(QXY, qxy); NAME(if_affine)(QYZ, qyz); NAME(if_affine)(QXZ, qxz);
+        NAME(if_affine)(QMY, qmy); NAME(if_affine)(QMZ, qmz);
+        const REAL rhoMean = (REAL)1 + d[0];
+        a[0] = (jx - (d[1] * a[1] + d[2] * a[2] + d[3] * a[3]) / (REAL)16) / rhoMean;
+        b[0] = (jy - (d[1] * b[1] + d[2] * b[2] + d[3] * b[3]) / (REAL)16) / rhoMean;
+        c[0] = (jz - (d[1] * c[1] + d[2] * c[2] + d[3] * c[3]) / (REAL)16) / rhoMean;
+        NAME(if_stencil)(gf, IF->childMap[x], tg);
+        const REAL stressScale = (REAL)0.5 / w1f;
+        for (int n = 0; n < 8; ++n) {
+            const REAL dx = (n & 1) ? (REAL)0.25 : -(REAL)0.25;
+            const REAL dy = (n & 2) ? (REAL)0.25 : -(REAL)0.25;
+            const REAL dz = (n & 4) ? (REAL)0.25 : -(REAL)0.25;
+            REAL f[27];
+            NAME(if_reconstruct_m)(f, NAME(if_affine_at)(d, dx, dy, dz), NAME(if_affine_at)(a, dx, dy, dz),
+                                   NAME(if_affine_at)(b, dx, dy, dz), NAME(if_affine_at)(c, dx, dy, dz),
+                                   -RC(1, 3) * stressScale * NAME(if_affine_at)(qyz, dx, dy, dz),
+                                   -RC(1, 3) * stressScale * NAME(if_affine_at)(qxz, dx, dy, dz),
+                                   -RC(1, 3) * stressScale * NAME(if_affine_at)(qxy, dx, dy, dz),
+                                   -RC(2, 3) * stressScale * NAME(if_affine_at)(qmy, dx, dy, dz),
+                                   -RC(2, 3) * stressScale * NAME(if_affine_at)(qmz, dx, dy, dz));
+            NAME(if_write)(gf, tg[n], f);
+        }
+    }
+}
+
 /* leftover fine cells without a full coarse block: centre + three signed face
    neighbours, updateInterface.h:689-884 */
 static void NAME(if_leftovers)(orc_sim *S, orc_grid *gc, orc_grid *gf)
@@ -399,8 +509,13 @@ static void NAME(if_leftovers)(orc_sim *S, orc_grid *gc, orc_grid *gf)
 void NAME(orc_update_interface)(orc_sim *S, int coarse)
 {
     orc_grid *gc = &S->g[coarse], *gf = &S->g[coarse + 1];
-    NAME(if_fine_to_coarse)(gc, gf);
-    NAME(if_coarse_to_fine)(gc, gf);
+    if (S->C.interfaceScheme == LBM3_INTERFACE_LINEAR) {
+        NAME(if_fine_to_coarse_linear)(gc, gf);
+        NAME(if_coarse_to_fine_linear)(gc, gf);
+    } else {
+        NAME(if_fine_to_coarse)(gc, gf);
+        NAME(if_coarse_to_fine)(gc, gf);
+    }
     NAME(if_leftovers)(S, gc, gf);
 }
 

@@ -6,7 +6,9 @@ nothing of the reference is run.
   *_kineticEnergy.txt, one "time star;normalized kinetic energy" row per
   sample, written by getKineticEnergy, taylorGreenVortexMain.cu:116-180):
   COARSE 128^3 uniform, FINE 256^3 uniform, QUADRATIC 128^3 with the central
-  cube refined once; Re = 1600, Ma = 0.1, 0 <= t* <= 20.
+  cube refined once, LINEAR the same with the linear interface
+  (INTERFACE_VERSIONS_BACKUP/updateInterfaceLINEAR.h); Re = 1600, Ma = 0.1,
+  0 <= t* <= 20.
 * Inlet histories of the non-reflective pipe test (cases/nonReflectiveTest/
   {characteristic,dirichlet,impedance2e-5,impedance2e-6,impedance2e-7}.txt,
   one row per iteration: iteration, area-averaged inlet velocity [m/s],
@@ -29,7 +31,7 @@ import os
 import shutil
 import sys
 
-TGV = ("COARSE", "FINE", "QUADRATIC")
+TGV = ("COARSE", "FINE", "QUADRATIC", "LINEAR")
 INLET = ("characteristic", "dirichlet", "impedance2e-5", "impedance2e-6", "impedance2e-7")
 POISEUILLE = ("COARSE", "FINE")
 

@@ -43,7 +43,7 @@ GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 # fixture name -> (active coarse cells per axis, grid levels, iterations between samples)
 # (taylorGreenVortexMain.cu:8-13,39-40)
-TGV_CURVES = {"coarse": (128, 1, 10), "quadratic": (128, 2, 10), "fine": (256, 1, 20)}
+TGV_CURVES = {"coarse": (128, 1, 10), "quadratic": (128, 2, 10), "fine": (256, 1, 20), "linear": (128, 2, 10)}
 
 
 def tgv_curve(name):
@@ -62,6 +62,8 @@ def tgv_reference_case(name):
     n, levels, period = TGV_CURVES[name]
     u0 = float(f32(0.1) * f32(0.5773502691896258))
     case, dom, tris = cases.taylor_green(n, u0=u0, reynolds=1600.0, pressure=True, levels=levels)
+    if name == "linear":
+        case.interfaceScheme = 1          # INTERFACE_VERSIONS_BACKUP/updateInterfaceLINEAR.h
     dt = f32(case.dtPhysGlobal)
     iterations = int(f32(20.0) / dt + f32(0.5))        # ITERATION_COUNT, taylorGreenVortexMain.cu:34-35
     samples = list(range(0, iterations + 1, period))

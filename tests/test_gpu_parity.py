@@ -208,8 +208,19 @@ def _small_sphere3():
                                    center=(20.0, 16.0, 16.0), diameter_fine=12.0, segments=(16, 32))
 
 
+def _linear(make):
+    """The same case with the linear interface (updateInterfaceLINEAR.h)."""
+    def mk():
+        case, dom, tris = make()
+        case.interfaceScheme = 1
+        return case, dom, tris
+    return mk
+
+
 MULTILEVEL = {"tgv2": lambda: cases.taylor_green(16, u0=0.05, reynolds=400.0, levels=2),
               "sphere3": _small_sphere3, "cut_sphere": _cut_sphere}
+MULTILEVEL["tgv2_linear"] = _linear(MULTILEVEL["tgv2"])
+MULTILEVEL["cut_sphere_linear"] = _linear(_cut_sphere)
 
 
 @pytest.mark.parametrize("name", sorted(MULTILEVEL))

@@ -106,8 +106,10 @@ def test_coefficients_reproduce_their_interpolant_family():
             assert np.max(np.abs(co[8 + 11 * c:19 + 11 * c] - a[c])) < 1e-12
 
 
-@pytest.mark.parametrize("make", ["tgv2", "sphere3"])
-def test_uniform_flow_is_kept_across_levels(make):
+@pytest.mark.parametrize("make,scheme", [("tgv2", 0), ("sphere3", 0), ("tgv2", 1), ("sphere3", 1)])
+def test_uniform_flow_is_kept_across_levels(make, scheme):
+    """A uniform flow stays exact on every level with the quadratic (scheme 0)
+    and the linear interface (scheme 1, updateInterfaceLINEAR.h)."""
     if make == "tgv2":
         case, dom, tris = cases.taylor_green(16, u0=0.05, reynolds=100.0, levels=2)
         case.init.kind = 0
@@ -119,6 +121,7 @@ def test_uniform_flow_is_kept_across_levels(make):
                                                   diameter_fine=8.0, segments=(12, 24))
         tris = []                 # the sphere would break uniformity; keep the grid and boundaries
         u = (case.init.ux, 0.0, 0.0)
+    case.interfaceScheme = scheme
     o = OracleSim(case, dom, tris, double=True)
     o.init()
     o.run(6)
@@ -143,3 +146,4 @@ def test_two_level_creeping_taylor_green_decay_rate():
     k = 2 * np.pi / n
     rate = -np.log(e1 / e0) / 200
     assert abs(rate / (6 * nu * k * k) - 1.0) < 0.02
+
