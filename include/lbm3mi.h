@@ -163,6 +163,17 @@ int lbm3_reset_wall_forces(lbm3_sim *s);   /* wall and rotor force accumulators 
  * normal kinetic power [W]}.  Normal velocities are along the outer normal.
  * FP64 throughout (the reference reduces in FP32). */
 int lbm3_open_boundary_stats(lbm3_sim *s, int openBCID, double *out7);
+/* Section cut through every level (exportSectionCutPlot{XY,ZY,ZX} without
+ * bounds or rotor frame, include/plotter/exportSectionCutPlot.h:11-330, 395-417):
+ * the plane at `cut` [mm] along its normal (z for LBM3_PLANE_XY, x for
+ * LBM3_PLANE_ZY, y for LBM3_PLANE_ZX), sampled at the finest level's cells
+ * that fit 16 M pixels, coarser levels painted over finer pixel blocks and
+ * blended between their two nearest planes.  dims = {vertical, horizontal
+ * pixel counts}; with out non-null, out[(v * H + h) * 6 + k] holds the six
+ * floats the reference writes to /dev/shm/sim_data.bin per pixel: pressure
+ * [Pa], velocity along the horizontal, vertical and normal axes [m/s], solid
+ * marker, cell size [mm].  Rotor fractions are not added to the marker. */
+int lbm3_section_cut(lbm3_sim *s, int plane, float cut, float *out, long long capacity, int *dims);
 /* zero the open-boundary accumulators and every level's iterationsSinceTrackerReset
  * (the reset at the end of updateTracker, trackerFunctions.h:476-486) */
 int lbm3_reset_open_boundary_stats(lbm3_sim *s);

@@ -175,6 +175,11 @@ typedef struct lbm3_case {
     int interfaceScheme;
 } lbm3_case;
 
+/* section-cut planes (PlaneEnum of the reference's plotter) */
+#define LBM3_PLANE_XY 0
+#define LBM3_PLANE_ZY 1
+#define LBM3_PLANE_ZX 2
+
 #define LBM3_INTERFACE_QUADRATIC 0
 #define LBM3_INTERFACE_LINEAR 1
 

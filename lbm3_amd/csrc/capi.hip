@@ -437,6 +437,14 @@ template <class T> int collide_cells(T *f, int n, const lbm3_bc *bc, T nu)
 
 using namespace lbm3;
 
+// one level's cells on a section plane into the image
+template <class S> static void cut_level(Sim &s, int L, const CutLevel &d, float *img)
+{
+    Level &l = s.L[L];
+    launch_section_cut<S>(reinterpret_cast<const S *>(l.f.p), topo_of(l), l.wallMap.p, s.codec, l.info.esotwistFlipper != 0,
+                          d, img, s.stream);
+}
+
 extern "C" {
 
 const char *lbm3_last_error(void) { return g_error.c_str(); }
@@ -969,6 +977,77 @@ int lbm3_reset_open_boundary_stats(lbm3_sim *h)
         if (l.openCum.p) HIPCHK(hipMemsetAsync(l.openCum.p, 0, sizeof(double) * (size_t)l.openCum.n, s.stream));
         l.info.iterationsSinceTrackerReset = 0;
     }
+    return 0;
+}
+
+int lbm3_section_cut(lbm3_sim *h, int plane, float cut, float *out, long long capacity, int *dims)
+{
+    if (!h || !dims) return set_error(LBM3_E_ARG, "null argument");
+    if (plane < LBM3_PLANE_XY || plane > LBM3_PLANE_ZX) return set_error(LBM3_E_ARG, "unknown section plane");
+    Sim &s = h->s;
+    for (int L = 0; L < s.levels; ++L)
+        if (const int rc = check_level(s, L)) return rc;
+    const int finest = s.levels - 1;
+    const lbm3_info &F = s.L[finest].info;
+    auto origin = [plane](const lbm3_info &I) { return plane == LBM3_PLANE_XY ? I.oz : (plane == LBM3_PLANE_ZY ? I.ox : I.oy); };
+    auto normalCount = [plane](const lbm3_info &I) {
+        return plane == LBM3_PLANE_XY ? I.cellCountZ : (plane == LBM3_PLANE_ZY ? I.cellCountX : I.cellCountY);
+    };
+    // the cut plane on the finest level (getIJKCellIndexFromXYZ, cellFunctions.h:96-101)
+    const int cutIndex = (int)((cut - origin(F)) / F.res + 0.5f);
+    if (cutIndex < 0 || cutIndex >= normalCount(F)) return set_error(LBM3_E_ARG, "section cut outside the finest grid");
+    const double cutPos = double(origin(F)) + double(cutIndex) * double(F.res);
+    // the finest level whose image fits the pixel limit (exportSectionCutPlot.h:57-76)
+    int H = plane == LBM3_PLANE_XY ? F.cellCountX : F.cellCountZ;
+    int V = plane == LBM3_PLANE_ZX ? F.cellCountX : F.cellCountY;
+    int imageLevel = finest;
+    while ((long long)H * V > 16000000LL && imageLevel > 0) {
+        --imageLevel;
+        H /= 2;
+        V /= 2;
+    }
+    if (H <= 0 || V <= 0 || (long long)H * V > 16000000LL) return set_error(LBM3_E_ARG, "section image empty or too large");
+    dims[0] = V;
+    dims[1] = H;
+    if (!out) return 0;
+    const long long pixels = (long long)H * V;
+    if (capacity < 6 * pixels) return set_error(LBM3_E_ARG, "buffer too small for the section image");
+    DBuf<float> img;
+    HIPCHK(img.alloc(6 * pixels));
+    launch_cut_clear(img.p, pixels, s.stream);
+    CutUnits u{};
+    u.plane = plane;
+    u.rhoPhys = s.C.rhoPhys;
+    for (int L = 0; L < s.levels; ++L) {
+        if (const int rc = settle_open(s, L)) return rc;
+        const lbm3_info &I = s.L[L].info;
+        u.res[L] = I.res;
+        u.dt[L] = I.dtPhys;
+        if (I.cellCount <= 0 || normalCount(I) <= 0 || I.res <= 0.f) continue;
+        double ni = (cutPos - double(origin(I))) / double(I.res);
+        ni = std::min(std::max(ni, 0.0), double(normalCount(I) - 1));
+        CutLevel d{};
+        d.plane = plane;
+        d.lower = (int)std::floor(ni);
+        d.alpha = (float)(ni - d.lower);
+        d.interpolate = d.alpha > 0.f;
+        d.down = L > imageLevel ? 1 << (L - imageLevel) : 1;
+        d.up = L < imageLevel ? 1 << (imageLevel - L) : 1;
+        d.pixH = H;
+        d.pixV = V;
+        d.gridID = L;
+        switch (s.storage) {
+        case LBM3_STORE_F16: cut_level<__half>(s, L, d, img.p); break;
+        case LBM3_STORE_BF16: cut_level<__hip_bfloat16>(s, L, d, img.p); break;
+        case LBM3_STORE_F64: cut_level<double>(s, L, d, img.p); break;
+        default: cut_level<float>(s, L, d, img.p); break;
+        }
+        HIPCHK(hipGetLastError());
+    }
+    launch_cut_finish(img.p, pixels, u, s.stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out, img.p, sizeof(float) * 6 * (size_t)pixels, hipMemcpyDeviceToHost, s.stream));
+    HIPCHK(hipStreamSynchronize(s.stream));
     return 0;
 }
 

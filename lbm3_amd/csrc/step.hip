@@ -1143,7 +1143,121 @@ void launch_multi_step(const MultiDesc<S> *D, const int *starts, int n, const in
     if (blocks[3] > 0) hipLaunchKernelGGL((k_multi_periodic<S>), dim3(blocks[3]), dim3(256), 0, s, D, starts + 3 * (n + 1), n);
 }
 
+// ---- section cut (exportSectionCutPlotGeneral, include/plotter/exportSectionCutPlot.h:11-330)
+
+// image of the section: per pixel {dRho, ux, uy, uz, marker, gridID} before
+// conversion; marker 1 (solid) and zeros where no level writes
+__global__ __launch_bounds__(256) void k_cut_clear(float *img, long long pixels)
+{
+    const long long p = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (p >= pixels) return;
+    float *v = img + 6 * p;
+    v[0] = v[1] = v[2] = v[3] = 0.f;
+    v[4] = 1.f;
+    v[5] = 0.f;
+}
+
+// the cells of one level on the section (:101-287): normal index `lower`,
+// blended with the next plane by alpha where both are fluid; every
+// down-th cell, each painting up x up pixels
+template <class S, bool FLIP>
+__global__ __launch_bounds__(256) void k_section_cut(const S *f, Topology topo, const int *wallMap, Codec cd, CutLevel d,
+                                                     float *img)
+{
+    using T = typename StoreTraits<S>::compute;
+    const int cell = blockIdx.x * 256 + threadIdx.x;
+    if (cell >= topo.N) return;
+    int nb[8], ijk[3];
+    decode(topo, cell, nb, ijk);
+    const int normal = d.plane == 0 ? ijk[2] : (d.plane == 1 ? ijk[0] : ijk[1]);
+    if (normal != d.lower) return;
+    const int hc = d.plane == 0 ? ijk[0] : ijk[2];
+    const int vc = d.plane == 2 ? ijk[0] : ijk[1];
+    if (hc % d.down != 0 || vc % d.down != 0) return;
+    const int ih = (hc / d.down) * d.up, iv = (vc / d.down) * d.up;
+    if (ih + d.up <= 0 || ih >= d.pixH || iv + d.up <= 0 || iv >= d.pixV) return;
+    int upper = cell;
+    int nbu[8];
+    for (int k = 0; k < 8; ++k) nbu[k] = nb[k];
+    if (d.interpolate) {
+        upper = d.plane == 0 ? nb[3] : (d.plane == 1 ? nb[1] : nb[2]);
+        if (upper < 0 || upper >= topo.N) return;
+        int ijku[3];
+        decode(topo, upper, nbu, ijku);
+        // only an adjacent centre counts (sparse rows may skip or wrap)
+        if (ijku[0] != ijk[0] + (d.plane == 1) || ijku[1] != ijk[1] + (d.plane == 2) || ijku[2] != ijk[2] + (d.plane == 0))
+            return;
+    }
+    const bool lowerSolid = wallMap[cell] == -3, upperSolid = wallMap[upper] == -3;
+    const bool selectedSolid = d.alpha <= 0.5f ? lowerSolid : upperSolid;
+    float val[4] = {0.f, 0.f, 0.f, 0.f};
+    float marker = selectedSolid ? 1.f : 0.f;
+    if (!selectedSolid) {
+        const bool blend = d.interpolate && !lowerSolid && !upperSolid;
+        T v[27], dd, u0, u1, u2;
+        load_pre<S, FLIP>(f, topo.N, lowerSolid ? nbu : nb, v, cd);
+        moments(v, dd, u0, u1, u2);
+        val[0] = float(dd); val[1] = float(u0); val[2] = float(u1); val[3] = float(u2);
+        if (blend) {
+            load_pre<S, FLIP>(f, topo.N, nbu, v, cd);
+            moments(v, dd, u0, u1, u2);
+            const float up4[4] = {float(dd), float(u0), float(u1), float(u2)};
+            for (int k = 0; k < 4; ++k) val[k] += d.alpha * (up4[k] - val[k]);
+        }
+        marker = marker < 0.f ? 0.f : (marker > 1.f ? 1.f : marker);
+    }
+    for (int sv = 0; sv < d.up; ++sv) {
+        const int y = iv + sv;
+        if (y < 0 || y >= d.pixV) continue;
+        for (int sh = 0; sh < d.up; ++sh) {
+            const int x = ih + sh;
+            if (x < 0 || x >= d.pixH) continue;
+            float *p = img + 6 * ((long long)y * d.pixH + x);
+            p[0] = val[0]; p[1] = val[1]; p[2] = val[2]; p[3] = val[3]; p[4] = marker; p[5] = float(d.gridID);
+        }
+    }
+}
+
+// physical units and the plane's velocity order, per pixel (:296-329):
+// {p [Pa], u horizontal, u vertical, u normal [m/s], marker, res [mm]}
+__global__ __launch_bounds__(256) void k_cut_finish(float *img, long long pixels, CutUnits u)
+{
+    const long long p = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (p >= pixels) return;
+    float *v = img + 6 * p;
+    const int g = int(v[5]);
+    const float res = u.res[g], dt = u.dt[g];
+    const float ss = 0.57735026918962576f * (res / 1000.f) / dt;
+    const float pr = v[0] * u.rhoPhys * ss * ss;
+    const float ux = v[1] * (res / 1000.f) / dt, uy = v[2] * (res / 1000.f) / dt, uz = v[3] * (res / 1000.f) / dt;
+    const float h = u.plane == 0 ? ux : uz;
+    const float vv = u.plane == 0 ? uy : (u.plane == 1 ? uy : ux);
+    const float n = u.plane == 0 ? uz : (u.plane == 1 ? ux : uy);
+    v[0] = pr; v[1] = h; v[2] = vv; v[3] = n; v[5] = res;
+}
+
+template <class S> void launch_section_cut(const S *f, const Topology &t, const int *wallMap, const Codec &cd, bool flip,
+                                           const CutLevel &d, float *img, hipStream_t s)
+{
+    if (t.N <= 0) return;
+    const unsigned g = (unsigned)((t.N + 255) / 256);
+    if (flip) k_section_cut<S, true><<<g, 256, 0, s>>>(f, t, wallMap, cd, d, img);
+    else k_section_cut<S, false><<<g, 256, 0, s>>>(f, t, wallMap, cd, d, img);
+}
+
+void launch_cut_clear(float *img, long long pixels, hipStream_t s)
+{
+    if (pixels > 0) k_cut_clear<<<(unsigned)((pixels + 255) / 256), 256, 0, s>>>(img, pixels);
+}
+
+void launch_cut_finish(float *img, long long pixels, const CutUnits &u, hipStream_t s)
+{
+    if (pixels > 0) k_cut_finish<<<(unsigned)((pixels + 255) / 256), 256, 0, s>>>(img, pixels, u);
+}
+
 #define LBM3_INSTANTIATE(S)                                                                                          \
+    template void launch_section_cut<S>(const S *, const Topology &, const int *, const Codec &, bool, const CutLevel &, \
+                                        float *, hipStream_t);                                                       \
     template void launch_collide_stream<S>(const StepArgs<S> &, bool, hipStream_t);                                 \
     template void launch_collide_wall<S>(const StepArgs<S> &, bool, hipStream_t);                                   \
     template void launch_open_cells<S>(const StepArgs<S> &, bool, hipStream_t);                                     \

@@ -15,6 +15,26 @@ struct InitDesc {
     lbm3_profile profile;     // kind 0: velocity scaled by the profile factor of the cell
 };
 
+// one level's part of a section cut (exportSectionCutPlotGeneral)
+struct CutLevel {
+    int plane;                // 0 XY (normal z), 1 ZY (normal x), 2 ZX (normal y)
+    int lower;                // normal index of the level's plane
+    float alpha;              // weight of the next plane
+    int interpolate;          // alpha > 0
+    int down, up;             // level cells per pixel / pixels per level cell, along each in-plane axis
+    int pixH, pixV;           // image size
+    int gridID;
+};
+struct CutUnits {
+    int plane;
+    float rhoPhys;
+    float res[LBM3_MAX_LEVELS], dt[LBM3_MAX_LEVELS];
+};
+template <class S> void launch_section_cut(const S *f, const Topology &t, const int *wallMap, const Codec &cd, bool flip,
+                                           const CutLevel &d, float *img, hipStream_t s);
+void launch_cut_clear(float *img, long long pixels, hipStream_t s);
+void launch_cut_finish(float *img, long long pixels, const CutUnits &u, hipStream_t s);
+
 // the radial profile of every open rule, and none for the default record
 struct OpenProfiles {
     lbm3_profile p[LBM3_MAX_RULES + 1];

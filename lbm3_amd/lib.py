@@ -55,6 +55,7 @@ def load():
         "lbm3_reset_wall_forces": (i32, [vp]),
         "lbm3_open_boundary_stats": (i32, [vp, i32, C.POINTER(f64)]),
         "lbm3_reset_open_boundary_stats": (i32, [vp]),
+        "lbm3_section_cut": (i32, [vp, i32, C.c_float, vp, i64, C.POINTER(i32)]),
         "lbm3_build_rotors": (i32, [vp, vp, vp, i32, i32]),
         "lbm3_set_rotor": (i32, [vp, i32, i32, C.c_float, i32, C.c_float, C.c_float, C.c_float]),
         "lbm3_rotor_info": (i32, [vp, i32, i32, C.POINTER(f64)]),
@@ -262,6 +263,19 @@ class Sim:
 
     def reset_open_boundary_stats(self):
         check(load().lbm3_reset_open_boundary_stats(self.h))
+
+    PLANES = {"xy": 0, "zy": 1, "zx": 2}
+
+    def section_cut(self, plane, cut):
+        """exportSectionCutPlot's image of the plane ("xy", "zy", "zx") at `cut`
+        mm: (vertical, horizontal, 6) float32 of {p [Pa], u horizontal, u vertical,
+        u normal [m/s], marker, res [mm]}."""
+        pl = self.PLANES[plane]
+        dims = (C.c_int * 2)()
+        check(load().lbm3_section_cut(self.h, pl, float(cut), None, 0, dims))
+        out = np.zeros((dims[0], dims[1], 6), np.float32)
+        check(load().lbm3_section_cut(self.h, pl, float(cut), out.ctypes.data, out.size, dims))
+        return out
 
     def build_rotors(self, rotors, level_mask=0xFF):
         """buildRotors on the levels of level_mask; rotors: list of (n, 9) triangle arrays."""

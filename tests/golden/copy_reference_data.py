@@ -19,9 +19,10 @@ nothing of the reference is run.
   Rows 1-2000 are kept whole, then every 100th row up to 50000.
 * Centre-line pressure of the Poiseuille pipe case after 20000 iterations
   (cases/poiseuilleFlow/results/INTERFACE/{COARSE,FINE}_centerlinePressure.txt,
-  "z [mm];pressure [Pa]" along the pipe axis from the x = 0 section cut,
+  "z [mm];pressure [Pa]" along the pipe axis from the x = 0 section cut
   poiseuilleFlowMain.cu:158-168, exportCenterlinePressure.py): uniform grids
-  of 0.4 mm and 0.2 mm cells.
+  of 0.4 mm and 0.2 mm cells, and 0.4 mm cells with the core refined once
+  (the quadratic interface).
 * The pipe mesh both pipe cases read (pipeD20L200STL.STL, binary STL,
   376 triangles).
 
@@ -33,7 +34,7 @@ import sys
 
 TGV = ("COARSE", "FINE", "QUADRATIC", "LINEAR")
 INLET = ("characteristic", "dirichlet", "impedance2e-5", "impedance2e-6", "impedance2e-7")
-POISEUILLE = ("COARSE", "FINE")
+POISEUILLE = ("COARSE", "FINE", "QUADRATIC")
 
 
 def _rows(path):

@@ -180,3 +180,77 @@ def centreline_pressure(sim, case):
         v = [f32(f32(m[where[(j, k)], 0]) * f32(case.rhoPhys) * cs * cs) for j in ((ny - 1) // 2, ny // 2)]
         p[k] = np.mean(np.array(v, np.float32))
     return float(info.oz) + np.arange(nz) * float(res), p
+
+
+def section_cut_reference(sim, case, plane, cut):
+    """exportSectionCutPlotGeneral (include/plotter/exportSectionCutPlot.h:11-330)
+    restated on an oracle's macroscopic fields, all levels: returns the
+    (V, H, 6) image {p, u horizontal, u vertical, u normal, marker, res} of
+    the plane ("xy", "zy", "zx") at `cut` mm, without rotor fractions."""
+    f32 = np.float32
+    P = {"xy": 0, "zy": 1, "zx": 2}[plane]
+    levels = sim.levels
+    infos = [sim.info(L) for L in range(levels)]
+    F = infos[-1]
+    org = lambda I: f32((I.oz, I.ox, I.oy)[P])
+    cnt = lambda I: (I.cellCountZ, I.cellCountX, I.cellCountY)[P]
+    cut_index = int((f32(cut) - org(F)) / f32(F.res) + f32(0.5))
+    assert 0 <= cut_index < cnt(F)
+    cut_pos = float(org(F)) + float(cut_index) * float(f32(F.res))
+    H = F.cellCountX if P == 0 else F.cellCountZ
+    V = F.cellCountX if P == 2 else F.cellCountY
+    image_level = levels - 1
+    while H * V > 16000000 and image_level > 0:
+        image_level -= 1
+        H //= 2
+        V //= 2
+    img = np.zeros((V, H, 6), np.float32)
+    img[..., 4] = 1.0
+    for L, I in enumerate(infos):
+        ni = (cut_pos - float(org(I))) / float(f32(I.res))
+        ni = min(max(ni, 0.0), cnt(I) - 1)
+        lower = int(np.floor(ni))
+        alpha = f32(ni - lower)
+        down = 2 ** (L - image_level) if L > image_level else 1
+        up = 2 ** (image_level - L) if L < image_level else 1
+        ijk = cell_ijk(sim, L)
+        m = sim.macros(L).astype(np.float32)
+        wm = sim.array("wallMap", L)
+        where = {tuple(c): k for k, c in enumerate(ijk.tolist())}
+        unit = [(0, 0, 1), (1, 0, 0), (0, 1, 0)][P]
+        for c in np.where(ijk[:, (2, 0, 1)[P]] == lower)[0]:
+            i, j, k = ijk[c]
+            hc, vc = (i if P == 0 else k), (i if P == 2 else j)
+            if hc % down or vc % down:
+                continue
+            ih, iv = (hc // down) * up, (vc // down) * up
+            if ih + up <= 0 or ih >= H or iv + up <= 0 or iv >= V:
+                continue
+            cu = c
+            if alpha > 0:
+                cu = where.get((i + unit[0], j + unit[1], k + unit[2]))
+                if cu is None:
+                    continue
+            lower_solid, upper_solid = wm[c] == -3, wm[cu] == -3
+            solid = lower_solid if alpha <= 0.5 else upper_solid
+            val = np.zeros(4, np.float32)
+            if not solid:
+                val = m[cu if lower_solid else c].copy()
+                if alpha > 0 and not lower_solid and not upper_solid:
+                    val = val + alpha * (m[cu] - val)
+            img[iv:iv + up, ih:ih + up, :4] = val
+            img[iv:iv + up, ih:ih + up, 4] = 1.0 if solid else 0.0
+            img[iv:iv + up, ih:ih + up, 5] = L
+    out = img.copy()
+    for L, I in enumerate(infos):
+        sel = img[..., 5] == L
+        res, dt = f32(I.res), f32(I.dtPhys)
+        ss = f32(0.57735026918962576) * (res / f32(1000.0)) / dt
+        vel = lambda v: v * (res / f32(1000.0)) / dt
+        ux, uy, uz = vel(img[..., 1]), vel(img[..., 2]), vel(img[..., 3])
+        comp = [(ux, uy, uz), (uz, uy, ux), (uz, ux, uy)][P]
+        out[..., 0] = np.where(sel, img[..., 0] * f32(case.rhoPhys) * ss * ss, out[..., 0])
+        for a in range(3):
+            out[..., 1 + a] = np.where(sel, comp[a], out[..., 1 + a])
+        out[..., 5] = np.where(sel, res, out[..., 5])
+    return out
