@@ -92,6 +92,9 @@ int finish_level_setup(Sim &s, int level)
     HIPCHK(l.rowTab.alloc(l.nRows > 0 ? l.nRows : 1));
     launch_fast_topo(topo_of(l), l.wallMap.p, nullptr, nullptr, l.cellRow.p, l.rowTab.p, s.stream);
     if (const int rc = refresh_bulk_mask(s, l)) return rc;
+    // interface neighbour tables that read this level are derived again
+    l.F2C.nbReady = l.C2F.nbReady = false;
+    if (level > 0) s.L[level - 1].F2C.nbReady = s.L[level - 1].C2F.nbReady = false;
     l.rotors.clear();              // rotors belong to the grid they were built on
     l.rotorDev.release();
     l.rotorCellCount = 0;
@@ -361,7 +364,7 @@ static void step_level_any(Sim &s, int level)
     }
 }
 
-void update_interface(Sim &s, int coarse);   // interface.hip
+void update_interface(Sim &s, int coarse);    // interface.hip
 
 static void update_all(Sim &s, int level)
 {
@@ -407,6 +410,8 @@ static int check_level(Sim &s, int level)
 // every level a sub-cycled update touches is built, and every interface it runs is checked
 static int prepare_open_fusion(Sim &s, int L);
 
+int prepare_interface(Sim &s, int coarse);    // interface.hip: neighbour tables of the transfer lists
+
 static int check_run(Sim &s, int from)
 {
     for (int L = from; L < s.levels; ++L) {
@@ -417,6 +422,8 @@ static int check_run(Sim &s, int from)
     for (int L = from; L < s.levels; ++L)
         if (!s.L[L].openFusionChecked)
             if (const int rc = prepare_open_fusion(s, L)) return rc;
+    for (int L = from; L + 1 < s.levels; ++L)
+        if (const int rc = prepare_interface(s, L)) return rc;
     return 0;
 }
 

@@ -41,8 +41,54 @@ __device__ __forceinline__ int stencil_corner(const FastTopo &t, int c0, int n)
     return nb[map[n]];
 }
 
+// the eight neighbour cells of table entry e (two int4)
+__device__ __forceinline__ void table_nb(const int4 *t, size_t e, int (&nb)[8])
+{
+    const int4 a = t[2 * e], b = t[2 * e + 1];
+    nb[0] = a.x; nb[1] = a.y; nb[2] = a.z; nb[3] = a.w;
+    nb[4] = b.x; nb[5] = b.y; nb[6] = b.z; nb[7] = b.w;
+}
+
+template <class S, bool FLIP>
+__device__ __forceinline__ void load_nb(const S *f, const int (&nb)[8], typename StoreTraits<S>::compute (&v)[27],
+                                        const Codec &cd)
+{
+#pragma unroll
+    for (int q = 0; q < 27; ++q) v[q] = StoreTraits<S>::load(f + pop_index(pre_plane(q, FLIP), nb[pre_nb(q)]), q, cd);
+}
+
+__device__ __forceinline__ void put_nb(int4 *t, size_t e, const int (&nb)[8])
+{
+    t[2 * e] = make_int4(nb[0], nb[1], nb[2], nb[3]);
+    t[2 * e + 1] = make_int4(nb[4], nb[5], nb[6], nb[7]);
+}
+
+// neighbour tables of one transfer list: per item, the stencil corners it reads
+// and the cell(s) it writes (F->C: the coarse cell; C->F: the 8 fine targets)
+__global__ __launch_bounds__(256) void k_if_tables(FastTopo src, FastTopo dst, int count, const int *stencilBase,
+                                                   const int *target, int c2f, int4 *srcNb, int4 *dstNb)
+{
+    const int gid = blockIdx.x * 256 + threadIdx.x;
+    const int item = gid >> 3, n = gid & 7;
+    if (item >= count) return;
+    int nb[8];
+    const int cs = stencil_corner(src, stencilBase[item], n);
+    decode_fast(src, cs, src.cellRow[cs].y, nb);
+    put_nb(srcNb, 8 * (size_t)item + n, nb);
+    if (c2f) {
+        const int ct = stencil_corner(dst, target[item], n);
+        decode_fast(dst, ct, dst.cellRow[ct].y, nb);
+        put_nb(dstNb, 8 * (size_t)item + n, nb);
+    } else if (n == 0) {
+        const int ct = target[item];
+        decode_fast(dst, ct, dst.cellRow[ct].y, nb);
+        put_nb(dstNb, item, nb);
+    }
+}
+
 template <class T> struct IfArgs {
     FastTopo coarse, fine;
+    const int4 *srcNb, *dstNb;     // neighbour tables of the transfer (InterfaceLists)
     void *fc, *ff;                 // populations of the two levels (storage type)
     T w1c, w1f;                    // omega1 of the two levels
     int count;
@@ -67,10 +113,10 @@ __global__ __launch_bounds__(256) void k_fine_to_coarse(IfArgs<typename StoreTra
     const int item = gid >> 3, n = gid & 7;
     const bool active = item < a.count;
     if (active) {
-        const int cell = stencil_corner(a.fine, a.child[item], n);
-        T v[27];
         int nb[8];
-        load_cell<S, FLIPF>(reinterpret_cast<const S *>(a.ff), a.fine, cell, v, nb, a.cd);
+        table_nb(a.srcNb, 8 * (size_t)item + n, nb);
+        T v[27];
+        load_nb<S, FLIPF>(reinterpret_cast<const S *>(a.ff), nb, v, a.cd);
         lds[threadIdx.x] = cell_values(v, a.w1f);
     }
     __syncthreads();
@@ -94,9 +140,8 @@ __global__ __launch_bounds__(256) void k_fine_to_coarse(IfArgs<typename StoreTra
         reconstruct(f, d, u, g[0][1] + g[1][0], g[1][2] + g[2][1], g[0][2] + g[2][0], g[0][0] - g[1][1], g[0][0] - g[2][2],
                     T(2) / a.w1c);
     }
-    const int cell = a.index[item];
     int nb[8];
-    decode_fast(a.coarse, cell, a.coarse.cellRow[cell].y, nb);
+    table_nb(a.dstNb, item, nb);
     S *fc = reinterpret_cast<S *>(a.fc);
     // lane n writes populations n, n+8, n+16, n+24
 #pragma unroll
@@ -115,10 +160,10 @@ __global__ __launch_bounds__(256) void k_coarse_to_fine(IfArgs<typename StoreTra
     const int item = gid >> 3, n = gid & 7;
     const bool active = item < a.count;
     if (active) {
-        const int cell = stencil_corner(a.coarse, a.index[item], n);
-        T v[27];
         int nb[8];
-        load_cell<S, FLIPC>(reinterpret_cast<const S *>(a.fc), a.coarse, cell, v, nb, a.cd);
+        table_nb(a.srcNb, 8 * (size_t)item + n, nb);
+        T v[27];
+        load_nb<S, FLIPC>(reinterpret_cast<const S *>(a.fc), nb, v, a.cd);
         lds[threadIdx.x] = cell_values(v, a.w1c);
     }
     __syncthreads();
@@ -138,9 +183,8 @@ __global__ __launch_bounds__(256) void k_coarse_to_fine(IfArgs<typename StoreTra
         reconstruct(f, d, u, g[0][1] + g[1][0], g[1][2] + g[2][1], g[0][2] + g[2][0], g[0][0] - g[1][1], g[0][0] - g[2][2],
                     T(0.5) / a.w1f);
     }
-    const int cell = stencil_corner(a.fine, a.child[item], n);
     int nb[8];
-    decode_fast(a.fine, cell, a.fine.cellRow[cell].y, nb);
+    table_nb(a.dstNb, 8 * (size_t)item + n, nb);
     S *ff = reinterpret_cast<S *>(a.ff);
 #pragma unroll
     for (int q = 0; q < 27; ++q) StoreTraits<S>::store(ff + pop_index(pre_plane(q, FLIPF), nb[pre_nb(q)]), f[q], q, a.cd);
@@ -220,11 +264,13 @@ template <class S, bool FC, bool FF> void launch_all(const IfArgs<typename Store
     IfArgs<typename StoreTraits<S>::compute> a = base;
     if (c.F2C.interfaceCount > 0) {
         a.count = c.F2C.interfaceCount; a.index = c.F2C.index.p; a.child = c.F2C.childMap.p;
+        a.srcNb = c.F2C.srcNb.p; a.dstNb = c.F2C.dstNb.p;
         if (linear) hipLaunchKernelGGL((k_fine_to_coarse<S, FC, FF, true>), dim3(groups_of_8(a.count)), dim3(256), 0, s, a);
         else hipLaunchKernelGGL((k_fine_to_coarse<S, FC, FF, false>), dim3(groups_of_8(a.count)), dim3(256), 0, s, a);
     }
     if (c.C2F.interfaceCount > 0) {
         a.count = c.C2F.interfaceCount; a.index = c.C2F.index.p; a.child = c.C2F.childMap.p;
+        a.srcNb = c.C2F.srcNb.p; a.dstNb = c.C2F.dstNb.p;
         if (linear) hipLaunchKernelGGL((k_coarse_to_fine<S, FC, FF, true>), dim3(groups_of_8(a.count)), dim3(256), 0, s, a);
         else hipLaunchKernelGGL((k_coarse_to_fine<S, FC, FF, false>), dim3(groups_of_8(a.count)), dim3(256), 0, s, a);
     }
@@ -234,6 +280,20 @@ template <class S, bool FC, bool FF> void launch_all(const IfArgs<typename Store
     }
 }
 
+// derive the neighbour tables of a transfer list (its first use after a build)
+int build_if_tables(InterfaceLists &L, const FastTopo &src, const FastTopo &dst, const int *stencilBase, const int *target,
+                    bool c2f, hipStream_t s)
+{
+    if (L.nbReady || L.interfaceCount <= 0) return 0;
+    const long long n = L.interfaceCount;
+    if (L.srcNb.alloc(16 * n) != hipSuccess || L.dstNb.alloc((c2f ? 16 : 2) * n) != hipSuccess) return LBM3_E_HIP;
+    hipLaunchKernelGGL(k_if_tables, dim3(groups_of_8(n)), dim3(256), 0, s, src, dst, L.interfaceCount, stencilBase, target,
+                       c2f ? 1 : 0, L.srcNb.p, L.dstNb.p);
+    if (hipGetLastError() != hipSuccess) return LBM3_E_HIP;
+    L.nbReady = true;
+    return 0;
+}
+
 template <class S> void interface_typed(Sim &s, int coarse)
 {
     using T = typename StoreTraits<S>::compute;
@@ -241,6 +301,7 @@ template <class S> void interface_typed(Sim &s, int coarse)
     IfArgs<T> a{};
     a.coarse = FastTopo{c.N, c.cellRow.p, c.rowTab.p};
     a.fine = FastTopo{f.N, f.cellRow.p, f.rowTab.p};
+
     a.fc = c.f.p;
     a.ff = f.f.p;
     // omega1 = 1 / (3 nu + 1/2) of each level, in the compute precision
@@ -272,6 +333,17 @@ void update_interface(Sim &s, int coarse)
     case LBM3_STORE_BF16: interface_typed<__hip_bfloat16>(s, coarse); break;
     default: interface_typed<float>(s, coarse); break;
     }
+}
+
+// the neighbour tables of the transfers between `coarse` and the next level:
+// F->C reads fine stencils (from childMap) and writes the coarse cell (index),
+// C->F reads coarse stencils (from index) and writes the fine ones (childMap)
+int prepare_interface(Sim &s, int coarse)
+{
+    Level &c = s.L[coarse], &f = s.L[coarse + 1];
+    const FastTopo ct{c.N, c.cellRow.p, c.rowTab.p}, ft{f.N, f.cellRow.p, f.rowTab.p};
+    if (const int rc = build_if_tables(c.F2C, ft, ct, c.F2C.childMap.p, c.F2C.index.p, false, s.stream)) return rc;
+    return build_if_tables(c.C2F, ct, ft, c.C2F.index.p, c.C2F.childMap.p, true, s.stream);
 }
 
 } // namespace lbm3

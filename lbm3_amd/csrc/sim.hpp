@@ -43,6 +43,11 @@ template <class T> struct DBuf {
 struct InterfaceLists {
     int interfaceCount = 0, leftoverCount = 0;
     DBuf<int> index, childMap, leftoverIndex, leftoverParent, leftoverNbrI, leftoverNbrJ, leftoverNbrK;
+    // the eight neighbour cells (two int4) of every stencil corner the transfer
+    // reads and of every cell it writes, derived from the two levels' topology
+    // before the first transfer (nbReady false: to be derived again)
+    DBuf<int4> srcNb, dstNb;
+    bool nbReady = false;
 };
 
 struct Level {
