@@ -163,6 +163,17 @@ int lbm3_reset_wall_forces(lbm3_sim *s);   /* wall and rotor force accumulators 
  * normal kinetic power [W]}.  Normal velocities are along the outer normal.
  * FP64 throughout (the reference reduces in FP32). */
 int lbm3_open_boundary_stats(lbm3_sim *s, int openBCID, double *out7);
+/* updateTracker (trackerFunctions.h:120-486) for one period: per wall body
+ * walls[6 b + {Fx, Fy, Fz, Tx, Ty, Tz}] (N, N m about the origin, the force the
+ * walls exert on the fluid), per rotor rotors[6 r + ...] (in the rotor frame,
+ * torque about its rotation point), per open boundary open[7 b + ...] (as
+ * lbm3_open_boundary_stats), each summed over the levels that have it and
+ * time-averaged over the steps since the last reset; then every accumulator
+ * and each level's iterationsSinceTrackerReset is reset.  FP64 (the
+ * reference's reductions are FP32).  The families the case does not track
+ * (trackWallForce, trackRotorForce, trackOpenBoundaries) must be asked for
+ * with a count of 0, or come back as zeros. */
+int lbm3_update_tracker(lbm3_sim *s, double *walls, int nWalls, double *rotors, int nRotors, double *open, int nOpen);
 /* Section cut through every level (exportSectionCutPlot{XY,ZY,ZX} without
  * bounds or rotor frame, include/plotter/exportSectionCutPlot.h:11-330, 395-417):
  * the plane at `cut` [mm] along its normal (z for LBM3_PLANE_XY, x for

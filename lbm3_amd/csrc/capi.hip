@@ -1058,6 +1058,46 @@ int lbm3_section_cut(lbm3_sim *h, int plane, float cut, float *out, long long ca
     return 0;
 }
 
+int lbm3_update_tracker(lbm3_sim *h, double *walls, int nWalls, double *rotors, int nRotors, double *open, int nOpen)
+{
+    if (!h) return set_error(LBM3_E_ARG, "null sim");
+    if (nWalls < 0 || nWalls > LBM3_MAX_BODIES || nRotors < 0 || nRotors > LBM3_MAX_ROTORS || nOpen < 0)
+        return set_error(LBM3_E_ARG, "bad tracker sizes");
+    if ((nWalls && !walls) || (nRotors && !rotors) || (nOpen && !open)) return set_error(LBM3_E_ARG, "null tracker output");
+    Sim &s = h->s;
+    for (int L = 0; L < s.levels; ++L) {
+        if (const int rc = check_level(s, L)) return rc;
+        if (s.L[L].info.iterationsSinceTrackerReset <= 0) return set_error(LBM3_E_ARG, "no time step since the last reset");
+    }
+    for (int i = 0; i < 6 * nWalls; ++i) walls[i] = 0.0;
+    for (int i = 0; i < 6 * nRotors; ++i) rotors[i] = 0.0;
+    const double zero[3] = {0.0, 0.0, 0.0};
+    for (int L = 0; L < s.levels; ++L) {
+        const lbm3_info &I = s.L[L].info;
+        // time average over the period, then physical units (convertToPhysicalForce,
+        // cellFunctions.h:296-301); torques about the origin come in N mm -> N m
+        const double dx = double(I.res) / 1000.0;
+        const double f = double(s.C.rhoPhys) * dx * dx * dx * dx / (double(I.dtPhys) * double(I.dtPhys)) /
+                         double(I.iterationsSinceTrackerReset);
+        double tmp[6 * LBM3_MAX_BODIES];
+        if (nWalls && s.C.trackWallForce) {
+            if (const int rc = lbm3_wall_forces(h, L, zero, nWalls, tmp)) return rc;
+            for (int b = 0; b < nWalls; ++b)
+                for (int k = 0; k < 6; ++k) walls[6 * b + k] += tmp[6 * b + k] * f * (k < 3 ? 1.0 : 0.001);
+        }
+        for (int r = 0; r < nRotors && s.C.trackRotorForce; ++r) {
+            if (r >= (int)s.L[L].rotors.size()) continue;
+            if (const int rc = lbm3_rotor_forces(h, L, r, tmp)) return rc;
+            for (int k = 0; k < 6; ++k) rotors[6 * r + k] += tmp[k] * f * (k < 3 ? 1.0 : 0.001);
+        }
+    }
+    for (int b = 0; b < nOpen; ++b)
+        if (const int rc = lbm3_open_boundary_stats(h, b, open + 7 * b)) return rc;
+    // the reset at the end of updateTracker (trackerFunctions.h:476-486)
+    if (const int rc = lbm3_reset_wall_forces(h)) return rc;
+    return lbm3_reset_open_boundary_stats(h);
+}
+
 static int check_rotor(lbm3_sim *h, int level, int rotor)
 {
     if (!h) return set_error(LBM3_E_ARG, "null sim");

@@ -56,6 +56,7 @@ def load():
         "lbm3_open_boundary_stats": (i32, [vp, i32, C.POINTER(f64)]),
         "lbm3_reset_open_boundary_stats": (i32, [vp]),
         "lbm3_section_cut": (i32, [vp, i32, C.c_float, vp, i64, C.POINTER(i32)]),
+        "lbm3_update_tracker": (i32, [vp, vp, i32, vp, i32, vp, i32]),
         "lbm3_build_rotors": (i32, [vp, vp, vp, i32, i32]),
         "lbm3_set_rotor": (i32, [vp, i32, i32, C.c_float, i32, C.c_float, C.c_float, C.c_float]),
         "lbm3_rotor_info": (i32, [vp, i32, i32, C.POINTER(f64)]),
@@ -263,6 +264,17 @@ class Sim:
 
     def reset_open_boundary_stats(self):
         check(load().lbm3_reset_open_boundary_stats(self.h))
+
+    def update_tracker(self, walls=0, rotors=0, open_boundaries=0):
+        """updateTracker for one period, physical units, then the reset:
+        {"walls": (walls, 6), "rotors": (rotors, 6), "open": (open_boundaries, 7)}."""
+        w = np.zeros((walls, 6))
+        r = np.zeros((rotors, 6))
+        o = np.zeros((open_boundaries, 7))
+        check(load().lbm3_update_tracker(self.h, w.ctypes.data if walls else None, walls,
+                                         r.ctypes.data if rotors else None, rotors,
+                                         o.ctypes.data if open_boundaries else None, open_boundaries))
+        return {"walls": w, "rotors": r, "open": o}
 
     PLANES = {"xy": 0, "zy": 1, "zx": 2}
 

@@ -50,3 +50,24 @@ def test_couette_torque_f32_matches_the_closed_form():
     assert abs(T[1] / Ta - 1.0) < 0.02 and abs(-T[0] / Ta - 1.0) < 0.02
     assert abs(T[0] + T[1]) < 0.005 * Ta
     g.close()
+
+
+def test_update_tracker_torque_in_physical_units():
+    """lbm3_update_tracker (updateTracker's wall part, trackerFunctions.h:356-474):
+    the period-averaged torque on the fluid in N m equals the closed form in
+    physical units, 4 pi rho nu L Omega R1^2 R2^2 / (R2^2 - R1^2) with the
+    case's rhoPhys, nuPhys [m^2/s], Omega [rad/s] and lengths in m, within
+    2%; a second call reads the next period only (the first one reset)."""
+    case, dom, tris, ci = cases.couette()
+    g = lib.Sim(case, "f32")
+    g.build(dom, tris)
+    g.init()
+    g.run(6000)
+    g.update_tracker(walls=2)                 # the start-up period, then the reset
+    g.run(2000)
+    t = g.update_tracker(walls=2)["walls"]
+    g.close()
+    R1, R2, L = ci["r_inner"] * 1e-3, ci["r_outer"] * 1e-3, ci["length"] * 1e-3
+    Ta = 4 * np.pi * case.rhoPhys * case.nuPhys * L * ci["omega"] * R1 ** 2 * R2 ** 2 / (R2 ** 2 - R1 ** 2)
+    print("tracker torque on the fluid: rotor %.6e stator %.6e N m, closed form %.6e" % (t[1, 5], t[0, 5], Ta))
+    assert abs(t[1, 5] / Ta - 1.0) < 0.02 and abs(-t[0, 5] / Ta - 1.0) < 0.02
