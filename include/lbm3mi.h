@@ -183,8 +183,18 @@ int lbm3_update_tracker(lbm3_sim *s, double *walls, int nWalls, double *rotors, 
  * pixel counts}; with out non-null, out[(v * H + h) * 6 + k] holds the six
  * floats the reference writes to /dev/shm/sim_data.bin per pixel: pressure
  * [Pa], velocity along the horizontal, vertical and normal axes [m/s], solid
- * marker, cell size [mm].  Rotor fractions are not added to the marker. */
-int lbm3_section_cut(lbm3_sim *s, int plane, float cut, float *out, long long capacity, int *dims);
+ * marker, cell size [mm].  bounds (null or all zero: the whole grid) crops the
+ * image as the reference's bounded overloads do.  Rotor fractions are not
+ * added to the marker. */
+int lbm3_section_cut(lbm3_sim *s, int plane, float cut, const lbm3_bounds *bounds, float *out, long long capacity,
+                     int *dims);
+/* getFlowReport{XY,ZY,ZX} (flowReportFunctions.h:11-372): the section image of
+ * lbm3_section_cut reduced over its fluid pixels, with the plane's positive
+ * normal: out7 = {area-averaged normal velocity [m/s], mass flow [kg/s],
+ * momentum thrust [N], area-averaged pressure [Pa], pressure power [W],
+ * normal kinetic power [W], fluid area [mm^2]} (FlowReportStruct order, FP64
+ * sums; zeros where the section holds no fluid). */
+int lbm3_flow_report(lbm3_sim *s, int plane, float cut, const lbm3_bounds *bounds, double *out7);
 /* zero the open-boundary accumulators and every level's iterationsSinceTrackerReset
  * (the reset at the end of updateTracker, trackerFunctions.h:476-486) */
 int lbm3_reset_open_boundary_stats(lbm3_sim *s);

@@ -987,11 +987,19 @@ int lbm3_reset_open_boundary_stats(lbm3_sim *h)
     return 0;
 }
 
-int lbm3_section_cut(lbm3_sim *h, int plane, float cut, float *out, long long capacity, int *dims)
+// The image of a section through every level before unit conversion
+// (exportSectionCutPlotGeneral / getFlowReportGeneral, exportSectionCutPlot.h:11-290,
+// flowReportFunctions.h:11-264): the plane at `cut` along its normal, cropped to
+// `bounds` (all zero or null: the whole grid), at the finest level whose image
+// fits 16 M pixels.  With img null only the size is computed.
+struct SectionImage {
+    int H = 0, V = 0, imageLevel = 0;
+    CutUnits units{};
+};
+
+static int section_image(Sim &s, int plane, float cut, const lbm3_bounds *bounds, SectionImage &out, DBuf<float> *img)
 {
-    if (!h || !dims) return set_error(LBM3_E_ARG, "null argument");
     if (plane < LBM3_PLANE_XY || plane > LBM3_PLANE_ZX) return set_error(LBM3_E_ARG, "unknown section plane");
-    Sim &s = h->s;
     for (int L = 0; L < s.levels; ++L)
         if (const int rc = check_level(s, L)) return rc;
     const int finest = s.levels - 1;
@@ -1004,32 +1012,49 @@ int lbm3_section_cut(lbm3_sim *h, int plane, float cut, float *out, long long ca
     const int cutIndex = (int)((cut - origin(F)) / F.res + 0.5f);
     if (cutIndex < 0 || cutIndex >= normalCount(F)) return set_error(LBM3_E_ARG, "section cut outside the finest grid");
     const double cutPos = double(origin(F)) + double(cutIndex) * double(F.res);
-    // the finest level whose image fits the pixel limit (exportSectionCutPlot.h:57-76)
-    int H = plane == LBM3_PLANE_XY ? F.cellCountX : F.cellCountZ;
-    int V = plane == LBM3_PLANE_ZX ? F.cellCountX : F.cellCountY;
+    // crop to the bounds (exportSectionCutPlot.h:33-55)
+    int i0 = 0, i1 = F.cellCountX, j0 = 0, j1 = F.cellCountY, k0 = 0, k1 = F.cellCountZ;
+    if (bounds && (bounds->xMin != 0.f || bounds->xMax != 0.f || bounds->yMin != 0.f || bounds->yMax != 0.f ||
+                   bounds->zMin != 0.f || bounds->zMax != 0.f)) {
+        const float xMin = F.ox - 0.5f * F.res, xMax = F.ox + F.res * (F.cellCountX - 0.5f);
+        const float yMin = F.oy - 0.5f * F.res, yMax = F.oy + F.res * (F.cellCountY - 0.5f);
+        const float zMin = F.oz - 0.5f * F.res, zMax = F.oz + F.res * (F.cellCountZ - 0.5f);
+        i0 = std::max(0, (int)ceilf((bounds->xMin - xMin) / F.res));
+        i1 = std::min(F.cellCountX, F.cellCountX - (int)ceilf((xMax - bounds->xMax) / F.res));
+        j0 = std::max(0, (int)ceilf((bounds->yMin - yMin) / F.res));
+        j1 = std::min(F.cellCountY, F.cellCountY - (int)ceilf((yMax - bounds->yMax) / F.res));
+        k0 = std::max(0, (int)ceilf((bounds->zMin - zMin) / F.res));
+        k1 = std::min(F.cellCountZ, F.cellCountZ - (int)ceilf((zMax - bounds->zMax) / F.res));
+    }
+    // the finest level whose image fits the pixel limit (:57-76)
+    int H, V, h0, v0;
+    if (plane == LBM3_PLANE_XY) { H = i1 - i0; V = j1 - j0; h0 = i0; v0 = j0; }
+    else if (plane == LBM3_PLANE_ZY) { H = k1 - k0; V = j1 - j0; h0 = k0; v0 = j0; }
+    else { H = k1 - k0; V = i1 - i0; h0 = k0; v0 = i0; }
     int imageLevel = finest;
     while ((long long)H * V > 16000000LL && imageLevel > 0) {
         --imageLevel;
-        H /= 2;
-        V /= 2;
+        H /= 2; h0 /= 2;
+        V /= 2; v0 /= 2;
     }
     if (H <= 0 || V <= 0 || (long long)H * V > 16000000LL) return set_error(LBM3_E_ARG, "section image empty or too large");
-    dims[0] = V;
-    dims[1] = H;
-    if (!out) return 0;
+    out.H = H;
+    out.V = V;
+    out.imageLevel = imageLevel;
+    out.units = CutUnits{};
+    out.units.plane = plane;
+    out.units.rhoPhys = s.C.rhoPhys;
+    for (int L = 0; L < s.levels; ++L) {
+        out.units.res[L] = s.L[L].info.res;
+        out.units.dt[L] = s.L[L].info.dtPhys;
+    }
+    if (!img) return 0;
     const long long pixels = (long long)H * V;
-    if (capacity < 6 * pixels) return set_error(LBM3_E_ARG, "buffer too small for the section image");
-    DBuf<float> img;
-    HIPCHK(img.alloc(6 * pixels));
-    launch_cut_clear(img.p, pixels, s.stream);
-    CutUnits u{};
-    u.plane = plane;
-    u.rhoPhys = s.C.rhoPhys;
+    HIPCHK(img->alloc(6 * pixels));
+    launch_cut_clear(img->p, pixels, s.stream);
     for (int L = 0; L < s.levels; ++L) {
         if (const int rc = settle_open(s, L)) return rc;
         const lbm3_info &I = s.L[L].info;
-        u.res[L] = I.res;
-        u.dt[L] = I.dtPhys;
         if (I.cellCount <= 0 || normalCount(I) <= 0 || I.res <= 0.f) continue;
         double ni = (cutPos - double(origin(I))) / double(I.res);
         ni = std::min(std::max(ni, 0.0), double(normalCount(I) - 1));
@@ -1042,19 +1067,81 @@ int lbm3_section_cut(lbm3_sim *h, int plane, float cut, float *out, long long ca
         d.up = L < imageLevel ? 1 << (imageLevel - L) : 1;
         d.pixH = H;
         d.pixV = V;
+        d.startH = h0;
+        d.startV = v0;
         d.gridID = L;
         switch (s.storage) {
-        case LBM3_STORE_F16: cut_level<__half>(s, L, d, img.p); break;
-        case LBM3_STORE_BF16: cut_level<__hip_bfloat16>(s, L, d, img.p); break;
-        case LBM3_STORE_F64: cut_level<double>(s, L, d, img.p); break;
-        default: cut_level<float>(s, L, d, img.p); break;
+        case LBM3_STORE_F16: cut_level<__half>(s, L, d, img->p); break;
+        case LBM3_STORE_BF16: cut_level<__hip_bfloat16>(s, L, d, img->p); break;
+        case LBM3_STORE_F64: cut_level<double>(s, L, d, img->p); break;
+        default: cut_level<float>(s, L, d, img->p); break;
         }
         HIPCHK(hipGetLastError());
     }
-    launch_cut_finish(img.p, pixels, u, s.stream);
+    return 0;
+}
+
+int lbm3_section_cut(lbm3_sim *h, int plane, float cut, const lbm3_bounds *bounds, float *out, long long capacity, int *dims)
+{
+    if (!h || !dims) return set_error(LBM3_E_ARG, "null argument");
+    Sim &s = h->s;
+    SectionImage si;
+    if (const int rc = section_image(s, plane, cut, bounds, si, nullptr)) return rc;
+    dims[0] = si.V;
+    dims[1] = si.H;
+    if (!out) return 0;
+    const long long pixels = (long long)si.H * si.V;
+    if (capacity < 6 * pixels) return set_error(LBM3_E_ARG, "buffer too small for the section image");
+    DBuf<float> img;
+    if (const int rc = section_image(s, plane, cut, bounds, si, &img)) return rc;
+    launch_cut_finish(img.p, pixels, si.units, s.stream);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(out, img.p, sizeof(float) * 6 * (size_t)pixels, hipMemcpyDeviceToHost, s.stream));
     HIPCHK(hipStreamSynchronize(s.stream));
+    return 0;
+}
+
+int lbm3_flow_report(lbm3_sim *h, int plane, float cut, const lbm3_bounds *bounds, double *out7)
+{
+    if (!h || !out7) return set_error(LBM3_E_ARG, "null argument");
+    Sim &s = h->s;
+    SectionImage si;
+    DBuf<float> img;
+    if (const int rc = section_image(s, plane, cut, bounds, si, &img)) return rc;
+    const long long pixels = (long long)si.H * si.V;
+    std::vector<float> host((size_t)(6 * pixels));
+    HIPCHK(hipMemcpyAsync(host.data(), img.p, sizeof(float) * host.size(), hipMemcpyDeviceToHost, s.stream));
+    HIPCHK(hipStreamSynchronize(s.stream));
+    // the reduction over the fluid pixels (flowReportFunctions.h:266-297), FP64 sums
+    double velocity = 0.0, densityVelocity = 0.0, momentum = 0.0, dRhoSum = 0.0, kinetic = 0.0;
+    long long count = 0;
+    for (long long p = 0; p < pixels; ++p) {
+        const float *v = &host[(size_t)(6 * p)];
+        if (!(v[4] < 0.5f)) continue;
+        const double d = v[0];
+        const double u = plane == LBM3_PLANE_XY ? v[3] : (plane == LBM3_PLANE_ZY ? v[1] : v[2]);
+        const double rhoU = (1.0 + d) * u;
+        velocity += u;
+        densityVelocity += d * u;
+        momentum += rhoU * std::fabs(u);
+        dRhoSum += d;
+        kinetic += 0.5 * rhoU * u * u;
+        ++count;
+    }
+    for (int k = 0; k < 7; ++k) out7[k] = 0.0;
+    if (count == 0) return 0;
+    // units of the image level (:299-335): pixels of equal area
+    const lbm3_info &R = s.L[si.imageLevel].info;
+    const double px = double(R.res) / 1000.0, area = px * px, vel = px / double(R.dtPhys);
+    const double pres = double(s.C.rhoPhys) * vel * vel / 3.0;
+    const double massFlux = double(s.C.rhoPhys) * area * vel;
+    out7[0] = velocity / double(count) * vel;                   // area-averaged normal velocity [m/s]
+    out7[1] = (velocity + densityVelocity) * massFlux;          // mass flow [kg/s]
+    out7[2] = momentum * massFlux * vel;                        // momentum thrust [N]
+    out7[3] = dRhoSum / double(count) * pres;                   // area-averaged pressure [Pa]
+    out7[4] = densityVelocity * area * pres * vel;              // pressure power [W]
+    out7[5] = kinetic * massFlux * vel * vel;                   // normal kinetic power [W]
+    out7[6] = double(count) * double(R.res) * double(R.res);    // area [mm^2]
     return 0;
 }
 

@@ -1175,7 +1175,7 @@ __global__ __launch_bounds__(256) void k_section_cut(const S *f, Topology topo, 
     const int vc = d.plane == 2 ? ijk[0] : ijk[1];
     if (hc % d.down != 0 || vc % d.down != 0) return;
     const int ih = (hc / d.down) * d.up, iv = (vc / d.down) * d.up;
-    if (ih + d.up <= 0 || ih >= d.pixH || iv + d.up <= 0 || iv >= d.pixV) return;
+    if (ih + d.up <= d.startH || ih >= d.startH + d.pixH || iv + d.up <= d.startV || iv >= d.startV + d.pixV) return;
     int upper = cell;
     int nbu[8];
     for (int k = 0; k < 8; ++k) nbu[k] = nb[k];
@@ -1207,10 +1207,10 @@ __global__ __launch_bounds__(256) void k_section_cut(const S *f, Topology topo, 
         marker = marker < 0.f ? 0.f : (marker > 1.f ? 1.f : marker);
     }
     for (int sv = 0; sv < d.up; ++sv) {
-        const int y = iv + sv;
+        const int y = iv + sv - d.startV;
         if (y < 0 || y >= d.pixV) continue;
         for (int sh = 0; sh < d.up; ++sh) {
-            const int x = ih + sh;
+            const int x = ih + sh - d.startH;
             if (x < 0 || x >= d.pixH) continue;
             float *p = img + 6 * ((long long)y * d.pixH + x);
             p[0] = val[0]; p[1] = val[1]; p[2] = val[2]; p[3] = val[3]; p[4] = marker; p[5] = float(d.gridID);

@@ -23,6 +23,7 @@ struct CutLevel {
     int interpolate;          // alpha > 0
     int down, up;             // level cells per pixel / pixels per level cell, along each in-plane axis
     int pixH, pixV;           // image size
+    int startH, startV;       // first pixel of the crop, in image-level cells
     int gridID;
 };
 struct CutUnits {

@@ -55,7 +55,8 @@ def load():
         "lbm3_reset_wall_forces": (i32, [vp]),
         "lbm3_open_boundary_stats": (i32, [vp, i32, C.POINTER(f64)]),
         "lbm3_reset_open_boundary_stats": (i32, [vp]),
-        "lbm3_section_cut": (i32, [vp, i32, C.c_float, vp, i64, C.POINTER(i32)]),
+        "lbm3_section_cut": (i32, [vp, i32, C.c_float, C.POINTER(T.Bounds), vp, i64, C.POINTER(i32)]),
+        "lbm3_flow_report": (i32, [vp, i32, C.c_float, C.POINTER(T.Bounds), C.POINTER(f64)]),
         "lbm3_update_tracker": (i32, [vp, vp, i32, vp, i32, vp, i32]),
         "lbm3_build_rotors": (i32, [vp, vp, vp, i32, i32]),
         "lbm3_set_rotor": (i32, [vp, i32, i32, C.c_float, i32, C.c_float, C.c_float, C.c_float]),
@@ -278,16 +279,27 @@ class Sim:
 
     PLANES = {"xy": 0, "zy": 1, "zx": 2}
 
-    def section_cut(self, plane, cut):
+    def section_cut(self, plane, cut, bounds=None):
         """exportSectionCutPlot's image of the plane ("xy", "zy", "zx") at `cut`
-        mm: (vertical, horizontal, 6) float32 of {p [Pa], u horizontal, u vertical,
-        u normal [m/s], marker, res [mm]}."""
+        mm, cropped to bounds (an lbm3_bounds or None): (vertical, horizontal, 6)
+        float32 of {p [Pa], u horizontal, u vertical, u normal [m/s], marker, res [mm]}."""
         pl = self.PLANES[plane]
+        b = C.byref(bounds) if bounds is not None else None
         dims = (C.c_int * 2)()
-        check(load().lbm3_section_cut(self.h, pl, float(cut), None, 0, dims))
+        check(load().lbm3_section_cut(self.h, pl, float(cut), b, None, 0, dims))
         out = np.zeros((dims[0], dims[1], 6), np.float32)
-        check(load().lbm3_section_cut(self.h, pl, float(cut), out.ctypes.data, out.size, dims))
+        check(load().lbm3_section_cut(self.h, pl, float(cut), b, out.ctypes.data, out.size, dims))
         return out
+
+    FLOW_REPORT = ("normal_velocity", "mass_flow", "momentum_thrust", "pressure", "pressure_power",
+                   "normal_kinetic_power", "area_mm2")
+
+    def flow_report(self, plane, cut, bounds=None):
+        """getFlowReport{XY,ZY,ZX}: the section's fluxes in physical units (dict)."""
+        out = (C.c_double * 7)()
+        b = C.byref(bounds) if bounds is not None else None
+        check(load().lbm3_flow_report(self.h, self.PLANES[plane], float(cut), b, out))
+        return dict(zip(self.FLOW_REPORT, out[:]))
 
     def build_rotors(self, rotors, level_mask=0xFF):
         """buildRotors on the levels of level_mask; rotors: list of (n, 9) triangle arrays."""

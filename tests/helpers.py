@@ -182,11 +182,13 @@ def centreline_pressure(sim, case):
     return float(info.oz) + np.arange(nz) * float(res), p
 
 
-def section_cut_reference(sim, case, plane, cut):
+def section_cut_reference(sim, case, plane, cut, lattice=False):
     """exportSectionCutPlotGeneral (include/plotter/exportSectionCutPlot.h:11-330)
     restated on an oracle's macroscopic fields, all levels: returns the
     (V, H, 6) image {p, u horizontal, u vertical, u normal, marker, res} of
-    the plane ("xy", "zy", "zx") at `cut` mm, without rotor fractions."""
+    the plane ("xy", "zy", "zx") at `cut` mm, without rotor fractions; with
+    lattice=True the image before conversion {dRho, ux, uy, uz, marker, level}
+    and the image level."""
     f32 = np.float32
     P = {"xy": 0, "zy": 1, "zx": 2}[plane]
     levels = sim.levels
@@ -241,6 +243,8 @@ def section_cut_reference(sim, case, plane, cut):
             img[iv:iv + up, ih:ih + up, :4] = val
             img[iv:iv + up, ih:ih + up, 4] = 1.0 if solid else 0.0
             img[iv:iv + up, ih:ih + up, 5] = L
+    if lattice:
+        return img, image_level
     out = img.copy()
     for L, I in enumerate(infos):
         sel = img[..., 5] == L
@@ -254,3 +258,23 @@ def section_cut_reference(sim, case, plane, cut):
             out[..., 1 + a] = np.where(sel, comp[a], out[..., 1 + a])
         out[..., 5] = np.where(sel, res, out[..., 5])
     return out
+
+
+def flow_report_reference(sim, case, plane, cut):
+    """getFlowReportGeneral's reduction (flowReportFunctions.h:266-335) over the
+    restated section image: dict as lib.Sim.flow_report."""
+    img, level = section_cut_reference(sim, case, plane, cut, lattice=True)
+    fluid = img[..., 4] < 0.5
+    d = img[..., 0][fluid].astype(np.float64)
+    u = img[..., {"xy": 3, "zy": 1, "zx": 2}[plane]][fluid].astype(np.float64)
+    n = len(d)
+    I = sim.info(level)
+    px = float(np.float32(I.res)) / 1000.0
+    area, vel = px * px, px / float(np.float32(I.dtPhys))
+    pres = float(np.float32(case.rhoPhys)) * vel * vel / 3.0
+    mf = float(np.float32(case.rhoPhys)) * area * vel
+    rho_u = (1.0 + d) * u
+    return {"normal_velocity": u.sum() / n * vel, "mass_flow": (u.sum() + (d * u).sum()) * mf,
+            "momentum_thrust": (rho_u * np.abs(u)).sum() * mf * vel, "pressure": d.sum() / n * pres,
+            "pressure_power": (d * u).sum() * area * pres * vel, "normal_kinetic_power": (0.5 * rho_u * u * u).sum() * mf * vel * vel,
+            "area_mm2": n * float(np.float32(I.res)) ** 2}

@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 
 from lbm3_amd import cases, geometry, lib
-from tests.helpers import GOLDEN, centreline_pressure, section_cut_reference
+from tests.helpers import GOLDEN, centreline_pressure, flow_report_reference, section_cut_reference
 from tests.oracle_lib import OracleSim
 
 pytestmark = pytest.mark.gpu
@@ -78,3 +78,41 @@ def test_two_level_poiseuille_centreline_matches_the_reference():
     dev = np.max(np.abs(p - ref[:, 1])) / np.max(np.abs(ref[:, 1]))
     print("two-level poiseuille: centre-line pressure max deviation %.2e of the inlet pressure" % dev)
     assert dev < 3e-3
+
+
+def test_flow_report_matches_the_restatement_f64():
+    """lbm3_flow_report (getFlowReport*) on two levels, FP64: every quantity
+    within 1e-6 of the reduction restated on the oracle's section image, for a
+    cross-section of the pipe and a cropped plane along it."""
+    case, dom, tris = _pipe(2)
+    o = OracleSim(case, dom, tris, double=True)
+    o.init()
+    o.run(30)
+    g = lib.Sim(case, "f64")
+    g.build(dom, tris)
+    g.init()
+    g.run(30)
+    for plane, cut in (("xy", 0.3), ("xy", -7.9), ("zy", 0.0)):
+        a, b = g.flow_report(plane, cut), flow_report_reference(o, case, plane, cut)
+        for k, v in b.items():
+            assert abs(a[k] - v) <= 1e-6 * max(abs(v), 1e-30), (plane, cut, k, a[k], v)
+    g.close()
+
+
+def test_flow_report_conserves_the_inlet_mass_flow():
+    """One level, 10000 steps: the mass flow through the mid-pipe section equals
+    the inlet's (open-boundary statistics, outer normal) within 1%, over the
+    bore's area (pi (10 mm)^2 within 1%)."""
+    case, dom, tris = _pipe(1)
+    s = lib.Sim(case, "f32")
+    s.build(dom, tris)
+    s.init()
+    s.run(10000)
+    s.reset_open_boundary_stats()
+    s.run(1)
+    inlet = s.open_boundary_stats(0)
+    rep = s.flow_report("xy", 0.0)
+    s.close()
+    print("mid-pipe mass flow %.6e kg/s, inlet %.6e kg/s, area %.2f mm^2" % (rep["mass_flow"], -inlet["mass_flow"], rep["area_mm2"]))
+    assert abs(rep["mass_flow"] / -inlet["mass_flow"] - 1.0) < 0.01
+    assert abs(rep["area_mm2"] / (np.pi * 100.0) - 1.0) < 0.01
