@@ -188,6 +188,11 @@ int lbm3_update_tracker(lbm3_sim *s, double *walls, int nWalls, double *rotors, 
  * added to the marker. */
 int lbm3_section_cut(lbm3_sim *s, int plane, float cut, const lbm3_bounds *bounds, float *out, long long capacity,
                      int *dims);
+/* exportVTK (include/exportVTK.h:17-158): every stored cell of every level as
+ * one vertex of a legacy ASCII VTK polydata file at `path` (coordinates in
+ * mm), with the point data wall, velocity_m_s, pressure_Pa, grid_id,
+ * grid_resolution_mm and interface_overlap. */
+int lbm3_export_vtk(lbm3_sim *s, const char *path);
 /* getFlowReport{XY,ZY,ZX} (flowReportFunctions.h:11-372): the section image of
  * lbm3_section_cut reduced over its fluid pixels, with the plane's positive
  * normal: out7 = {area-averaged normal velocity [m/s], mass flow [kg/s],

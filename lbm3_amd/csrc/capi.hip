@@ -1101,6 +1101,93 @@ int lbm3_section_cut(lbm3_sim *h, int plane, float cut, const lbm3_bounds *bound
     return 0;
 }
 
+int lbm3_export_vtk(lbm3_sim *h, const char *path)
+{
+    if (!h || !path) return set_error(LBM3_E_ARG, "null argument");
+    Sim &s = h->s;
+    long long total = 0;
+    for (int L = 0; L < s.levels; ++L) {
+        if (const int rc = check_level(s, L)) return rc;
+        if (const int rc = settle_open(s, L)) return rc;
+        total += s.L[L].N;
+    }
+    // the nine fields per cell, level by level (exportVTK.h:33-112)
+    std::vector<std::vector<float>> fields((size_t)s.levels);
+    for (int L = 0; L < s.levels; ++L) {
+        Level &l = s.L[L];
+        DBuf<float> d;
+        HIPCHK(d.alloc(9LL * (l.N > 0 ? l.N : 1)));
+        const bool flip = l.info.esotwistFlipper != 0;
+        const Topology t = topo_of(l);
+        switch (s.storage) {
+        case LBM3_STORE_F16: launch_vtk_fields<__half>(reinterpret_cast<const __half *>(l.f.p), t, l.wallMap.p, l.wallData.p, s.codec, flip, l.geom, s.C.rhoPhys, d.p, s.stream); break;
+        case LBM3_STORE_BF16: launch_vtk_fields<__hip_bfloat16>(reinterpret_cast<const __hip_bfloat16 *>(l.f.p), t, l.wallMap.p, l.wallData.p, s.codec, flip, l.geom, s.C.rhoPhys, d.p, s.stream); break;
+        case LBM3_STORE_F64: launch_vtk_fields<double>(reinterpret_cast<const double *>(l.f.p), t, l.wallMap.p, l.wallData.p, s.codec, flip, l.geom, s.C.rhoPhys, d.p, s.stream); break;
+        default: launch_vtk_fields<float>(reinterpret_cast<const float *>(l.f.p), t, l.wallMap.p, l.wallData.p, s.codec, flip, l.geom, s.C.rhoPhys, d.p, s.stream); break;
+        }
+        HIPCHK(hipGetLastError());
+        fields[(size_t)L].resize(9 * (size_t)l.N);
+        if (l.N > 0)
+            HIPCHK(hipMemcpyAsync(fields[(size_t)L].data(), d.p, sizeof(float) * fields[(size_t)L].size(), hipMemcpyDeviceToHost,
+                                  s.stream));
+        HIPCHK(hipStreamSynchronize(s.stream));
+    }
+    // legacy ASCII polydata, one vertex per stored cell (exportVTK.h:118-156)
+    FILE *fp = fopen(path, "w");
+    if (!fp) return set_error(LBM3_E_ARG, std::string("cannot open VTK file: ") + path);
+    auto fl = [](float v) {
+        char b[32];
+        snprintf(b, sizeof b, "%.9g", (double)v);
+        return std::string(b);
+    };
+    std::string o;
+    o.reserve(1 << 20);
+    auto flush = [&](bool force) {
+        if (force || o.size() > (1u << 20)) { fwrite(o.data(), 1, o.size(), fp); o.clear(); }
+    };
+    o += "# vtk DataFile Version 3.0\nLBM snapshot; coordinates in mm; grid count " + std::to_string(s.levels) + "\n";
+    o += "ASCII\nDATASET POLYDATA\nPOINTS " + std::to_string(total) + " float\n";
+    auto each = [&](int field, auto &&emit) {
+        for (int L = 0; L < s.levels; ++L) {
+            const long long N = s.L[L].N;
+            const float *F = fields[(size_t)L].data() + (size_t)field * (size_t)N;
+            for (long long c = 0; c < N; ++c) { emit(L, F, c); flush(false); }
+        }
+    };
+    for (int L = 0; L < s.levels; ++L) {
+        const long long N = s.L[L].N;
+        const float *F = fields[(size_t)L].data();
+        for (long long c = 0; c < N; ++c) {
+            o += fl(F[c]) + ' ' + fl(F[N + c]) + ' ' + fl(F[2 * N + c]) + '\n';
+            flush(false);
+        }
+    }
+    o += "VERTICES " + std::to_string(total) + ' ' + std::to_string(2 * total) + '\n';
+    for (long long c = 0; c < total; ++c) { o += "1 " + std::to_string(c) + '\n'; flush(false); }
+    o += "POINT_DATA " + std::to_string(total) + "\nSCALARS wall int 1\nLOOKUP_TABLE default\n";
+    each(3, [&](int, const float *F, long long c) { o += std::to_string((int)F[c]) + '\n'; });
+    o += "VECTORS velocity_m_s float\n";
+    for (int L = 0; L < s.levels; ++L) {
+        const long long N = s.L[L].N;
+        const float *F = fields[(size_t)L].data();
+        for (long long c = 0; c < N; ++c) {
+            o += fl(F[4 * N + c]) + ' ' + fl(F[5 * N + c]) + ' ' + fl(F[6 * N + c]) + '\n';
+            flush(false);
+        }
+    }
+    o += "SCALARS pressure_Pa float 1\nLOOKUP_TABLE default\n";
+    each(7, [&](int, const float *F, long long c) { o += fl(F[c]) + '\n'; });
+    o += "SCALARS grid_id int 1\nLOOKUP_TABLE default\n";
+    each(0, [&](int L, const float *, long long) { o += std::to_string((int)s.L[L].info.gridID) + '\n'; });
+    o += "SCALARS grid_resolution_mm float 1\nLOOKUP_TABLE default\n";
+    each(0, [&](int L, const float *, long long) { o += fl(s.L[L].info.res) + '\n'; });
+    o += "SCALARS interface_overlap int 1\nLOOKUP_TABLE default\n";
+    each(8, [&](int, const float *F, long long c) { o += std::to_string((int)F[c]) + '\n'; });
+    flush(true);
+    const bool ok = fclose(fp) == 0;
+    return ok ? 0 : set_error(LBM3_E_ARG, std::string("failed writing VTK file: ") + path);
+}
+
 int lbm3_flow_report(lbm3_sim *h, int plane, float cut, const lbm3_bounds *bounds, double *out7)
 {
     if (!h || !out7) return set_error(LBM3_E_ARG, "null argument");

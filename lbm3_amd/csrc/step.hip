@@ -1236,6 +1236,49 @@ __global__ __launch_bounds__(256) void k_cut_finish(float *img, long long pixels
     v[0] = pr; v[1] = h; v[2] = vv; v[3] = n; v[5] = res;
 }
 
+// per cell of a level, the nine fields of exportVTK (include/exportVTK.h:64-112):
+// x, y, z [mm], wall, ux, uy, uz [m/s], pressure [Pa], interface overlap
+template <class S, bool FLIP>
+__global__ __launch_bounds__(256) void k_vtk_fields(const S *f, Topology topo, const int *wallMap, const uint32_t *wallData,
+                                                    Codec cd, LevelGeom g, float rhoPhys, float *out)
+{
+    using T = typename StoreTraits<S>::compute;
+    const int cell = blockIdx.x * 256 + threadIdx.x;
+    if (cell >= topo.N) return;
+    int nb[8], ijk[3];
+    decode(topo, cell, nb, ijk);
+    const int wm = wallMap[cell];
+    const bool wall = wm == -3;
+    float v[9];
+    v[0] = float(ijk[0]) * g.res + g.ox;
+    v[1] = float(ijk[1]) * g.res + g.oy;
+    v[2] = float(ijk[2]) * g.res + g.oz;
+    v[3] = wall ? 1.f : 0.f;
+    v[4] = v[5] = v[6] = v[7] = v[8] = 0.f;
+    if (!wall) {
+        v[8] = (wm == -2 || (wm >= 0 && (wallData[wm] & 1u))) ? 1.f : 0.f;
+        T q[27], d, u0, u1, u2;
+        load_pre<S, FLIP>(f, topo.N, nb, q, cd);
+        moments(q, d, u0, u1, u2);
+        const float ss = 0.57735026918962576f * (g.res / 1000.f) / g.dtPhys;
+        v[4] = float(u0) * (g.res / 1000.f) / g.dtPhys;
+        v[5] = float(u1) * (g.res / 1000.f) / g.dtPhys;
+        v[6] = float(u2) * (g.res / 1000.f) / g.dtPhys;
+        v[7] = float(d) * rhoPhys * ss * ss;
+    }
+    for (int k = 0; k < 9; ++k) out[(size_t)k * topo.N + cell] = v[k];
+}
+
+template <class S> void launch_vtk_fields(const S *f, const Topology &t, const int *wallMap, const uint32_t *wallData,
+                                          const Codec &cd, bool flip, const LevelGeom &g, float rhoPhys, float *out,
+                                          hipStream_t s)
+{
+    if (t.N <= 0) return;
+    const unsigned b = (unsigned)((t.N + 255) / 256);
+    if (flip) k_vtk_fields<S, true><<<b, 256, 0, s>>>(f, t, wallMap, wallData, cd, g, rhoPhys, out);
+    else k_vtk_fields<S, false><<<b, 256, 0, s>>>(f, t, wallMap, wallData, cd, g, rhoPhys, out);
+}
+
 template <class S> void launch_section_cut(const S *f, const Topology &t, const int *wallMap, const Codec &cd, bool flip,
                                            const CutLevel &d, float *img, hipStream_t s)
 {
@@ -1256,6 +1299,8 @@ void launch_cut_finish(float *img, long long pixels, const CutUnits &u, hipStrea
 }
 
 #define LBM3_INSTANTIATE(S)                                                                                          \
+    template void launch_vtk_fields<S>(const S *, const Topology &, const int *, const uint32_t *, const Codec &, bool,  \
+                                       const LevelGeom &, float, float *, hipStream_t);                                \
     template void launch_section_cut<S>(const S *, const Topology &, const int *, const Codec &, bool, const CutLevel &, \
                                         float *, hipStream_t);                                                       \
     template void launch_collide_stream<S>(const StepArgs<S> &, bool, hipStream_t);                                 \

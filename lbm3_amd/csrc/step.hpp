@@ -34,6 +34,9 @@ struct CutUnits {
 template <class S> void launch_section_cut(const S *f, const Topology &t, const int *wallMap, const Codec &cd, bool flip,
                                            const CutLevel &d, float *img, hipStream_t s);
 void launch_cut_clear(float *img, long long pixels, hipStream_t s);
+template <class S> void launch_vtk_fields(const S *f, const Topology &t, const int *wallMap, const uint32_t *wallData,
+                                          const Codec &cd, bool flip, const LevelGeom &g, float rhoPhys, float *out,
+                                          hipStream_t s);
 void launch_cut_finish(float *img, long long pixels, const CutUnits &u, hipStream_t s);
 
 // the radial profile of every open rule, and none for the default record

@@ -57,6 +57,7 @@ def load():
         "lbm3_reset_open_boundary_stats": (i32, [vp]),
         "lbm3_section_cut": (i32, [vp, i32, C.c_float, C.POINTER(T.Bounds), vp, i64, C.POINTER(i32)]),
         "lbm3_flow_report": (i32, [vp, i32, C.c_float, C.POINTER(T.Bounds), C.POINTER(f64)]),
+        "lbm3_export_vtk": (i32, [vp, C.c_char_p]),
         "lbm3_update_tracker": (i32, [vp, vp, i32, vp, i32, vp, i32]),
         "lbm3_build_rotors": (i32, [vp, vp, vp, i32, i32]),
         "lbm3_set_rotor": (i32, [vp, i32, i32, C.c_float, i32, C.c_float, C.c_float, C.c_float]),
@@ -290,6 +291,10 @@ class Sim:
         out = np.zeros((dims[0], dims[1], 6), np.float32)
         check(load().lbm3_section_cut(self.h, pl, float(cut), b, out.ctypes.data, out.size, dims))
         return out
+
+    def export_vtk(self, path):
+        """exportVTK: every stored cell of every level to a legacy ASCII VTK file."""
+        check(load().lbm3_export_vtk(self.h, str(path).encode()))
 
     FLOW_REPORT = ("normal_velocity", "mass_flow", "momentum_thrust", "pressure", "pressure_power",
                    "normal_kinetic_power", "area_mm2")

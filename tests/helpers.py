@@ -278,3 +278,66 @@ def flow_report_reference(sim, case, plane, cut):
             "momentum_thrust": (rho_u * np.abs(u)).sum() * mf * vel, "pressure": d.sum() / n * pres,
             "pressure_power": (d * u).sum() * area * pres * vel, "normal_kinetic_power": (0.5 * rho_u * u * u).sum() * mf * vel * vel,
             "area_mm2": n * float(np.float32(I.res)) ** 2}
+
+
+def read_vtk(path):
+    """Parse the legacy ASCII polydata exportVTK writes (include/exportVTK.h:118-156):
+    dict with the header line, points (N,3), vertex pairs (N,2) and every
+    point-data array by name."""
+    with open(path) as fh:
+        lines = fh.read().split("\n")
+    assert lines[0] == "# vtk DataFile Version 3.0" and lines[2] == "ASCII" and lines[3] == "DATASET POLYDATA"
+    out = {"title": lines[1]}
+    n = int(lines[4].split()[1])
+    assert lines[4] == "POINTS %d float" % n
+    at = 5
+    take = lambda k, dt: np.array([r.split() for r in lines[at:at + k]], dt).reshape(k, -1)
+    out["points"] = take(n, np.float32)
+    at += n
+    assert lines[at] == "VERTICES %d %d" % (n, 2 * n)
+    at += 1
+    out["vertices"] = take(n, np.int64)
+    at += n
+    assert lines[at] == "POINT_DATA %d" % n
+    at += 1
+    while at < len(lines) and lines[at]:
+        head = lines[at].split()
+        at += 1
+        if head[0] == "SCALARS":
+            assert lines[at] == "LOOKUP_TABLE default"
+            at += 1
+        dt = np.int64 if head[2] == "int" else np.float32
+        out[head[1]] = take(n, dt).squeeze(1) if head[0] == "SCALARS" else take(n, dt)
+        at += n
+    assert at == len(lines) - 1
+    return out
+
+
+def vtk_reference(sim, case):
+    """exportVTK's nine fields per cell (include/exportVTK.h:64-112) restated on
+    an oracle's macroscopic fields, levels in order: dict of arrays."""
+    f32 = np.float32
+    parts = {k: [] for k in ("points", "wall", "velocity_m_s", "pressure_Pa", "grid_id", "grid_resolution_mm",
+                             "interface_overlap")}
+    for L in range(sim.levels):
+        I = sim.info(L)
+        ijk = cell_ijk(sim, L)
+        wm = sim.array("wallMap", L)
+        wd = sim.array("wallData", L)
+        m = sim.macros(L).astype(f32)
+        res, dt = f32(I.res), f32(I.dtPhys)
+        parts["points"].append(np.stack([ijk[:, 0].astype(f32) * res + f32(I.ox), ijk[:, 1].astype(f32) * res + f32(I.oy),
+                                         ijk[:, 2].astype(f32) * res + f32(I.oz)], 1))
+        wall = wm == -3
+        wd = np.concatenate([wd, np.zeros(1, wd.dtype)])                   # a level without walls has none
+        overlap = ~wall & ((wm == -2) | ((wm >= 0) & ((wd[np.maximum(wm, 0)] & 1) != 0)))
+        vel = m[:, 1:4] * (res / f32(1000.0)) / dt
+        ss = f32(0.57735026918962576) * (res / f32(1000.0)) / dt
+        p = m[:, 0] * f32(case.rhoPhys) * ss * ss
+        parts["wall"].append(wall.astype(np.int64))
+        parts["velocity_m_s"].append(np.where(wall[:, None], f32(0), vel))
+        parts["pressure_Pa"].append(np.where(wall, f32(0), p))
+        parts["grid_id"].append(np.full(len(wm), I.gridID, np.int64))
+        parts["grid_resolution_mm"].append(np.full(len(wm), res, f32))
+        parts["interface_overlap"].append(overlap.astype(np.int64))
+    return {k: np.concatenate(v) for k, v in parts.items()}
