@@ -188,6 +188,17 @@ int lbm3_update_tracker(lbm3_sim *s, double *walls, int nWalls, double *rotors, 
  * added to the marker. */
 int lbm3_section_cut(lbm3_sim *s, int plane, float cut, const lbm3_bounds *bounds, float *out, long long capacity,
                      int *dims);
+/* Checkpoint and restart (no counterpart in the reference, which keeps its
+ * state in device memory only): lbm3_save_checkpoint writes the state a time
+ * step changes -- populations, esotwist parity and iteration counters of every
+ * level, open-boundary reference states and statistics, wall and rotor force
+ * accumulators, rotor angles -- to `path`; lbm3_load_checkpoint reads it into
+ * a simulation built from the same case and grid (checked: storage, level
+ * count, cell, wall, open-boundary and rotor counts, grid sizes), after which
+ * its steps reproduce the saved simulation's bit for bit.  Any open boundary
+ * condition still pending is saved as pending. */
+int lbm3_save_checkpoint(lbm3_sim *s, const char *path);
+int lbm3_load_checkpoint(lbm3_sim *s, const char *path);
 /* exportVTK (include/exportVTK.h:17-158): every stored cell of every level as
  * one vertex of a legacy ASCII VTK polydata file at `path` (coordinates in
  * mm), with the point data wall, velocity_m_s, pressure_Pa, grid_id,

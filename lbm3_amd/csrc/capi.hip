@@ -452,6 +452,61 @@ template <class S> static void cut_level(Sim &s, int L, const CutLevel &d, float
                           d, img, s.stream);
 }
 
+// checkpoint file: magic, {version, storage, levels}, fluid updates, then per
+// level a CheckpointLevel, the level's lbm3_info, the buffers checkpoint_buffers
+// names (raw device bytes) and each rotor's angle
+static const char kCheckpointMagic[8] = {'L', 'B', 'M', '3', 'C', 'K', 'P', 'T'};
+static constexpr int kCheckpointVersion = 1;
+struct CheckpointLevel {
+    long long N, W, nOpen, nRotors;
+    int openSettled, pad;
+};
+
+static CheckpointLevel checkpoint_level(const Level &l)
+{
+    return {l.N, l.W, l.nOpen, (long long)l.rotors.size(), l.openSettled ? 1 : 0, 0};
+}
+
+// the device state a time step changes: populations, open-boundary reference
+// states and statistics, wall and rotor force accumulators
+static std::vector<std::pair<void *, size_t>> checkpoint_buffers(Level &l, int storage)
+{
+    std::vector<std::pair<void *, size_t>> b;
+    b.push_back({l.f.p, (size_t)l.f.n});
+    b.push_back({l.openRefs.p, 2 * (size_t)l.nOpen * compute_bytes(storage)});
+    b.push_back({l.openCum.p, sizeof(double) * 2 * (size_t)l.nOpen});
+    b.push_back({l.gWall.p, sizeof(double) * (size_t)l.gWall.n});
+    for (auto &r : l.rotors) b.push_back({r.g.p, sizeof(double) * (size_t)r.g.n});
+    return b;
+}
+
+static constexpr size_t kStageBytes = size_t(64) << 20;
+
+static int dump_device(FILE *fp, const void *dev, size_t bytes, std::vector<unsigned char> &stage, hipStream_t st, bool &ok)
+{
+    stage.resize(std::min(bytes, kStageBytes));
+    for (size_t o = 0; ok && o < bytes; o += kStageBytes) {
+        const size_t n = std::min(kStageBytes, bytes - o);
+        HIPCHK(hipMemcpyAsync(stage.data(), static_cast<const unsigned char *>(dev) + o, n, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        ok = fwrite(stage.data(), 1, n, fp) == n;
+    }
+    return 0;
+}
+
+static int load_device(FILE *fp, void *dev, size_t bytes, std::vector<unsigned char> &stage, hipStream_t st, bool &ok)
+{
+    stage.resize(std::min(bytes, kStageBytes));
+    for (size_t o = 0; ok && o < bytes; o += kStageBytes) {
+        const size_t n = std::min(kStageBytes, bytes - o);
+        ok = fread(stage.data(), 1, n, fp) == n;
+        if (!ok) break;
+        HIPCHK(hipMemcpyAsync(static_cast<unsigned char *>(dev) + o, stage.data(), n, hipMemcpyHostToDevice, st));
+        HIPCHK(hipStreamSynchronize(st));
+    }
+    return 0;
+}
+
 extern "C" {
 
 const char *lbm3_last_error(void) { return g_error.c_str(); }
@@ -1099,6 +1154,99 @@ int lbm3_section_cut(lbm3_sim *h, int plane, float cut, const lbm3_bounds *bound
     HIPCHK(hipMemcpyAsync(out, img.p, sizeof(float) * 6 * (size_t)pixels, hipMemcpyDeviceToHost, s.stream));
     HIPCHK(hipStreamSynchronize(s.stream));
     return 0;
+}
+
+int lbm3_save_checkpoint(lbm3_sim *h, const char *path)
+{
+    if (!h || !path) return set_error(LBM3_E_ARG, "null argument");
+    Sim &s = h->s;
+    for (int L = 0; L < s.levels; ++L)
+        if (const int rc = check_level(s, L)) return rc;
+    HIPCHK(hipStreamSynchronize(s.stream));
+    FILE *fp = fopen(path, "wb");
+    if (!fp) return set_error(LBM3_E_ARG, std::string("cannot open checkpoint file: ") + path);
+    std::vector<unsigned char> stage;
+    bool ok = fwrite(kCheckpointMagic, 1, 8, fp) == 8;
+    const int head[3] = {kCheckpointVersion, s.storage, s.levels};
+    ok = ok && fwrite(head, sizeof head, 1, fp) == 1 && fwrite(&s.fluidUpdates, sizeof s.fluidUpdates, 1, fp) == 1;
+    int rc = 0;
+    for (int L = 0; ok && !rc && L < s.levels; ++L) {
+        Level &l = s.L[L];
+        const CheckpointLevel d = checkpoint_level(l);
+        ok = fwrite(&d, sizeof d, 1, fp) == 1 && fwrite(&l.info, sizeof l.info, 1, fp) == 1;
+        for (const auto &b : checkpoint_buffers(l, s.storage))
+            if (ok && !rc) rc = dump_device(fp, b.first, b.second, stage, s.stream, ok);
+        for (const auto &r : l.rotors)
+            ok = ok && fwrite(&r.radiansElapsed, sizeof r.radiansElapsed, 1, fp) == 1;
+    }
+    ok = (fclose(fp) == 0) && ok;
+    if (rc) return rc;
+    return ok ? 0 : set_error(LBM3_E_ARG, std::string("failed writing checkpoint file: ") + path);
+}
+
+int lbm3_load_checkpoint(lbm3_sim *h, const char *path)
+{
+    if (!h || !path) return set_error(LBM3_E_ARG, "null argument");
+    Sim &s = h->s;
+    for (int L = 0; L < s.levels; ++L)
+        if (const int rc = check_level(s, L)) return rc;
+    FILE *fp = fopen(path, "rb");
+    if (!fp) return set_error(LBM3_E_ARG, std::string("cannot open checkpoint file: ") + path);
+    auto fail = [&](const char *why) {
+        fclose(fp);
+        return set_error(LBM3_E_ARG, std::string("checkpoint ") + path + ": " + why);
+    };
+    char magic[8];
+    int head[3];
+    long long updates = 0;
+    if (fread(magic, 1, 8, fp) != 8 || memcmp(magic, kCheckpointMagic, 8) != 0) return fail("not a checkpoint file");
+    if (fread(head, sizeof head, 1, fp) != 1 || fread(&updates, sizeof updates, 1, fp) != 1) return fail("truncated header");
+    if (head[0] != kCheckpointVersion) return fail("unsupported version");
+    if (head[1] != s.storage) return fail("population storage differs from the simulation's");
+    if (head[2] != s.levels) return fail("level count differs from the simulation's");
+    // read and check every level's shape before anything on the device changes
+    std::vector<lbm3_info> infos((size_t)s.levels);
+    std::vector<long> at((size_t)s.levels);
+    std::vector<char> settled((size_t)s.levels);
+    for (int L = 0; L < s.levels; ++L) {
+        Level &l = s.L[L];
+        CheckpointLevel d;
+        if (fread(&d, sizeof d, 1, fp) != 1 || fread(&infos[(size_t)L], sizeof(lbm3_info), 1, fp) != 1)
+            return fail("truncated level record");
+        const CheckpointLevel e = checkpoint_level(l);
+        const lbm3_info &I = infos[(size_t)L];
+        if (d.N != e.N || d.W != e.W || d.nOpen != e.nOpen || d.nRotors != e.nRotors || I.cellCount != l.info.cellCount ||
+            I.cellCountX != l.info.cellCountX || I.cellCountY != l.info.cellCountY || I.cellCountZ != l.info.cellCountZ ||
+            I.gridID != l.info.gridID || I.res != l.info.res)
+            return fail("grid differs from the simulation's");
+        at[(size_t)L] = ftell(fp);
+        long skip = 0;
+        for (const auto &b : checkpoint_buffers(l, s.storage)) skip += (long)b.second;
+        skip += (long)(sizeof(double) * l.rotors.size());
+        if (fseek(fp, skip, SEEK_CUR) != 0) return fail("seek failed");
+        settled[(size_t)L] = d.openSettled != 0;
+    }
+    // the file ends where the last level's data ends (a short file changes nothing)
+    const long end = ftell(fp);
+    if (fseek(fp, 0, SEEK_END) != 0 || ftell(fp) != end) return fail("file size does not match its records");
+    std::vector<unsigned char> stage;
+    for (int L = 0; L < s.levels; ++L) {
+        Level &l = s.L[L];
+        if (fseek(fp, at[(size_t)L], SEEK_SET) != 0) return fail("seek failed");
+        bool ok = true;
+        for (const auto &b : checkpoint_buffers(l, s.storage)) {
+            const int rc = load_device(fp, b.first, b.second, stage, s.stream, ok);
+            if (rc) { fclose(fp); return rc; }
+            if (!ok) return fail("truncated level data");
+        }
+        for (auto &r : l.rotors)
+            if (fread(&r.radiansElapsed, sizeof r.radiansElapsed, 1, fp) != 1) return fail("truncated rotor state");
+        l.info = infos[(size_t)L];
+        l.openSettled = settled[(size_t)L] != 0;
+    }
+    fclose(fp);
+    s.fluidUpdates = updates;
+    return check_hip(hipStreamSynchronize(s.stream), "checkpoint upload");
 }
 
 int lbm3_export_vtk(lbm3_sim *h, const char *path)

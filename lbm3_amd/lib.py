@@ -58,6 +58,8 @@ def load():
         "lbm3_section_cut": (i32, [vp, i32, C.c_float, C.POINTER(T.Bounds), vp, i64, C.POINTER(i32)]),
         "lbm3_flow_report": (i32, [vp, i32, C.c_float, C.POINTER(T.Bounds), C.POINTER(f64)]),
         "lbm3_export_vtk": (i32, [vp, C.c_char_p]),
+        "lbm3_save_checkpoint": (i32, [vp, C.c_char_p]),
+        "lbm3_load_checkpoint": (i32, [vp, C.c_char_p]),
         "lbm3_update_tracker": (i32, [vp, vp, i32, vp, i32, vp, i32]),
         "lbm3_build_rotors": (i32, [vp, vp, vp, i32, i32]),
         "lbm3_set_rotor": (i32, [vp, i32, i32, C.c_float, i32, C.c_float, C.c_float, C.c_float]),
@@ -291,6 +293,14 @@ class Sim:
         out = np.zeros((dims[0], dims[1], 6), np.float32)
         check(load().lbm3_section_cut(self.h, pl, float(cut), b, out.ctypes.data, out.size, dims))
         return out
+
+    def save_checkpoint(self, path):
+        """The state a time step changes, to a file (restart with load_checkpoint)."""
+        check(load().lbm3_save_checkpoint(self.h, str(path).encode()))
+
+    def load_checkpoint(self, path):
+        """Restore a save_checkpoint file into a simulation built from the same case and grid."""
+        check(load().lbm3_load_checkpoint(self.h, str(path).encode()))
 
     def export_vtk(self, path):
         """exportVTK: every stored cell of every level to a legacy ASCII VTK file."""

@@ -1,0 +1,146 @@
+"""Checkpoint and restart (lbm3_save_checkpoint / lbm3_load_checkpoint, no
+counterpart in the reference) on the GPU: a simulation restored from a
+checkpoint and stepped on reproduces the saved one bit for bit -- populations
+through the macroscopic fields of every level, open-boundary statistics, wall
+and rotor forces, iteration counters and esotwist parity -- and a checkpoint
+that does not fit the simulation is refused without changing it."""
+import os
+
+import numpy as np
+import pytest
+
+from lbm3_amd import cases, geometry, lib
+
+pytestmark = pytest.mark.gpu
+
+
+def _pipe(levels, res=0.4):
+    tri = geometry.read_binary_stl(os.path.join(lib_golden(), "pipeD20L200STL.STL"))
+    return cases.pipe(tri, length=20.0, res=res, reynolds=100.0, u_mean=0.01, inlet_non_reflective=True,
+                      profile_init=True, levels=levels)
+
+
+def lib_golden():
+    return os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _observe(s, levels, walls=True):
+    out = {}
+    for L in range(levels):
+        out["macros%d" % L] = s.macros(L)
+        I = s.info(L)
+        out["info%d" % L] = (I.iterationsFinished, I.iterationsSinceTrackerReset, I.esotwistFlipper)
+        if walls:
+            out["walls%d" % L] = np.array(s.wall_forces(L))
+    return out
+
+
+def _same(a, b):
+    assert a.keys() == b.keys()
+    for k in a:
+        if isinstance(a[k], np.ndarray):
+            assert np.array_equal(a[k], b[k]), k
+        else:
+            assert a[k] == b[k], k
+
+
+@pytest.mark.parametrize("storage", ["f32", "f16"])
+def test_restart_reproduces_the_pipe_run(tmp_path, storage):
+    """Two-level pipe with a non-reflective inlet: 20 coarse steps, checkpoint
+    with the open boundary condition still pending, 15 more steps; a second
+    simulation built from the same case, restored and stepped 15 times ends
+    identical."""
+    case, dom, tris = _pipe(2)
+    a = lib.Sim(case, storage)
+    a.build(dom, tris)
+    a.init()
+    a.run(20)
+    path = tmp_path / "pipe.ckpt"
+    a.save_checkpoint(path)
+    a.run(15)
+    want = _observe(a, 2)
+    want["open"] = [a.open_boundary_stats(b) for b in range(2)]
+    a.close()
+    b = lib.Sim(case, storage)
+    b.build(dom, tris)
+    b.init()
+    b.run(3)                                   # state the checkpoint replaces
+    b.load_checkpoint(path)
+    b.run(15)
+    got = _observe(b, 2)
+    got["open"] = [b.open_boundary_stats(x) for x in range(2)]
+    b.close()
+    _same(want, got)
+    assert want["info0"][0] == 35 and want["info1"][0] == 70
+
+
+def test_restart_reproduces_the_rotor_duct_f32(tmp_path):
+    """Rotor in a duct (two levels, 4-blade rotor): the rotor angle and its
+    force accumulators come back with the populations."""
+    case, dom, statics, rotor, ri = cases.rotor_duct(48, 24, 24, seed=1, levels=2, n_theta=256, n_axial=8)
+
+    def make():
+        s = lib.Sim(case, "f32")
+        s.build(dom, statics)
+        s.build_rotors([rotor], 3)
+        for L in range(2):
+            s.set_rotor(L, 0, ri["omega"], ri["axis"], ri["origin"])
+        s.init()
+        return s
+
+    a = make()
+    a.run(12)
+    path = tmp_path / "rotor.ckpt"
+    a.save_checkpoint(path)
+    a.run(9)
+    want = _observe(a, 2, walls=False)
+    want["rotor"] = np.concatenate([a.rotor_forces(L, 0) for L in range(2)] + [a.rotor_info(L, 0) for L in range(2)])
+    a.close()
+    b = make()
+    b.load_checkpoint(path)
+    b.run(9)
+    got = _observe(b, 2, walls=False)
+    got["rotor"] = np.concatenate([b.rotor_forces(L, 0) for L in range(2)] + [b.rotor_info(L, 0) for L in range(2)])
+    b.close()
+    _same(want, got)
+
+
+def test_a_checkpoint_that_does_not_fit_is_refused(tmp_path):
+    """Other storage, other grid, a short file, not a checkpoint: each refused
+    with LBM3_E_ARG, the simulation unchanged."""
+    case, dom, tris = _pipe(1)
+    a = lib.Sim(case, "f32")
+    a.build(dom, tris)
+    a.init()
+    a.run(5)
+    path = tmp_path / "one.ckpt"
+    a.save_checkpoint(path)
+    before = a.macros(0)
+    other = lib.Sim(case, "f16")
+    other.build(dom, tris)
+    other.init()
+    with pytest.raises(lib.LBM3Error, match="storage"):
+        other.load_checkpoint(path)
+    other.close()
+    case2, dom2, tris2 = _pipe(1, res=0.5)
+    c = lib.Sim(case2, "f32")
+    c.build(dom2, tris2)
+    c.init()
+    with pytest.raises(lib.LBM3Error, match="grid"):
+        c.load_checkpoint(path)
+    c.close()
+    data = path.read_bytes()
+    short = tmp_path / "short.ckpt"
+    short.write_bytes(data[:-8])
+    a.run(2)
+    mid = a.macros(0)
+    with pytest.raises(lib.LBM3Error, match="size"):
+        a.load_checkpoint(short)
+    assert np.array_equal(a.macros(0), mid)
+    junk = tmp_path / "junk.ckpt"
+    junk.write_bytes(b"\0" * 64)
+    with pytest.raises(lib.LBM3Error, match="not a checkpoint"):
+        a.load_checkpoint(junk)
+    a.load_checkpoint(path)
+    assert np.array_equal(a.macros(0), before)
+    a.close()
