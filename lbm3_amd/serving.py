@@ -7,13 +7,14 @@ time from the first arrival to the last result).
 The mix follows the serving configuration of the hot-path contract:
 Taylor-Green vortices of 32^3 to 128^3, body-force channels of 64x32x32 to
 256x64x64 and spheres in a 128x64x64 to 192x96x96 box, Reynolds numbers and
-500 to 4000 steps drawn from the seed."""
+500 to 4000 steps drawn from the seed.  Each request carries its wire-format
+spec (lbm3_amd.wire), from which it is built."""
 import time
 
 import numpy as np
 
-from . import cases
 from . import lib
+from . import wire
 
 TGV_SIZES = (32, 48, 64, 96, 128)
 CHANNEL_SIZES = ((64, 32, 32), (128, 32, 32), (128, 64, 64), (256, 64, 64))
@@ -45,7 +46,10 @@ def make_requests(n=256, seed=0, steps=(500, 4000), steps_scale=1.0, load=0.9, g
             p = {"dims": tuple(int(v) for v in size), "diameter": d, "reynolds": float(rng.uniform(50.0, 300.0)),
                  "u": float(rng.uniform(0.03, 0.06))}
             cells = size[0] * size[1] * size[2]
-        reqs.append({"id": i, "kind": kind, "params": p, "steps": st, "cells": cells})
+        spec = dict(p, id=str(i), kind=str(kind), steps=st, precision="f32", seed=int(seed))
+        if "dims" in spec:
+            spec["dims"] = list(spec["dims"])
+        reqs.append({"id": i, "kind": kind, "params": p, "steps": st, "cells": cells, "spec": spec})
     work = sum(r["cells"] * r["steps"] for r in reqs)
     mean_gap = work / n / (glups * 1e9) / load
     t = np.cumsum(rng.exponential(mean_gap, n))
@@ -55,17 +59,9 @@ def make_requests(n=256, seed=0, steps=(500, 4000), steps_scale=1.0, load=0.9, g
 
 
 def build(req):
-    """(case, domain, triangles) of a request."""
-    p = req["params"]
-    if req["kind"] == "tgv":
-        return cases.taylor_green(p["n"], u0=0.05, reynolds=p["reynolds"])
-    if req["kind"] == "channel":
-        nx, ny, nz = p["dims"]
-        return cases.channel(nx, ny, nz, force=p["force"], nu=p["nu"])
-    nx, ny, nz = p["dims"]
-    d = p["diameter"]
-    return cases.sphere(nx, ny, nz, diameter=d, center=(nx / 4.0, ny / 2.0, nz / 2.0), u_inlet=p["u"],
-                        reynolds=p["reynolds"], segments=(60, 120))
+    """(case, domain, triangles) of a request, through its wire-format spec
+    (lbm3_amd.wire: checked, then mapped to the case descriptors)."""
+    return wire.to_case(wire.parse_request(req["spec"]))
 
 
 def _summary(tts, updates, span):

@@ -77,3 +77,12 @@ def test_record_and_store(tmp_path):
     assert sm["records"] == 5 and sm["ok"] == 4
     assert sm["p50_s"] == float(np.percentile(times, 50)) and sm["p99_s"] == float(np.percentile(times, 99))
     assert sm["mlups"] == 4e9 / 10.0 / 1e6
+
+
+def test_the_serving_mix_is_wire_format():
+    """Every request of the seeded serving mix carries a spec the wire format
+    accepts, and builds from it."""
+    from lbm3_amd import serving
+    for r in serving.make_requests(n=256, seed=1, steps_scale=0.02):
+        spec = wire.parse_request(json.dumps(r["spec"]))
+        assert spec["kind"] == r["kind"] and spec["steps"] == r["steps"] and wire.cells_of(spec) == r["cells"]
