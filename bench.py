@@ -160,10 +160,10 @@ def traffic_from_profiles(storage):
 
 def counters_workload():
     """A short fixed sequence for rocprofv3 --pmc runs: the sphere workload for
-    10 steps in FP32 and in FP16, then three 1 GiB copies (the calibration
+    10 steps in FP32, FP16 and BF16, then three 1 GiB copies (the calibration
     kernel whose bytes are known)."""
     from lbm3_amd import lib
-    for st in ("f32", "f16"):
+    for st in ("f32", "f16", "bf16"):
         case, dom, tris = workload()
         s = lib.Sim(case, st)
         s.build(dom, tris)
