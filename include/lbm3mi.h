@@ -194,7 +194,8 @@ int lbm3_section_cut(lbm3_sim *s, int plane, float cut, const lbm3_bounds *bound
  * level, open-boundary reference states and statistics, wall and rotor force
  * accumulators, rotor angles -- to `path`; lbm3_load_checkpoint reads it into
  * a simulation built from the same case and grid (checked: storage, level
- * count, cell, wall, open-boundary and rotor counts, grid sizes), after which
+ * count, cell, wall, open-boundary and rotor counts, grid sizes, origin,
+ * resolution, time step and viscosity of every level), after which
  * its steps reproduce the saved simulation's bit for bit.  Any open boundary
  * condition still pending is saved as pending. */
 int lbm3_save_checkpoint(lbm3_sim *s, const char *path);

@@ -1217,7 +1217,8 @@ int lbm3_load_checkpoint(lbm3_sim *h, const char *path)
         const lbm3_info &I = infos[(size_t)L];
         if (d.N != e.N || d.W != e.W || d.nOpen != e.nOpen || d.nRotors != e.nRotors || I.cellCount != l.info.cellCount ||
             I.cellCountX != l.info.cellCountX || I.cellCountY != l.info.cellCountY || I.cellCountZ != l.info.cellCountZ ||
-            I.gridID != l.info.gridID || I.res != l.info.res)
+            I.gridID != l.info.gridID || I.res != l.info.res || I.ox != l.info.ox || I.oy != l.info.oy || I.oz != l.info.oz ||
+            I.dtPhys != l.info.dtPhys || I.nu != l.info.nu)
             return fail("grid differs from the simulation's");
         at[(size_t)L] = ftell(fp);
         long skip = 0;
@@ -1241,7 +1242,10 @@ int lbm3_load_checkpoint(lbm3_sim *h, const char *path)
         }
         for (auto &r : l.rotors)
             if (fread(&r.radiansElapsed, sizeof r.radiansElapsed, 1, fp) != 1) return fail("truncated rotor state");
-        l.info = infos[(size_t)L];
+        // the step state of the level's info; the rest comes from the case it was built from
+        l.info.iterationsFinished = infos[(size_t)L].iterationsFinished;
+        l.info.iterationsSinceTrackerReset = infos[(size_t)L].iterationsSinceTrackerReset;
+        l.info.esotwistFlipper = infos[(size_t)L].esotwistFlipper;
         l.openSettled = settled[(size_t)L] != 0;
     }
     fclose(fp);

@@ -129,6 +129,15 @@ def test_a_checkpoint_that_does_not_fit_is_refused(tmp_path):
     with pytest.raises(lib.LBM3Error, match="grid"):
         c.load_checkpoint(path)
     c.close()
+    case3, dom3, tris3 = cases.pipe(geometry.read_binary_stl(os.path.join(lib_golden(), "pipeD20L200STL.STL")),
+                                    length=20.0, res=0.4, reynolds=50.0, u_mean=0.01, inlet_non_reflective=True,
+                                    profile_init=True, levels=1)
+    v = lib.Sim(case3, "f32")                  # same grid, another viscosity
+    v.build(dom3, tris3)
+    v.init()
+    with pytest.raises(lib.LBM3Error, match="grid"):
+        v.load_checkpoint(path)
+    v.close()
     data = path.read_bytes()
     short = tmp_path / "short.ckpt"
     short.write_bytes(data[:-8])
