@@ -100,6 +100,34 @@ def measure_multilevel(storage, steps=200, warmup=20):
     return out
 
 
+def measure_graph(storage, n=48, steps=400, warmup=20):
+    """Launch-bound case: two-level periodic Taylor-Green vortex (n^3 coarse,
+    the inner half refined), whole coarse steps launched one by one (lbm3_run)
+    and replayed from a captured HIP graph (lbm3_run_graph); wall time around
+    each, synchronized."""
+    from lbm3_amd import cases, lib
+    case, dom, tris = cases.taylor_green(n, levels=2)
+    sim = lib.Sim(case, storage)
+    fu = sim.build(dom, tris)
+    sim.init()
+    sim.run(warmup)
+    sim.sync()
+    t0 = time.perf_counter()
+    sim.run(steps)
+    sim.sync()
+    eager = time.perf_counter() - t0
+    sim.run_graph(2)
+    t0 = time.perf_counter()
+    sim.run_graph(steps)
+    graph = time.perf_counter() - t0
+    out = {"case": "taylor_green %d^3, 2 levels" % n, "storage": storage, "fluid_updates_per_coarse_step": fu,
+           "eager_ms_per_coarse_step": round(eager / steps * 1e3, 4), "graph_ms_per_coarse_step": round(graph / steps * 1e3, 4),
+           "eager_mlups": round(fu * steps / eager / 1e6, 1), "graph_mlups": round(fu * steps / graph / 1e6, 1),
+           "speedup": round(eager / graph, 3)}
+    sim.close()
+    return out
+
+
 def measure_rotor(storage, windows=2, window=100, warmup=20):
     """Rotor in a duct (coarse 192x96x96, two levels, seeded N-blade rotor of
     ~200k triangles voxelized on the GPU, tip speed 0.08): MLUPS of whole
@@ -255,7 +283,8 @@ def main():
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_threads)
     other = None
     if not args.no_other_cases:
-        other = {"multilevel_sphere": measure_multilevel("f32"), "rotor_duct": measure_rotor("f32")}
+        other = {"multilevel_sphere": measure_multilevel("f32"), "rotor_duct": measure_rotor("f32"),
+                 "graph_replay": measure_graph("f32")}
         print("other cases: %s" % other, file=sys.stderr, flush=True)
     serve = None
     if args.serving == "quick":

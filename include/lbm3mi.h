@@ -91,6 +91,15 @@ int lbm3_update_single_grid(lbm3_sim *s, int level);
 int lbm3_update_all_grids(lbm3_sim *s, int level);
 /* `steps` coarse time steps of updateAllGrids(grids, 0). */
 int lbm3_run(lbm3_sim *s, int steps);
+/* lbm3_run through a HIP graph (no counterpart in the reference): the launch
+ * sequence of two coarse steps (every level's esotwist parity back where it
+ * was) is captured once per call and replayed steps / 2 times, an odd last
+ * step launched directly; returns when the work is done.  Results are those of
+ * lbm3_run bit for bit.  For simulations without open boundaries or rotors
+ * (their kernels take per-step arguments: LBM3_E_STATE otherwise), e.g.
+ * periodic boxes, channels and multilevel Taylor-Green grids, where many
+ * short launches per coarse step make launch overhead count. */
+int lbm3_run_graph(lbm3_sim *s, int steps);
 int lbm3_synchronize(lbm3_sim *s);
 
 /* Timed run: `steps` coarse steps between HIP events on the sim's stream.

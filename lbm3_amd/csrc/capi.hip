@@ -744,6 +744,55 @@ int lbm3_run(lbm3_sim *h, int steps)
     return check_hip(hipGetLastError(), "run launch");
 }
 
+int lbm3_run_graph(lbm3_sim *h, int steps)
+{
+    if (!h || steps < 0) return set_error(LBM3_E_ARG, "bad run arguments");
+    Sim &s = h->s;
+    if (const int rc = check_run(s, 0)) return rc;
+    for (int L = 0; L < s.levels; ++L)
+        if (s.L[L].nOpen > 0 || !s.L[L].rotors.empty())
+            return set_error(LBM3_E_STATE, "graph replay needs levels without open boundaries or rotors "
+                                           "(their kernels take arguments that change every step)");
+    if (s.timing) return set_error(LBM3_E_STATE, "graph replay during a timed run");
+    if (steps >= 2) {
+        // two coarse steps bring every level's esotwist parity back, so one
+        // captured pair replays with the same kernel arguments
+        lbm3_info before[LBM3_MAX_LEVELS];
+        for (int L = 0; L < s.levels; ++L) before[L] = s.L[L].info;
+        hipGraph_t graph = nullptr;
+        HIPCHK(hipStreamBeginCapture(s.stream, hipStreamCaptureModeThreadLocal));
+        update_all(s, 0);
+        update_all(s, 0);
+        const hipError_t ec = hipStreamEndCapture(s.stream, &graph);
+        int dIter[LBM3_MAX_LEVELS], dReset[LBM3_MAX_LEVELS];
+        for (int L = 0; L < s.levels; ++L) {
+            dIter[L] = s.L[L].info.iterationsFinished - before[L].iterationsFinished;
+            dReset[L] = s.L[L].info.iterationsSinceTrackerReset - before[L].iterationsSinceTrackerReset;
+            s.L[L].info = before[L];          // nothing ran yet
+        }
+        if (ec != hipSuccess) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return check_hip(ec, "step capture");
+        }
+        hipGraphExec_t exec = nullptr;
+        hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        for (int k = 0; e == hipSuccess && k < steps / 2; ++k) {
+            e = hipGraphLaunch(exec, s.stream);
+            for (int L = 0; L < s.levels && e == hipSuccess; ++L) {
+                s.L[L].info.iterationsFinished += dIter[L];
+                s.L[L].info.iterationsSinceTrackerReset += dReset[L];
+            }
+        }
+        const hipError_t es = hipStreamSynchronize(s.stream);   // the graph is released after its last replay
+        if (exec) (void)hipGraphExecDestroy(exec);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) return check_hip(e, "step graph replay");
+        if (es != hipSuccess) return check_hip(es, "step graph replay");
+    }
+    if (steps % 2) update_all(s, 0);
+    return check_hip(hipGetLastError(), "graph run");
+}
+
 int lbm3_synchronize(lbm3_sim *h)
 {
     if (!h) return set_error(LBM3_E_ARG, "null sim");

@@ -58,6 +58,7 @@ def load():
         "lbm3_section_cut": (i32, [vp, i32, C.c_float, C.POINTER(T.Bounds), vp, i64, C.POINTER(i32)]),
         "lbm3_flow_report": (i32, [vp, i32, C.c_float, C.POINTER(T.Bounds), C.POINTER(f64)]),
         "lbm3_export_vtk": (i32, [vp, C.c_char_p]),
+        "lbm3_run_graph": (i32, [vp, i32]),
         "lbm3_save_checkpoint": (i32, [vp, C.c_char_p]),
         "lbm3_load_checkpoint": (i32, [vp, C.c_char_p]),
         "lbm3_update_tracker": (i32, [vp, vp, i32, vp, i32, vp, i32]),
@@ -293,6 +294,11 @@ class Sim:
         out = np.zeros((dims[0], dims[1], 6), np.float32)
         check(load().lbm3_section_cut(self.h, pl, float(cut), b, out.ctypes.data, out.size, dims))
         return out
+
+    def run_graph(self, steps):
+        """run(steps) through a captured HIP graph of two coarse steps (no open
+        boundaries or rotors); returns when the steps are done."""
+        check(load().lbm3_run_graph(self.h, int(steps)))
 
     def save_checkpoint(self, path):
         """The state a time step changes, to a file (restart with load_checkpoint)."""
