@@ -160,16 +160,21 @@ def measure_rotor(storage, windows=2, window=100, warmup=20):
     return out
 
 
-def cpu_baseline(threads, steps=2):
-    """The CPU oracle (C, OpenMP over cells) on the same sphere workload, `steps` time steps."""
+def cpu_baseline(threads, seconds=10.0, batch=10, max_steps=400):
+    """The CPU oracle (C, OpenMP over cells) on the same sphere workload:
+    batches of `batch` time steps until `seconds` of timed work (at most
+    `max_steps` steps)."""
     os.environ.setdefault("OMP_NUM_THREADS", str(threads))
     from tests.oracle_lib import OracleSim
     case, dom, tris = workload()
     o = OracleSim(case, dom, tris)
     o.init()
     o.run(1)
+    steps = 0
     t0 = time.time()
-    o.run(steps)
+    while steps < max_steps and (steps == 0 or time.time() - t0 < seconds):
+        o.run(batch)
+        steps += batch
     dt = time.time() - t0
     return {"value": round(o.fluid_updates * steps / dt / 1e6, 3), "unit": "MLUPS", "cores": threads, "kind": "port",
             "sample": "oracle (C restatement, OpenMP over cells) on the full 256x128x128 sphere grid, %d time steps "
