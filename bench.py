@@ -242,9 +242,17 @@ def main():
                     help="skip the multilevel-sphere and rotor-duct MLUPS lines")
     ap.add_argument("--counters-workload", action="store_true",
                     help="run only the short sequence that the HBM byte counters are collected on")
+    ap.add_argument("--case", choices=["multilevel", "rotor", "graph"],
+                    help="measure only this configuration (FP32) and print its JSON object")
     args = ap.parse_args()
     if args.counters_workload:
         counters_workload()
+        return
+    if args.case:
+        run = {"multilevel": lambda: measure_multilevel("f32", steps=args.steps, warmup=args.warmup),
+               "rotor": lambda: measure_rotor("f32", warmup=args.warmup),
+               "graph": lambda: measure_graph("f32", steps=args.steps, warmup=args.warmup)}[args.case]
+        print(json.dumps(run()), flush=True)
         return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (requested --gpus %d)" % args.gpus,
