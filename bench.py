@@ -181,14 +181,15 @@ def cpu_baseline(threads, seconds=10.0, batch=10, max_steps=400):
                       "after 1 untimed step, %.1f s" % (steps, dt)}
 
 
-def traffic_from_profiles(storage):
+def traffic_from_profiles(storage, bulk_cells):
+    """Measured HBM bytes per bulk launch, if the counters were taken on the same cell split."""
     path = os.path.join(ROOT, "profiles", "hbm_traffic_sphere.json")
     if not os.path.exists(path):
         return None
     with open(path) as fh:
         d = json.load(fh)
     e = d.get(storage)
-    return None if e is None else e.get("bytes_per_launch")
+    return None if e is None or e.get("cells") != bulk_cells else e.get("bytes_per_launch")
 
 
 def counters_workload():
@@ -203,8 +204,10 @@ def counters_workload():
         s.init()
         s.run(10)
         s.sync()
+        k = s.cell_kinds(0)
         s.close()
-        print("counters workload: %s sphere, 10 steps" % st, flush=True)
+        print("counters workload: %s sphere, 10 steps, cells bulk %d open %d open_group %d"
+              % (st, k["bulk"], k["open"], k["open_group"]), flush=True)
     lib.copy_probe(1 << 30, 3)
     print("counters workload: 3 copies of 1 GiB", flush=True)
 
@@ -292,7 +295,7 @@ def main():
     copy_ms = lib.copy_probe(copy_bytes, 20)
     # measured HBM bytes of the K15 bulk kernel (the K17 variant has no counter run of its own)
     for run in [main_run] + [v for k, v in variants.items() if k != "k17"]:
-        run["roofline"]["traffic"] = traffic_from_profiles(run["storage"])
+        run["roofline"]["traffic"] = traffic_from_profiles(run["storage"], run["bulk_cells"])
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_threads)
     other = None
     if not args.no_other_cases:

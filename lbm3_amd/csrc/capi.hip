@@ -210,9 +210,10 @@ static int prepare_open_fusion(Sim &s, int L)
         HIPCHK(hipMemcpyAsync(cell.data(), l.openCell.p, sizeof(int) * cell.size(), hipMemcpyDeviceToHost, s.stream));
         HIPCHK(hipMemcpyAsync(meta.data(), l.openMeta.p, sizeof(uint32_t) * meta.size(), hipMemcpyDeviceToHost, s.stream));
         HIPCHK(hipStreamSynchronize(s.stream));
-        std::vector<std::pair<int, int>> first((size_t)(l.N / LBM3_OPEN_GROUP + 1), {-1, 0});   // {first open cell, key}
+        const int G = open_group_cells((int)storage_bytes(s.storage));
+        std::vector<std::pair<int, int>> first((size_t)(l.N / G + 1), {-1, 0});   // {first open cell, key}
         for (int x = 0; x < l.nOpen; ++x) {
-            const int g = cell[(size_t)x] / LBM3_OPEN_GROUP;
+            const int g = cell[(size_t)x] / G;
             if (first[(size_t)g].first < 0 || cell[(size_t)x] < first[(size_t)g].first)
                 first[(size_t)g] = {cell[(size_t)x], int(meta[(size_t)x] & 0x1FFFu)};
         }
@@ -224,7 +225,7 @@ static int prepare_open_fusion(Sim &s, int L)
         for (const auto &o : order) groups.push_back(o.second);
         l.nOpenGroups = (int)groups.size();
         if (upload(l.openGroups, groups.data(), (long long)groups.size(), s.stream)) return LBM3_E_HIP;
-        launch_mark_groups(l.openGroups.p, l.nOpenGroups, LBM3_OPEN_GROUP, l.N, l.cellRow.p, s.stream);
+        launch_mark_groups(l.openGroups.p, l.nOpenGroups, G, l.N, l.cellRow.p, s.stream);
         HIPCHK(hipGetLastError());
     }
     if (cand) {

@@ -1009,7 +1009,7 @@ template <class S>
 void launch_open_cells(const StepArgs<S> &a, bool flip, hipStream_t s)
 {
     if (a.open.count <= 0) return;
-    constexpr int G = LBM3_OPEN_GROUP;
+    constexpr int G = open_group_cells(int(sizeof(S)));
     const dim3 g((unsigned)((a.open.count + 256 / G - 1) / (256 / G))), b(256);
     const bool k17 = relax_of<typename StoreTraits<S>::compute>(a.bulk).k17 != 0;
     if (flip) { if (k17) hipLaunchKernelGGL((k_open_groups<S, true, 1, G>), g, b, 0, s, a); else hipLaunchKernelGGL((k_open_groups<S, true, 2, G>), g, b, 0, s, a); }
