@@ -24,9 +24,10 @@ extern "C" {
 #define LBM3_OK 0
 #define LBM3_E_ARG -1      /* invalid argument */
 #define LBM3_E_HIP -2      /* HIP runtime error */
-#define LBM3_E_BUILD -3    /* grid builder failed (missing periodic/open-BC source, empty STL) */
+#define LBM3_E_BUILD -3    /* grid builder failed (missing periodic/open-BC source, empty or open STL) */
 #define LBM3_E_STATE -4    /* call out of order (e.g. step before build) */
 #define LBM3_E_MEMORY -5   /* device allocation failed or exceeds the budget */
+#define LBM3_E_DIVERGED -6 /* the watchdog found non-finite populations (lbm3_set_watchdog) */
 
 typedef struct lbm3_sim lbm3_sim;
 
@@ -73,12 +74,24 @@ int lbm3_sim_destroy(lbm3_sim *s);
 /* buildGrids (gridBuilderFunctions.h:1577): voxelize the static STL bodies
  * (triangles as 9 floats ax,ay,az,bx,...,cz; triCounts[b] triangles per body),
  * build every level on the GPU and allocate the populations.  Returns the
- * reference's fluidUpdatesPerIteration (sum over levels of (cells - walls)*2^L). */
+ * reference's fluidUpdatesPerIteration (sum over levels of (cells - walls)*2^L).
+ * Each body first passes readSTL's checks (STLFunctions.h:317-462): triangles
+ * with a repeated vertex are dropped, and a body with an edge not shared by
+ * exactly two triangles (checkSTLEdges, :130-315) is refused with LBM3_E_BUILD;
+ * the counts stay readable as "stl<b>EdgeCount" (lbm3_download_array). */
 int lbm3_build_grids(lbm3_sim *s, const float *triangles, const int *triCounts, int nBodies, lbm3_bounds domain,
                      long long *fluidUpdatesPerIteration);
 
 /* Upload a level built elsewhere (same layout as gridBuilderToGrid output). */
 int lbm3_import_level(lbm3_sim *s, int level, const lbm3_level_arrays *a);
+
+/* The checks lbm3_import_level makes before any device call, on the host
+ * only: sizes, null arrays with nonzero counts, rows numbered in the order of
+ * their first cells and holding consecutive cells, every cell's j+/k+/jk+
+ * neighbour (row neighbour + offset in the row, cellFunctions.h:41-46) inside
+ * the level, wall, open-boundary, periodic and interface cells in range.
+ * LBM3_E_ARG with a message naming the first failed check. */
+int lbm3_validate_level(const lbm3_level_arrays *a);
 
 /* applyInitialCondition (boundaryConditions/applyInitialCondition.h:8) on one level. */
 int lbm3_apply_initial_condition(lbm3_sim *s, int level);
@@ -91,6 +104,14 @@ int lbm3_update_single_grid(lbm3_sim *s, int level);
 int lbm3_update_all_grids(lbm3_sim *s, int level);
 /* `steps` coarse time steps of updateAllGrids(grids, 0). */
 int lbm3_run(lbm3_sim *s, int steps);
+/* Divergence watchdog of lbm3_run (no counterpart in the reference, which has
+ * no in-loop check): with period > 0, lbm3_run sums the mass over every level
+ * (lbm3_mass) after each coarse step whose count is a multiple of `period`
+ * and after its last step, and returns LBM3_E_DIVERGED, naming the step, when
+ * the sum is not finite (a NaN or infinite population in a counted cell).
+ * Each check reads the populations once and waits for the device.  0 (the
+ * default) turns it off. */
+int lbm3_set_watchdog(lbm3_sim *s, int period);
 /* lbm3_run through a HIP graph (no counterpart in the reference): the launch
  * sequence of two coarse steps (every level's esotwist parity back where it
  * was) is captured once per call and replayed steps / 2 times, an odd last
@@ -122,7 +143,9 @@ int lbm3_download_populations(lbm3_sim *s, int level, void *out, long long bytes
 int lbm3_upload_populations(lbm3_sim *s, int level, const void *in, long long bytes);
 /* Named built arrays (shifter, rowI.., wallMap, wallIndex, wallData, linkLength,
  * periodicIndex, periodicSource, openIndex, openSource, openMeta, rayHitScan,
- * rayMap, f2cIndex, ..., rotor<r>Map/Index/Interp/Force).  "cellRow" is the per-step kernels' cell record, 8
+ * rayMap, f2cIndex, ..., rotor<r>Map/Index/Interp/Force, stl<b>EdgeCount: the
+ * shared-edge counts AB, BC, CA of every kept triangle of static body b of the
+ * last build, any level).  "cellRow" is the per-step kernels' cell record, 8
  * bytes per cell: {kind, row} with kind >= 0 wall-adjacent entry, -1/-2 bulk,
  * -3 solid, -4 a cell the rotors can reach, <= -8 open cell (the last two
  * updated by kernels of their own).  With out == NULL only
@@ -159,6 +182,12 @@ int lbm3_rotor_forces(lbm3_sim *s, int level, int rotor, double *out6);
 /* kinetic energy sum over counted cells (taylorGreenVortexMain.cu:116-180, lattice units,
  * level L weighted by 8^-L) */
 int lbm3_kinetic_energy(lbm3_sim *s, double *out);
+/* Global mass balance in FP64 (no counterpart in the reference): over the same
+ * counted cells as the kinetic energy, level L weighted by 8^-L,
+ * out2[0] = sum of dRho and out2[1] = the number of counted cells, so the
+ * mass in lattice units (rho0 = 1, one coarse cell = 1) is out2[1] + out2[0].
+ * On a periodic box out2[0] stays constant to rounding. */
+int lbm3_mass(lbm3_sim *s, double *out2);
 /* per-body sums of the accumulated momentum-exchange force and its torque about
  * `origin` (trackerFunctions.h:356-474, FP64): out[body*6 + {Fx,Fy,Fz,Tx,Ty,Tz}] */
 int lbm3_wall_forces(lbm3_sim *s, int level, const double *origin, int nBodies, double *out);

@@ -117,6 +117,7 @@ struct DevSTL {
     float binSize = 0, oxBin = 0, oyBin = 0, ozBin = 0;
     int bx = 0, by = 0, bz = 0;
     DBuf<int> firstInBin, binArray;
+    std::vector<int> edges;    // shared-edge counts, 3 per kept triangle (AB, BC, CA)
 };
 
 struct RayMap {
@@ -180,6 +181,69 @@ void init_info(const lbm3_case &C, std::vector<BLevel> &G, lbm3_bounds B)
         }
         I.nu = (I.dtPhys * C.nuPhys) / ((I.res / 1000.f) * (I.res / 1000.f));
     }
+}
+
+// ---------------------------------------------------------------------------
+// closed-mesh check (checkSTLEdges, STLFunctions.h:130-315): every edge of a
+// closed STL is shared by exactly two triangles.  One lane per triangle counts,
+// for its edges AB and BC, the triangles of the bin holding its vertex B that
+// have the same edge in either orientation (itself included), and for CA those
+// of the bin holding A; the binning puts every triangle into each bin its
+// widened bounding box overlaps, so a neighbour sharing an edge is in both
+// bins.  The reference prints "Check OK" whatever the counts (its error flag
+// array is never set, STLFunctions.h:161-162,289) although its message (:314)
+// says the build stops; here an open STL is refused with LBM3_E_BUILD.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool same_point(const float *a, const float *b)
+{
+    return a[0] == b[0] && a[1] == b[1] && a[2] == b[2];
+}
+
+int stl_edges(DevSTL &S, hipStream_t s)
+{
+    DBuf<int> ec;
+    BCHK(ec.alloc(3LL * S.T));
+    const float *v0 = S.v[0].p, *v1 = S.v[1].p, *v2 = S.v[2].p, *v3 = S.v[3].p, *v4 = S.v[4].p, *v5 = S.v[5].p;
+    const float *v6 = S.v[6].p, *v7 = S.v[7].p, *v8 = S.v[8].p;
+    const int *first = S.firstInBin.p, *bins = S.binArray.p;
+    const float ox = S.oxBin, oy = S.oyBin, oz = S.ozBin, bs = S.binSize;
+    const int BX = S.bx, BY = S.by;
+    int *out = ec.p;
+    for_n(S.T, s, [=] __device__(long long t) {
+        const float t1[9] = {v0[t], v1[t], v2[t], v3[t], v4[t], v5[t], v6[t], v7[t], v8[t]};
+        int cnt[3] = {0, 0, 0};
+        for (int side = 0; side < 2; ++side) {
+            const float *p = side == 0 ? t1 + 3 : t1;      // vertex B, then vertex A
+            const int ib = (int)((p[0] - ox) / bs), jb = (int)((p[1] - oy) / bs), kb = (int)((p[2] - oz) / bs);
+            const long long bin = ((long long)kb * BY + jb) * BX + ib;
+            const int e0 = side == 0 ? 0 : 2, e1 = side == 0 ? 2 : 3;
+            for (int r = first[bin]; r < first[bin + 1]; ++r) {
+                const int u = bins[r];
+                const float t2[9] = {v0[u], v1[u], v2[u], v3[u], v4[u], v5[u], v6[u], v7[u], v8[u]};
+                for (int e = e0; e < e1; ++e) {
+                    const float *a1 = t1 + 3 * e, *b1 = t1 + 3 * ((e + 1) % 3);
+                    for (int f = 0; f < 3; ++f) {
+                        const float *a2 = t2 + 3 * f, *b2 = t2 + 3 * ((f + 1) % 3);
+                        cnt[e] += (same_point(a1, a2) && same_point(b1, b2)) ? 1 : 0;
+                        cnt[e] += (same_point(a1, b2) && same_point(b1, a2)) ? 1 : 0;
+                    }
+                }
+            }
+        }
+        out[3 * t] = cnt[0];
+        out[3 * t + 1] = cnt[1];
+        out[3 * t + 2] = cnt[2];
+    });
+    BCHK(hipGetLastError());
+    S.edges.assign(3 * (size_t)S.T, 0);
+    BCHK(hipMemcpyAsync(S.edges.data(), ec.p, sizeof(int) * S.edges.size(), hipMemcpyDeviceToHost, s));
+    BCHK(hipStreamSynchronize(s));
+    long long faulty = 0;
+    for (const int c : S.edges) faulty += c != 2;
+    if (faulty)
+        return set_error(LBM3_E_BUILD, "checkSTLEdges: " + std::to_string(faulty) +
+                                           " edges are not shared by exactly two triangles; the STL is not closed");
+    return 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -271,7 +335,8 @@ int stl_upload(DevSTL &S, const float *tri, int n, float RES, hipStream_t s)
             BCHK(S.binArray.alloc(total));
         }
     }
-    return check_hip(hipGetLastError(), "STL bins");
+    BCHK(hipGetLastError());
+    return stl_edges(S, s);
 }
 
 // ---------------------------------------------------------------------------
@@ -1339,8 +1404,11 @@ int gpu_build(Sim &sim, const float *tri, const int *triCounts, int nBodies, lbm
     std::vector<DevSTL> stl(nBodies);
     {
         long long off = 0;
+        sim.stlEdges.assign((size_t)nBodies, {});
         for (int b = 0; b < nBodies; ++b) {
-            if (const int rc = stl_upload(stl[b], tri + 9 * off, triCounts[b], C.resGlobal, s)) return rc;
+            const int rc = stl_upload(stl[b], tri + 9 * off, triCounts[b], C.resGlobal, s);
+            sim.stlEdges[(size_t)b] = stl[b].edges;       // kept also when the check refuses the body
+            if (rc) return rc;
             off += triCounts[b];
         }
     }

@@ -264,10 +264,15 @@ template <class S> static void settle_typed(Sim &s, Level &l)
     launch_open<S>(open_args<S>(s, l, l.info.iterationsFinished - 1), l.info.esotwistFlipper != 0, s.stream);
 }
 
+// A condition is pending after a fused step and after a checkpoint load that
+// saved one; it is applied here whether or not the level currently fuses.
 static int settle_open(Sim &s, int L)
 {
     Level &l = s.L[L];
-    if (!l.fusedOpen || l.openSettled) return 0;
+    if (l.openSettled || l.nOpen == 0) {
+        l.openSettled = true;
+        return 0;
+    }
     switch (s.storage) {
     case LBM3_STORE_F16: settle_typed<__half>(s, l); break;
     case LBM3_STORE_BF16: settle_typed<__hip_bfloat16>(s, l); break;
@@ -574,24 +579,36 @@ static int validate_level(const lbm3_level_arrays &a)
     if (!a.shifter || !a.rowI || !a.rowJ || !a.rowK || !a.rowJPlus || !a.rowKPlus || !a.rowJKPlus || !a.wallMap)
         return bad("missing topology array");
     if (W > 0 && (!a.wallIndex || !a.wallData || !a.linkLength)) return bad("missing wall arrays");
-    long long nextRow = 0;      // rows are numbered in the order of their first cells (the bulk kernel relies on it)
+    if (a.nOpenIDs > 0 && !a.openCount) return bad("missing open-boundary counts");
+    if (a.nPeriodic > 0 && (!a.periodicIndex || !a.periodicSource)) return bad("missing periodic arrays");
+    // rows are numbered in the order of their first cells (the bulk kernel relies
+    // on it) and hold consecutive cells: a cell with a negative shifter is at that
+    // offset from the last row head before it (gridBuilderFunctions.h:1107-1125)
+    long long nextRow = 0, head = -1;
+    std::vector<long long> rowLen((size_t)R, 0);
     for (long long c = 0; c < N; ++c) {
         const int sh = a.shifter[c];
         if (sh >= 0) {
             if (sh >= R) return bad("shifter row out of range");
             if (sh != nextRow) return bad("rows not numbered in the order of their first cells");
             ++nextRow;
-        } else if (c + sh < 0 || a.shifter[c + sh] < 0 || a.shifter[c + sh] >= R) {
-            return bad("shifter offset does not reach a row head");
+            head = c;
+        } else if (head < 0 || c + sh != head) {
+            return bad("shifter offset does not reach its row head");
         }
+        ++rowLen[(size_t)a.shifter[head]];
         const int wm = a.wallMap[c];
         if (wm < -3 || wm >= W) return bad("wallMap out of range");
     }
     if (nextRow != R) return bad("a row without a first cell");
-    for (long long r = 0; r < R; ++r)
-        if (a.rowJPlus[r] < 0 || a.rowJPlus[r] >= N || a.rowKPlus[r] < 0 || a.rowKPlus[r] >= N || a.rowJKPlus[r] < 0 ||
-            a.rowJKPlus[r] >= N)
+    // the decode adds a cell's offset in its row to the neighbour rows' first
+    // cells (cellFunctions.h:41-46): every cell of the row must land inside the level
+    for (long long r = 0; r < R; ++r) {
+        const long long last = rowLen[(size_t)r] - 1;
+        if (a.rowJPlus[r] < 0 || a.rowJPlus[r] + last >= N || a.rowKPlus[r] < 0 || a.rowKPlus[r] + last >= N ||
+            a.rowJKPlus[r] < 0 || a.rowJKPlus[r] + last >= N)
             return bad("row neighbour out of range");
+    }
     for (long long w = 0; w < W; ++w)
         if (a.wallIndex[w] < 0 || a.wallIndex[w] >= N || a.wallMap[a.wallIndex[w]] != w) return bad("wallIndex inconsistent with wallMap");
     long long nOpen = 0;
@@ -599,6 +616,7 @@ static int validate_level(const lbm3_level_arrays &a)
         if (a.openCount[b] < 0) return bad("negative open count");
         nOpen += a.openCount[b];
     }
+    if (nOpen > 0 && (!a.openIndex || !a.openSource)) return bad("missing open-boundary cell lists");
     for (long long x = 0; x < nOpen; ++x)
         if (a.openIndex[x] < 0 || a.openIndex[x] >= N || a.openSource[x] < 0 || a.openSource[x] >= N) return bad("open cell out of range");
     for (long long x = 0; x < a.nPeriodic; ++x)
@@ -618,6 +636,12 @@ static int validate_level(const lbm3_level_arrays &a)
             !signedCell(a.leftoverNbrJ[x]) || !signedCell(a.leftoverNbrK[x]))
             return bad("leftover cell out of range");
     return 0;
+}
+
+int lbm3_validate_level(const lbm3_level_arrays *a)
+{
+    if (!a) return set_error(LBM3_E_ARG, "null argument");
+    return validate_level(*a);
 }
 
 // fine-level cells named by a coarse level's interface lists must exist
@@ -737,12 +761,28 @@ int lbm3_update_all_grids(lbm3_sim *h, int level)
     return check_hip(hipGetLastError(), "updateAllGrids launch");
 }
 
+static int finite_state(Sim &s);
+static int mass_sums(Sim &s, double *out2);
+
 int lbm3_run(lbm3_sim *h, int steps)
 {
     if (!h || steps < 0) return set_error(LBM3_E_ARG, "bad run arguments");
-    if (const int rc = check_run(h->s, 0)) return rc;
-    for (int t = 0; t < steps; ++t) update_all(h->s, 0);
+    Sim &s = h->s;
+    if (const int rc = check_run(s, 0)) return rc;
+    for (int t = 0; t < steps; ++t) {
+        update_all(s, 0);
+        // divergence watchdog: every `watchdog` coarse steps and after the last one
+        if (s.watchdog > 0 && ((s.L[0].info.iterationsFinished % s.watchdog) == 0 || t == steps - 1))
+            if (const int rc = finite_state(s)) return rc;
+    }
     return check_hip(hipGetLastError(), "run launch");
+}
+
+int lbm3_set_watchdog(lbm3_sim *h, int period)
+{
+    if (!h || period < 0) return set_error(LBM3_E_ARG, "bad watchdog period");
+    h->s.watchdog = period;
+    return 0;
 }
 
 int lbm3_run_graph(lbm3_sim *h, int steps)
@@ -946,6 +986,17 @@ int lbm3_download_array(lbm3_sim *h, int level, const char *name, void *out, lon
         else if (!strcmp(f, "Interp")) { src = R.interp.p; n = 64LL * R.dev.nonZero; }
         else if (!strcmp(f, "Force")) { src = R.g.p; n = 81LL * R.dev.nonZero; es = 8; }
         else return set_error(LBM3_E_ARG, "unknown rotor array");
+    } else if (!strncmp(name, "stl", 3) && name[3] >= '0' && name[3] <= '9' && !strcmp(name + 4, "EdgeCount")) {
+        // shared-edge counts of a static body of the last build (host copy; any level)
+        const size_t b = (size_t)(name[3] - '0');
+        if (b >= s.stlEdges.size()) return set_error(LBM3_E_ARG, "no such STL body");
+        const std::vector<int> &e = s.stlEdges[b];
+        *count = (long long)e.size();
+        if (out) {
+            if (capacity < (long long)e.size()) return set_error(LBM3_E_ARG, "buffer too small");
+            memcpy(out, e.data(), sizeof(int) * e.size());
+        }
+        return 0;
     } else if (!strcmp(name, "openRefs")) {
         if (l.built)
             if (const int rc = settle_open(s, level)) return rc;
@@ -988,6 +1039,50 @@ int lbm3_kinetic_energy(lbm3_sim *h, double *out)
         total += v / (double)(1LL << (3 * level));
     }
     *out = total;
+    return 0;
+}
+
+// the mass sum over every level is finite exactly when no counted cell holds
+// a NaN or infinite population
+static int finite_state(Sim &s)
+{
+    double m[2] = {0.0, 0.0};
+    if (const int rc = mass_sums(s, m)) return rc;
+    if (!std::isfinite(m[0]))
+        return set_error(LBM3_E_DIVERGED, "non-finite populations after coarse step " +
+                                              std::to_string(s.L[0].info.iterationsFinished));
+    return 0;
+}
+
+int lbm3_mass(lbm3_sim *h, double *out2)
+{
+    if (!h || !out2) return set_error(LBM3_E_ARG, "null argument");
+    return mass_sums(h->s, out2);
+}
+
+static int mass_sums(Sim &s, double *out2)
+{
+    for (int level = 0; level < s.levels; ++level) {
+        if (const int rc = check_level(s, level)) return rc;
+        if (const int rc = settle_open(s, level)) return rc;
+        Level &l = s.L[level];
+        const long long nb = (l.N + 255) / 256;
+        if (s.partial.n < 2 * nb) HIPCHK(s.partial.alloc(2 * nb));
+        if (!s.scalar.p) HIPCHK(s.scalar.alloc(8));
+        const bool flip = l.info.esotwistFlipper != 0;
+        const Topology t = topo_of(l);
+        const double w = (double)(1LL << (3 * level));
+        const bool acc = level > 0;
+        switch (s.storage) {
+        case LBM3_STORE_F16: launch_mass<__half>(reinterpret_cast<__half *>(l.f.p), t, l.wallMap.p, l.wallData.p, s.partial.p, s.scalar.p, s.codec, flip, s.stream, w, acc); break;
+        case LBM3_STORE_BF16: launch_mass<__hip_bfloat16>(reinterpret_cast<__hip_bfloat16 *>(l.f.p), t, l.wallMap.p, l.wallData.p, s.partial.p, s.scalar.p, s.codec, flip, s.stream, w, acc); break;
+        case LBM3_STORE_F64: launch_mass<double>(reinterpret_cast<double *>(l.f.p), t, l.wallMap.p, l.wallData.p, s.partial.p, s.scalar.p, s.codec, flip, s.stream, w, acc); break;
+        default: launch_mass<float>(reinterpret_cast<float *>(l.f.p), t, l.wallMap.p, l.wallData.p, s.partial.p, s.scalar.p, s.codec, flip, s.stream, w, acc); break;
+        }
+        HIPCHK(hipGetLastError());
+    }
+    HIPCHK(hipMemcpyAsync(out2, s.scalar.p, 2 * sizeof(double), hipMemcpyDeviceToHost, s.stream));
+    HIPCHK(hipStreamSynchronize(s.stream));
     return 0;
 }
 
@@ -1280,6 +1375,14 @@ int lbm3_load_checkpoint(lbm3_sim *h, const char *path)
     // the file ends where the last level's data ends (a short file changes nothing)
     const long end = ftell(fp);
     if (fseek(fp, 0, SEEK_END) != 0 || ftell(fp) != end) return fail("file size does not match its records");
+    // decide each level's open-cell fusion now, as the saved simulation had at
+    // its first run: a condition saved as pending is then completed by the next
+    // step's open-cell kernel exactly as it would have been there
+    for (int L = 0; L < s.levels; ++L)
+        if (!s.L[L].openFusionChecked) {
+            s.L[L].openSettled = true;       // the state the load replaces needs no completion
+            if (const int rc = prepare_open_fusion(s, L)) { fclose(fp); return rc; }
+        }
     std::vector<unsigned char> stage;
     for (int L = 0; L < s.levels; ++L) {
         Level &l = s.L[L];

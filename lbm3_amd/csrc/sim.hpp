@@ -119,6 +119,8 @@ struct Sim {
     hipEvent_t fork = nullptr, join = nullptr;
     DBuf<double> partial;            // reduction scratch
     DBuf<double> scalar;
+    int watchdog = 0;                // lbm3_run checks for non-finite populations every this many coarse steps (0: off)
+    std::vector<std::vector<int>> stlEdges;   // shared-edge counts [3 T] of each static body of the last build
     // timing of the level-0 collide-stream launches
     bool timing = false;
     std::vector<std::array<hipEvent_t, 3>> ev;   // before bulk, between, after wall kernel

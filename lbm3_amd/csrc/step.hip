@@ -13,7 +13,7 @@
 //                    streaming, all BCIDs in one launch (updateGrid.h:183-355)
 //  k_periodic        periodic ghost copy (updateGrid.h:357-382)
 //  k_init, k_open_ref_init   initial condition (applyInitialCondition.h:8-84)
-//  k_macros, k_kinetic_energy, k_wall_force   observables, FP64 sums
+//  k_macros, k_cell_sums, k_wall_force        observables, FP64 sums
 //
 // Populations are SoA planes f[q*N + cell] in the storage type S; each thread
 // owns one cell, keeps its 27 populations in registers and writes back into the
@@ -758,15 +758,18 @@ __device__ __forceinline__ double block_sum(double v, double *lds)
     return s;
 }
 
-// per-block partial sums of the kinetic energy 0.5 (1 + dRho) |u|^2 over counted cells
-template <class S, bool FLIP>
-__global__ __launch_bounds__(256) void k_kinetic_energy(const S *f, Topology topo, const int *wallMap, const uint32_t *wallData,
-                                                        double *partial, Codec cd)
+// per-block partial sums over the counted cells -- not solid, not a periodic
+// ghost, not an interface duplicate (taylorGreenVortexMain.cu:141-145):
+// MASS false: the kinetic energy 0.5 (1 + dRho) |u|^2 into partial[block];
+// MASS true: dRho into partial[block] and the cell count into partial[gridDim.x + block]
+template <class S, bool FLIP, bool MASS>
+__global__ __launch_bounds__(256) void k_cell_sums(const S *f, Topology topo, const int *wallMap, const uint32_t *wallData,
+                                                   double *partial, Codec cd)
 {
     __shared__ double lds[4];
     using T = typename StoreTraits<S>::compute;
     const int cell = blockIdx.x * 256 + threadIdx.x;
-    double e = 0.0;
+    double e = 0.0, n = 0.0;
     if (cell < topo.N) {
         const int wm = wallMap[cell];
         const bool skip = wm == -3 || wm == -2 || (wm >= 0 && (wallData[wm] & 1u));
@@ -777,20 +780,29 @@ __global__ __launch_bounds__(256) void k_kinetic_energy(const S *f, Topology top
             load_pre<S, FLIP>(f, topo.N, nb, v, cd);
             T d, ux, uy, uz;
             moments(v, d, ux, uy, uz);
-            e = 0.5 * (1.0 + double(d)) * (double(ux) * double(ux) + double(uy) * double(uy) + double(uz) * double(uz));
+            if (MASS) e = double(d);
+            else e = 0.5 * (1.0 + double(d)) * (double(ux) * double(ux) + double(uy) * double(uy) + double(uz) * double(uz));
+            n = 1.0;
         }
     }
     const double s = block_sum(e, lds);
     if (threadIdx.x == 0) partial[blockIdx.x] = s;
+    if (MASS) {
+        __syncthreads();
+        const double c = block_sum(n, lds);
+        if (threadIdx.x == 0) partial[gridDim.x + blockIdx.x] = c;
+    }
 }
 
-__global__ __launch_bounds__(256) void k_sum_partials(const double *partial, int n, double *out)
+// one block: out = sum of partial[0..n) / divisor, added to the value out holds
+// when accumulate is set (levels weighted and summed on the device, in order)
+__global__ __launch_bounds__(256) void k_sum_partials(const double *partial, int n, double *out, double divisor, int accumulate)
 {
     __shared__ double lds[4];
     double v = 0.0;
     for (int i = threadIdx.x; i < n; i += 256) v += partial[i];
     const double s = block_sum(v, lds);
-    if (threadIdx.x == 0) *out = s;
+    if (threadIdx.x == 0) *out = accumulate ? *out + s / divisor : s / divisor;
 }
 
 // open-boundary flow statistics (trackerFunctions.h:145-170): per block, the
@@ -1107,12 +1119,24 @@ void launch_macros(const S *f, const Topology &t, typename StoreTraits<S>::compu
 
 template <class S>
 int launch_kinetic_energy(const S *f, const Topology &t, const int *wallMap, const uint32_t *wallData, double *partial,
-                          double *out, const Codec &cd, bool flip, hipStream_t s)
+                          double *out, const Codec &cd, bool flip, hipStream_t s, double divisor, bool accumulate)
 {
     const unsigned nb = blocks_for(t.N);
-    if (flip) hipLaunchKernelGGL((k_kinetic_energy<S, true>), dim3(nb), dim3(256), 0, s, f, t, wallMap, wallData, partial, cd);
-    else hipLaunchKernelGGL((k_kinetic_energy<S, false>), dim3(nb), dim3(256), 0, s, f, t, wallMap, wallData, partial, cd);
-    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, partial, (int)nb, out);
+    if (flip) hipLaunchKernelGGL((k_cell_sums<S, true, false>), dim3(nb), dim3(256), 0, s, f, t, wallMap, wallData, partial, cd);
+    else hipLaunchKernelGGL((k_cell_sums<S, false, false>), dim3(nb), dim3(256), 0, s, f, t, wallMap, wallData, partial, cd);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, partial, (int)nb, out, divisor, accumulate ? 1 : 0);
+    return (int)nb;
+}
+
+template <class S>
+int launch_mass(const S *f, const Topology &t, const int *wallMap, const uint32_t *wallData, double *partial, double *out2,
+                const Codec &cd, bool flip, hipStream_t s, double divisor, bool accumulate)
+{
+    const unsigned nb = blocks_for(t.N);
+    if (flip) hipLaunchKernelGGL((k_cell_sums<S, true, true>), dim3(nb), dim3(256), 0, s, f, t, wallMap, wallData, partial, cd);
+    else hipLaunchKernelGGL((k_cell_sums<S, false, true>), dim3(nb), dim3(256), 0, s, f, t, wallMap, wallData, partial, cd);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, partial, (int)nb, out2, divisor, accumulate ? 1 : 0);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, partial + nb, (int)nb, out2 + 1, divisor, accumulate ? 1 : 0);
     return (int)nb;
 }
 
@@ -1316,7 +1340,9 @@ void launch_cut_finish(float *img, long long pixels, const CutUnits &u, hipStrea
     template void launch_macros<S>(const S *, const Topology &, typename StoreTraits<S>::compute *, const Codec &,    \
                                    bool, hipStream_t);                                                            \
     template int launch_kinetic_energy<S>(const S *, const Topology &, const int *, const uint32_t *, double *,       \
-                                          double *, const Codec &, bool, hipStream_t);
+                                          double *, const Codec &, bool, hipStream_t, double, bool);              \
+    template int launch_mass<S>(const S *, const Topology &, const int *, const uint32_t *, double *, double *,       \
+                                const Codec &, bool, hipStream_t, double, bool);
 LBM3_INSTANTIATE(float)
 LBM3_INSTANTIATE(double)
 LBM3_INSTANTIATE(__half)

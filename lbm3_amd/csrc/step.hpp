@@ -69,8 +69,16 @@ template <class S> void launch_init(S *f, const Topology &t, const InitDesc &d, 
 template <class S> void launch_open_ref_init(const OpenArgs<S> &a, int X, int Y, int Z, bool flip, hipStream_t s);
 template <class S> void launch_macros(const S *f, const Topology &t, typename StoreTraits<S>::compute *out, const Codec &cd,
                                       bool flip, hipStream_t s);
+// FP64 sums over the counted cells of a level: the kinetic energy into *out,
+// or dRho into out2[0] and the cell count into out2[1] (launch_mass), each
+// divided by `divisor` and added to what the output holds when `accumulate`;
+// partial holds one double per 256 cells (two for the mass); returns the blocks
 template <class S> int launch_kinetic_energy(const S *f, const Topology &t, const int *wallMap, const uint32_t *wallData,
-                                             double *partial, double *out, const Codec &cd, bool flip, hipStream_t s);
+                                             double *partial, double *out, const Codec &cd, bool flip, hipStream_t s,
+                                             double divisor = 1.0, bool accumulate = false);
+template <class S> int launch_mass(const S *f, const Topology &t, const int *wallMap, const uint32_t *wallData,
+                                   double *partial, double *out2, const Codec &cd, bool flip, hipStream_t s,
+                                   double divisor = 1.0, bool accumulate = false);
 void launch_wall_force(const double *gWall, int W, const uint32_t *wallData, const int *wallIndex, const Topology &t,
                        const LevelGeom &g, const double origin[3], int nBodies, double *partial, hipStream_t s);
 template <class T> void launch_collide_batch(T *f, int n, const Relax<T> &R, hipStream_t s);

@@ -37,11 +37,13 @@ def load():
         "lbm3_sim_destroy": (i32, [vp]),
         "lbm3_build_grids": (i32, [vp, vp, vp, i32, T.Bounds, C.POINTER(i64)]),
         "lbm3_import_level": (i32, [vp, i32, vp]),
+        "lbm3_validate_level": (i32, [vp]),
         "lbm3_apply_initial_condition": (i32, [vp, i32]),
         "lbm3_apply_initial_fields": (i32, [vp, i32, vp]),
         "lbm3_update_single_grid": (i32, [vp, i32]),
         "lbm3_update_all_grids": (i32, [vp, i32]),
         "lbm3_run": (i32, [vp, i32]),
+        "lbm3_set_watchdog": (i32, [vp, i32]),
         "lbm3_synchronize": (i32, [vp]),
         "lbm3_run_timed": (i32, [vp, i32, i32, C.POINTER(f64)]),
         "lbm3_get_info": (i32, [vp, i32, C.POINTER(T.Info)]),
@@ -51,6 +53,7 @@ def load():
         "lbm3_upload_populations": (i32, [vp, i32, vp, i64]),
         "lbm3_download_array": (i32, [vp, i32, C.c_char_p, vp, i64, C.POINTER(i64)]),
         "lbm3_kinetic_energy": (i32, [vp, C.POINTER(f64)]),
+        "lbm3_mass": (i32, [vp, C.POINTER(f64)]),
         "lbm3_wall_forces": (i32, [vp, i32, C.POINTER(f64), i32, C.POINTER(f64)]),
         "lbm3_reset_wall_forces": (i32, [vp]),
         "lbm3_open_boundary_stats": (i32, [vp, i32, C.POINTER(f64)]),
@@ -107,6 +110,41 @@ INTERFACE_ARRAYS = ("f2cIndex", "f2cChild", "c2fIndex", "c2fChild", "leftoverInd
                     "leftoverNbrI", "leftoverNbrJ", "leftoverNbrK")
 
 
+def level_struct(arrays, info, counts=None):
+    """(LevelArrays, kept numpy arrays) from named host arrays; `counts`
+    overrides the sizes taken from the arrays (W, nOpenIDs, nPeriodic, ...)."""
+    keep = {k: np.ascontiguousarray(v) for k, v in arrays.items()}
+    a = LevelArrays()
+    a.info = info
+    a.nRows = len(keep["rowI"])
+    for n in ("shifter", "rowI", "rowJ", "rowK", "rowJPlus", "rowKPlus", "rowJKPlus", "wallMap", "wallIndex",
+              "wallData", "linkLength", "openCount", "openIndex", "openSource", "periodicIndex", "periodicSource"):
+        v = keep.get(n)
+        setattr(a, n, v.ctypes.data if v is not None and v.size else None)
+    a.W = len(keep["wallIndex"])
+    a.nOpenIDs = len(keep["openCount"])
+    a.nPeriodic = len(keep["periodicIndex"])
+    a.wallCount = int(np.asarray(keep.get("wallCount", 0)).reshape(-1)[0])
+    for n in INTERFACE_ARRAYS:
+        v = keep.get(n)
+        if v is not None and v.size:
+            setattr(a, n, v.ctypes.data)
+    a.f2cCount = len(keep.get("f2cIndex", ()))
+    a.c2fCount = len(keep.get("c2fIndex", ()))
+    a.leftoverCount = len(keep.get("leftoverIndex", ()))
+    for k, v in (counts or {}).items():
+        setattr(a, k, v)
+    return a, keep
+
+
+def validate_level(arrays, info, counts=None):
+    """lbm3_validate_level: the import checks, on the host (no GPU needed)."""
+    a, keep = level_struct(arrays, info, counts)
+    rc = load().lbm3_validate_level(C.byref(a))
+    del keep
+    check(rc)
+
+
 STORE = {"f32": T.STORE_F32, "f16": T.STORE_F16, "bf16": T.STORE_BF16, "f64": T.STORE_F64}
 _UINT_ARRAYS = ("wallData", "linkLength", "openMeta")
 
@@ -145,25 +183,8 @@ class Sim:
 
     def import_level(self, level, arrays, info):
         """Upload a level from named host arrays (the gridBuilderToGrid layout)."""
-        keep = {k: np.ascontiguousarray(v) for k, v in arrays.items()}
+        a, keep = level_struct(arrays, info)
         self._keep.append(keep)
-        a = LevelArrays()
-        a.info = info
-        a.nRows = len(keep["rowI"])
-        for n in ("shifter", "rowI", "rowJ", "rowK", "rowJPlus", "rowKPlus", "rowJKPlus", "wallMap", "wallIndex",
-                  "wallData", "linkLength", "openCount", "openIndex", "openSource", "periodicIndex", "periodicSource"):
-            setattr(a, n, keep[n].ctypes.data if keep[n].size else None)
-        a.W = len(keep["wallIndex"])
-        a.nOpenIDs = len(keep["openCount"])
-        a.nPeriodic = len(keep["periodicIndex"])
-        a.wallCount = int(np.asarray(keep.get("wallCount", 0)).reshape(-1)[0])
-        for n in INTERFACE_ARRAYS:
-            v = keep.get(n)
-            if v is not None and v.size:
-                setattr(a, n, v.ctypes.data)
-        a.f2cCount = len(keep.get("f2cIndex", ()))
-        a.c2fCount = len(keep.get("c2fIndex", ()))
-        a.leftoverCount = len(keep.get("leftoverIndex", ()))
         check(load().lbm3_import_level(self.h, level, C.byref(a)))
 
     # --- run ---
@@ -178,6 +199,9 @@ class Sim:
 
     def run(self, steps):
         check(load().lbm3_run(self.h, steps))
+
+    def set_watchdog(self, period):
+        check(load().lbm3_set_watchdog(self.h, period))
 
     def step_level(self, level):
         check(load().lbm3_update_single_grid(self.h, level))
@@ -250,6 +274,12 @@ class Sim:
         v = C.c_double()
         check(load().lbm3_kinetic_energy(self.h, C.byref(v)))
         return v.value
+
+    def mass(self):
+        """(sum of dRho, counted cells) in FP64, level L weighted by 8^-L."""
+        out = (C.c_double * 2)()
+        check(load().lbm3_mass(self.h, out))
+        return out[0], out[1]
 
     def wall_forces(self, level=0, origin=(0.0, 0.0, 0.0), bodies=1):
         o = (C.c_double * 3)(*origin)

@@ -49,10 +49,12 @@ class RequestError(ValueError):
 
 def _number(spec, key, typ, low, high):
     v = spec[key]
-    if isinstance(v, bool) or not isinstance(v, (int, float)) or (typ is int and float(v) != int(v)):
+    # a Python int is an integer at any size; a float only if it has no fraction
+    integral = isinstance(v, int) or (isinstance(v, float) and math.isfinite(v) and v.is_integer())
+    if isinstance(v, bool) or not isinstance(v, (int, float)) or (typ is int and not integral):
         raise RequestError("%s: %r is not %s" % (key, v, "an integer" if typ is int else "a number"))
     v = typ(v)
-    if not math.isfinite(v) or (low is not None and v < low) or (high is not None and v > high):
+    if (typ is float and not math.isfinite(v)) or (low is not None and v < low) or (high is not None and v > high):
         raise RequestError("%s: %r outside [%s, %s]" % (key, v, low, high))
     return v
 

@@ -171,6 +171,7 @@ static void stl_edges(orc_stl *S)
     const int T = S->triangleCount;
     const int bxy = S->binCountX * S->binCountY;
     S->edgeCount = NEWI(3 * T);
+    #pragma omp parallel for schedule(dynamic, 256)
     for (int t = 0; t < T; ++t) {
         const float t1[9] = {S->ax[t], S->ay[t], S->az[t], S->bx[t], S->by[t], S->bz[t], S->cx[t], S->cy[t], S->cz[t]};
         for (int side = 0; side < 2; ++side) {
@@ -247,6 +248,23 @@ int orc_stl_read(orc_stl *S, const char *path, float RES)
     const int rc = orc_stl_from_triangles(S, tri, (int)n32, RES);
     free(tri);
     return rc;
+}
+
+/* shared-edge counts of an STL body (checkSTLEdges, STLFunctions.h:130-287):
+ * out[3 t + e] for edge e (AB, BC, CA) of the t-th kept triangle; returns the
+ * number of kept triangles (degenerate ones dropped as readSTL does), or a
+ * negative code; out may be NULL to ask for the count */
+int orc_stl_edge_counts(const float *tri9, int n, float resGlobal, int *out, int capacity)
+{
+    orc_stl S;
+    if (orc_stl_from_triangles(&S, tri9, n, resGlobal) != 0) { orc_stl_free(&S); return -1; }
+    const int T = S.triangleCount;
+    if (out) {
+        if (capacity < 3 * T) { orc_stl_free(&S); return -2; }
+        memcpy(out, S.edgeCount, sizeof(int) * 3 * (size_t)T);
+    }
+    orc_stl_free(&S);
+    return T;
 }
 
 void orc_stl_free(orc_stl *S)

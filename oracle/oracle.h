@@ -136,6 +136,7 @@ typedef struct orc_sim {
 int orc_stl_read(orc_stl *stl, const char *path, float resGlobal);
 int orc_stl_from_triangles(orc_stl *stl, const float *tri9, int n, float resGlobal);
 void orc_stl_free(orc_stl *stl);
+int orc_stl_edge_counts(const float *tri9, int n, float resGlobal, int *out, int capacity);
 
 /* build + run */
 orc_sim *orc_sim_create(const lbm3_case *C, int isDouble);

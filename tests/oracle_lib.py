@@ -59,6 +59,7 @@ def lib():
         L.orc_rotor_forces.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p]
         L.orc_open_boundary_stats.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
         L.orc_reset_open_boundary_stats.argtypes = [C.c_void_p]
+        L.orc_stl_edge_counts.argtypes = [C.c_void_p, C.c_int, C.c_float, C.c_void_p, C.c_int]
         for t, real in (("f32", C.c_float), ("f64", C.c_double)):
             getattr(L, "orc_if_values_batch_" + t).argtypes = [C.c_void_p, C.c_int, real, C.c_void_p]
             getattr(L, "orc_if_coefficients_" + t).argtypes = [C.c_void_p, C.c_void_p]
@@ -68,6 +69,16 @@ def lib():
 
 
 _INT_DTYPES = {4: np.int32, 8: np.int64}
+
+
+def stl_edge_counts(triangles, res):
+    """Shared-edge counts [kept triangles][3] of one STL body (checkSTLEdges)."""
+    tri = np.ascontiguousarray(triangles, np.float32).reshape(-1, 9)
+    n = lib().orc_stl_edge_counts(tri.ctypes.data, len(tri), res, None, 0)
+    assert n >= 0, n
+    out = np.zeros((n, 3), np.int32)
+    assert lib().orc_stl_edge_counts(tri.ctypes.data, len(tri), res, out.ctypes.data, out.size) == n
+    return out
 
 
 class OracleSim:

@@ -5,11 +5,14 @@ through the macroscopic fields of every level, open-boundary statistics, wall
 and rotor forces, iteration counters and esotwist parity -- and a checkpoint
 that does not fit the simulation is refused without changing it."""
 import os
+import tempfile
 
 import numpy as np
 import pytest
 
 from lbm3_amd import cases, geometry, lib
+from tests.helpers import rel_max
+from tests.oracle_lib import OracleSim
 
 pytestmark = pytest.mark.gpu
 
@@ -72,6 +75,70 @@ def test_restart_reproduces_the_pipe_run(tmp_path, storage):
     b.close()
     _same(want, got)
     assert want["info0"][0] == 35 and want["info1"][0] == 70
+
+
+@pytest.mark.parametrize("storage", ["f32", "f16"])
+def test_fresh_restart_with_a_fused_open_level(tmp_path, storage):
+    """Sphere whose open faces touch no wall, so level 0 runs its open cells in
+    the open-cell kernel (the boundary condition of a step completed when the
+    next step loads the cell).  The checkpoint is taken with that condition
+    pending and loaded into a freshly built simulation that has never run;
+    the continued runs agree bit for bit."""
+    case, dom, tris = cases.sphere(48, 24, 24, diameter=6.0, center=(14.0, 12.0, 12.0), segments=(16, 32))
+    a = lib.Sim(case, storage)
+    a.build(dom, tris)
+    a.init()
+    a.run(17)
+    assert a.cell_kinds(0)["open"] > 0                   # the open cells have their own kernel
+    path = tmp_path / "sphere.ckpt"
+    a.save_checkpoint(path)
+    a.run(11)
+    want = _observe(a, 1)
+    want["open"] = [a.open_boundary_stats(x) for x in range(3)]
+    want["ke"] = a.kinetic_energy()
+    a.close()
+    b = lib.Sim(case, storage)
+    b.build(dom, tris)
+    b.load_checkpoint(path)                              # no init, no run before
+    assert b.cell_kinds(0)["open"] > 0
+    b.run(11)
+    got = _observe(b, 1)
+    got["open"] = [b.open_boundary_stats(x) for x in range(3)]
+    got["ke"] = b.kinetic_energy()
+    b.close()
+    _same(want, got)
+
+
+def test_restored_fp64_run_equals_the_oracle():
+    """FP64 sphere: 9 steps, checkpoint, a fresh simulation restored from it and
+    stepped 13 more equals the oracle run of 22 steps (macroscopic fields and
+    wall force within 1e-10)."""
+    case, dom, tris = cases.sphere(40, 20, 20, diameter=6.0, center=(12.0, 10.0, 10.0), segments=(16, 32))
+    o = OracleSim(case, dom, tris, double=True)
+    o.init()
+    # the initial-condition hooks evaluate in float, whose last bit may differ
+    # between libm and the device: start from the oracle's populations
+    f0 = o.array("f", 0).reshape(27, -1).copy()
+    o.run(22)
+    a = lib.Sim(case, "f64")
+    a.build(dom, tris)
+    a.init()
+    a.set_populations(0, f0)
+    a.run(9)
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "f64.ckpt")
+        a.save_checkpoint(path)
+        a.close()
+        b = lib.Sim(case, "f64")
+        b.build(dom, tris)
+        b.load_checkpoint(path)
+    b.run(13)
+    mg, mo = b.macros(0), o.macros(0)
+    err = max(rel_max(1 + mg[:, 0], 1 + mo[:, 0]), rel_max(mg[:, 1:], mo[:, 1:]))
+    print("restored FP64 sphere vs oracle after 9 + 13 steps: %.2e" % err)
+    assert err < 1e-10
+    assert b.info(0).iterationsFinished == 22
+    b.close()
 
 
 def test_restart_reproduces_the_rotor_duct_f32(tmp_path):

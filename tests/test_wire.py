@@ -86,3 +86,18 @@ def test_the_serving_mix_is_wire_format():
     for r in serving.make_requests(n=256, seed=1, steps_scale=0.02):
         spec = wire.parse_request(json.dumps(r["spec"]))
         assert spec["kind"] == r["kind"] and spec["steps"] == r["steps"] and wire.cells_of(spec) == r["cells"]
+
+
+def test_integer_fields_take_any_python_int_in_range():
+    """A seed above 2**53 (not exact in a double) is still an integer; a float
+    with no fraction is accepted, one with a fraction is not."""
+    big = 2 ** 63 - 1
+    assert wire.parse_request(dict(SPHERE, seed=big))["seed"] == big
+    assert wire.parse_request(dict(SPHERE, seed=2 ** 53 + 1))["seed"] == 2 ** 53 + 1
+    assert wire.parse_request(dict(SPHERE, seed=7.0))["seed"] == 7
+    with pytest.raises(wire.RequestError, match="outside"):
+        wire.parse_request(dict(SPHERE, seed=2 ** 63))
+    with pytest.raises(wire.RequestError, match="integer"):
+        wire.parse_request(dict(SPHERE, seed=7.5))
+    with pytest.raises(wire.RequestError, match="integer"):
+        wire.parse_request(dict(SPHERE, seed=float("inf")))
