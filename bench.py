@@ -20,6 +20,7 @@ a second, much smaller kernel whose time is reported beside it.
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -49,6 +50,8 @@ def measure(storage, steps, warmup, k17=False):
     walls = int((sim.array("wallMap") == -3).sum())
     wall_adjacent = int(sim.array("wallIndex").size)
     sim.init()
+    # launch choices picked by timing them on this box (lbm3_autotune; the state is kept)
+    tuned, table = sim.autotune(steps=10)
     sim.run(warmup)
     sim.sync()
     total_ms, _, _, _ = sim.run_timed(steps, kernel_events=False)
@@ -72,11 +75,57 @@ def measure(storage, steps, warmup, k17=False):
         "cells": info.cellCount, "walls": walls, "fluid_updates_per_step": fluid_updates,
         "bytes_per_launch": bytes_per_launch, "build_s": round(build_s, 3),
         "steps_done": warmup + 2 * steps,
+        "tune": {"chosen": tuned, "candidates": len(table),
+                 "ms_per_step_best": round(min(r[4] for r in table), 4),
+                 "ms_per_step_worst": round(max(r[4] for r in table), 4)},
     }
     u = sim.macros(0)[:, 1:].astype(np.float64)
     u[sim.array("wallMap") == -3] = np.nan     # solid cells hold no flow
     sim.close()
     return out, u
+
+
+def sphere_run(storage, steps, reynolds=100.0, window=1000):
+    """The sphere workload for `steps` steps: velocity field and the drag
+    coefficient C_D = 2 F_drag / (rho U^2 pi D^2 / 4) from the FP64
+    momentum-exchange force averaged over the last `window` steps
+    (flowAroundSphereMain.cu:212-213)."""
+    from lbm3_amd import cases, lib
+    case, dom, tris = cases.sphere(256, 128, 128, diameter=32.0, center=(64.0, 64.0, 64.0), u_inlet=0.05,
+                                   reynolds=reynolds, limiter=0.1, k15_on_level0=True)
+    s = lib.Sim(case, storage)
+    s.build(dom, tris)
+    s.init()
+    s.run(steps - window)
+    s.reset_wall_forces()
+    s.run(window)
+    s.sync()
+    fx = s.wall_forces(0)[0][0] / window
+    u = s.macros(0)[:, 1:].astype(np.float64)
+    u[s.array("wallMap") == -3] = np.nan
+    s.close()
+    cd = -2.0 * fx / (0.05 ** 2 * np.pi * 32.0 ** 2 / 4.0)    # the force on the fluid is minus the drag
+    return u, float(cd)
+
+
+def accuracy_runs(steps, storages=("f32", "f16", "bf16")):
+    """Relative L2 velocity error of each storage precision against the FP64
+    run of the same kernels after `steps` steps of the sphere workload, and the
+    FP64 drag coefficient at Re 100 and 300 (Schiller-Naumann: 1.09 and 0.69
+    in free stream; here the sides hold the inlet velocity and the sphere
+    blocks 4.9% of the section)."""
+    u64, cd100 = sphere_run("f64", steps)
+    mask = np.isfinite(u64).all(axis=1)
+    out = {"steps": steps, "drag_coefficient_f64": {"re100": round(cd100, 5)}}
+    for st in storages:
+        u, cd = sphere_run(st, steps)
+        m = mask & np.isfinite(u).all(axis=1)
+        out["%s_rel_l2_u_vs_f64" % st] = float(np.linalg.norm((u - u64)[m]) / np.linalg.norm(u64[m]))
+        out["drag_coefficient_%s" % st] = round(cd, 5)
+    _, cd300 = sphere_run("f64", steps, reynolds=300.0)
+    out["drag_coefficient_f64"]["re300"] = round(cd300, 5)
+    print("accuracy: %s" % out, file=sys.stderr, flush=True)
+    return out
 
 
 def measure_multilevel(storage, steps=200, warmup=20):
@@ -160,11 +209,9 @@ def measure_rotor(storage, windows=2, window=100, warmup=20):
     return out
 
 
-def cpu_baseline(threads, seconds=10.0, batch=10, max_steps=400):
-    """The CPU oracle (C, OpenMP over cells) on the same sphere workload:
-    batches of `batch` time steps until `seconds` of timed work (at most
-    `max_steps` steps)."""
-    os.environ.setdefault("OMP_NUM_THREADS", str(threads))
+def _cpu_sample(seconds, batch=10, max_steps=400):
+    """Steps of the CPU oracle on the sphere workload until `seconds` of timed
+    work: (MLUPS, steps, seconds)."""
     from tests.oracle_lib import OracleSim
     case, dom, tris = workload()
     o = OracleSim(case, dom, tris)
@@ -176,9 +223,68 @@ def cpu_baseline(threads, seconds=10.0, batch=10, max_steps=400):
         o.run(batch)
         steps += batch
     dt = time.time() - t0
-    return {"value": round(o.fluid_updates * steps / dt / 1e6, 3), "unit": "MLUPS", "cores": threads, "kind": "port",
-            "sample": "oracle (C restatement, OpenMP over cells) on the full 256x128x128 sphere grid, %d time steps "
-                      "after 1 untimed step, %.1f s" % (steps, dt)}
+    return o.fluid_updates * steps / dt / 1e6, steps, dt
+
+
+def _host_cpu():
+    """lscpu's model name, sockets, cores per socket, threads per core, NUMA nodes."""
+    info = {}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+    except (OSError, subprocess.SubprocessError):
+        return info
+    for line in out.splitlines():
+        k, _, v = line.partition(":")
+        info[k.strip()] = v.strip()
+    res = {"model": info.get("Model name")}
+    try:
+        res["physical_cores"] = int(info["Socket(s)"]) * int(info["Core(s) per socket"])
+        res["threads_per_core"] = int(info["Thread(s) per core"])
+    except (KeyError, ValueError):
+        pass
+    if "NUMA node(s)" in info:
+        res["numa_nodes"] = info["NUMA node(s)"]
+    return res
+
+
+def native_oracle():
+    """The oracle built for this host (-O3 -march=native, contraction off as in
+    the parity build): oracle/_build/liboracle_native.so, made here because the
+    build container's CPU differs from the GPU box's host."""
+    subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "native"], check=True,
+                   capture_output=True, text=True)
+    return os.path.join(ROOT, "oracle", "_build", "liboracle_native.so")
+
+
+def cpu_baseline(threads, seconds=4.0, runs=3, single_seconds=6.0):
+    """The CPU oracle (C, OpenMP over cells, -O3 -march=native) on the same
+    sphere workload: the median of `runs` bounded samples with `threads`
+    threads pinned close, and one single-thread sample in a child process."""
+    lib_path = native_oracle()
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    from tests import oracle_lib
+    oracle_lib.LIB = lib_path
+    samples = [_cpu_sample(seconds) for _ in range(runs)]
+    mlups = sorted(s[0] for s in samples)[len(samples) // 2]
+    env = dict(os.environ, OMP_NUM_THREADS="1", LBM3_ORACLE_LIB=lib_path)
+    code = ("import sys, json; sys.path.insert(0, %r); from tests import oracle_lib; import os; "
+            "oracle_lib.LIB = os.environ['LBM3_ORACLE_LIB']; import bench; "
+            "print(json.dumps(bench._cpu_sample(%r, batch=1, max_steps=50)))" % (ROOT, single_seconds))
+    single = None
+    try:
+        r = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True, timeout=120)
+        if r.returncode == 0:
+            v, st, dt = json.loads(r.stdout.strip().splitlines()[-1])
+            single = {"value": round(v, 3), "steps": st, "seconds": round(dt, 1)}
+    except (subprocess.SubprocessError, ValueError, IndexError):
+        single = None
+    steps = sum(s[1] for s in samples)
+    return {"value": round(mlups, 3), "unit": "MLUPS", "cores": threads, "kind": "port",
+            "sample": "oracle (C restatement, -O3 -march=native, OpenMP over cells, %d threads, OMP_PROC_BIND=close) "
+                      "on the full 256x128x128 sphere grid: median of %d runs of 10-step batches for about %.0f s each "
+                      "(%d steps in all) after 1 untimed step" % (threads, runs, seconds, steps),
+            "runs_mlups": [round(s[0], 3) for s in samples], "single_thread": single, "host": _host_cpu()}
 
 
 def traffic_from_profiles(storage, bulk_cells):
@@ -237,8 +343,11 @@ def main():
     ap.add_argument("--variants", default="f16,bf16,k17")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
-    ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16")))
-    ap.add_argument("--serving", choices=["quick", "full", "none"], default="quick",
+    ap.add_argument("--accuracy-steps", type=int, default=5000,
+                    help="steps of the sphere runs compared with FP64 (velocity error, drag coefficient)")
+    ap.add_argument("--cpu-threads", type=int, default=len(os.sched_getaffinity(0)),
+                    help="CPU-baseline threads (default: every CPU this process may run on)")
+    ap.add_argument("--serving", choices=["quick", "full", "none"], default="full",
                     help="serving workload: quick (32 requests, FP32), full (256 requests, "
                          "500-4000 steps, FP32 and FP16), none")
     ap.add_argument("--no-other-cases", action="store_true",
@@ -263,34 +372,16 @@ def main():
         sys.exit(2)
     from lbm3_amd import lib
     name, mem, cus = lib.device_info()
-    main_run, u32 = measure("f32", args.steps, args.warmup)
+    main_run, _ = measure("f32", args.steps, args.warmup)
     variants = {}
-    errs = {}
     for v in [x for x in args.variants.split(",") if x]:
         if v == "k17":   # FP32 storage, K17 with limiter 0.1 in the bulk (a different flow: no error vs FP64 here)
             r, _ = measure("f32", args.steps, args.warmup, k17=True)
             variants[v] = r
             continue
-        r, u = measure(v, args.steps, args.warmup)
+        r, _ = measure(v, args.steps, args.warmup)
         variants[v] = r
-        errs[v] = u
-    # velocity error of each storage precision against an FP64 run of the same number of steps
-    accuracy = None
-    if not args.no_accuracy:
-        from lbm3_amd import lib as L2
-        case, dom, tris = workload()
-        s64 = L2.Sim(case, "f64")
-        s64.build(dom, tris)
-        s64.init()
-        s64.run(main_run["steps_done"])
-        s64.sync()
-        u64 = s64.macros(0)[:, 1:].astype(np.float64)
-        s64.close()
-        mask = np.isfinite(u64).all(axis=1) & np.isfinite(u32).all(axis=1)
-        accuracy = {"steps": main_run["steps_done"],
-                    "f32_rel_l2_u_vs_f64": float(np.linalg.norm((u32 - u64)[mask]) / np.linalg.norm(u64[mask]))}
-        for v, u in errs.items():
-            accuracy["%s_rel_l2_u_vs_f64" % v] = float(np.linalg.norm((u - u64)[mask]) / np.linalg.norm(u64[mask]))
+    accuracy = None if args.no_accuracy else accuracy_runs(args.accuracy_steps)
     copy_bytes = 1 << 30
     copy_ms = lib.copy_probe(copy_bytes, 20)
     # measured HBM bytes of the K15 bulk kernel (the K17 variant has no counter run of its own)
@@ -326,6 +417,7 @@ def main():
                    "fluid_updates_per_step": main_run["fluid_updates_per_step"]},
         "roofline": main_run["roofline"],
         "cpu_baseline": cpu,
+        "tune": main_run["tune"],
         "kernel_us": round(main_run["kernel_us"], 2),
         "boundary_kernels_us": round(main_run["boundary_kernels_us"], 2),
         "kernel_mlups": round(main_run["kernel_mlups"], 1),

@@ -122,6 +122,22 @@ int lbm3_set_watchdog(lbm3_sim *s, int period);
  * short launches per coarse step make launch overhead count. */
 int lbm3_run_graph(lbm3_sim *s, int steps);
 int lbm3_synchronize(lbm3_sim *s);
+/* Autotuner of the step's launch choices (no counterpart in the reference):
+ * times `steps` whole coarse steps (after one untimed step) for every
+ * combination of the bulk kernel's occupancy cap (dynamic LDS per block 0,
+ * 24, 40 or 64 KiB), its block order (round-robin over the XCDs or one
+ * contiguous part per XCD), the side kernels' placement (second stream beside
+ * the bulk kernel or in order after it) and, where a level completes its open
+ * boundary in the open-cell kernel, that kernel's group width (8 or 16 cells),
+ * keeps the fastest, and restores the simulation's state as it was (the
+ * populations and every step counter; a pending open-boundary condition is
+ * applied first).  out[5 r + {LDS bytes, XCD order, side stream, group cells,
+ * ms per step}] for the first `cap` candidates; *nOut the candidates timed. */
+int lbm3_autotune(lbm3_sim *s, int steps, double *out, int cap, int *nOut);
+/* The launch choices in use: {bulk LDS bytes, XCD order, side stream, open
+ * group cells}; lbm3_set_tune sets them (LDS 0..65536, group 8 or 16). */
+int lbm3_get_tune(lbm3_sim *s, int *out4);
+int lbm3_set_tune(lbm3_sim *s, const int *in4);
 
 /* Timed run: `steps` coarse steps between HIP events on the sim's stream.
  * out[0] = total ms; with kernelEvents != 0 also out[1] = summed ms of the
@@ -151,6 +167,62 @@ int lbm3_upload_populations(lbm3_sim *s, int level, const void *in, long long by
  * updated by kernels of their own).  With out == NULL only
  * *count is returned. */
 int lbm3_download_array(lbm3_sim *s, int level, const char *name, void *out, long long capacity, long long *count);
+/* The same arrays in device memory, not copied: *ptr points into the
+ * simulation's own buffer (valid until the level is rebuilt or the sim is
+ * destroyed), *count its element count.  "f" gives the populations (storage
+ * type; tiled, see lbm3_population_layout), after any pending open-boundary
+ * completion is applied.  Replaces the case files' direct view access
+ * (Grid.fArray.getConstView(), Grid.IJKNBR.*, Grid.Wall.*:
+ * taylorGreenVortexMain.cu:142-155). */
+int lbm3_device_array(lbm3_sim *s, int level, const char *name, void **ptr, long long *count);
+/* Population layout of "f": slot (q, cell) at (cell / tile) * 27 * tile +
+ * q * tile + cell % tile; *bytes per value (4 FP32, 2 FP16/BF16, 8 FP64).
+ * FP32 and FP64 hold the shifted populations f - w (applyCollision.h:8-10). */
+int lbm3_population_layout(lbm3_sim *s, int *tile, int *bytes);
+
+/* A case's four device hooks compiled with the case file and evaluated by it
+ * (the reference inlines getRefinementModifier, getInitialCondition, getOpenBC
+ * and getLocalBC into its kernels: markerFunctions.h:258-276,
+ * applyInitialCondition.h:22-35, gridBuilderFunctions.h:1232-1300,1486-1503,
+ * updateGrid.h:86-95,256).  Each callback evaluates its hook at n cells of the
+ * level `info` describes, whose coordinates come as device arrays i, j, k;
+ * writes n results into a device array; enqueues its work on `stream` (a
+ * hipStream_t); returns 0 (anything else fails the calling entry point with
+ * LBM3_E_ARG).  A null callback leaves that hook to the case's rule table.
+ * Results are data: the library classifies them once (lbm3_build_grids,
+ * lbm3_apply_initial_condition, lbm3_refresh_hooks), so the step kernels read
+ * tables, not user code. */
+typedef struct lbm3_hooks {
+    void *user;
+    /* out[c] bit 0: the marker the hook leaves for an incoming marker false,
+     * bit 1: for an incoming marker true */
+    int (*refinement)(void *user, const lbm3_info *info, const int *i, const int *j, const int *k, int n,
+                      uint8_t *out, void *stream);
+    /* out4[4c..4c+3] = BC.dRho, ux, uy, uz after the hook on a default BCStruct */
+    int (*initial_condition)(void *user, const lbm3_info *info, const int *i, const int *j, const int *k, int n,
+                             float *out4, void *stream);
+    /* out[c] = the BCStruct after the hook on a default BCStruct */
+    int (*open_bc)(void *user, const lbm3_info *info, const int *i, const int *j, const int *k, int n,
+                   lbm3_bc *out, void *stream);
+    /* in[c]: a default BCStruct with wallID and a state (dRho, ux, uy, uz) as the
+     * step fills it; out[c] = the BCStruct after the hook.  The hook must not
+     * depend on the state except by passing the velocity through (a wall that
+     * moves with the fluid): the library evaluates it on two states and refuses
+     * anything else, and refuses parameters (force, limiter, viscosity
+     * multiplier, IBB) that differ between cells of one wall id or between
+     * fluid cells of a level.  The wall velocity may differ from cell to cell. */
+    int (*local_bc)(void *user, const lbm3_info *info, const int *i, const int *j, const int *k, const lbm3_bc *in,
+                    int n, lbm3_bc *out, void *stream);
+} lbm3_hooks;
+/* Before lbm3_build_grids; hooks == NULL returns to the rule tables. */
+int lbm3_set_hooks(lbm3_sim *s, const lbm3_hooks *hooks);
+/* Host-side InfoStruct fields a case may change between steps (iRegulatorInlet,
+ * iRegulatorOutlet, useRotors: BruteforcePump.cu:226-229,293-298), seen by the
+ * hooks from then on. */
+int lbm3_set_info_controls(lbm3_sim *s, int level, float iRegulatorInlet, float iRegulatorOutlet, int useRotors);
+/* Evaluate the open-boundary and local hooks again on every built level (after
+ * the case changed what they read) and replace the tables the step uses. */
+int lbm3_refresh_hooks(lbm3_sim *s);
 
 /* buildRotors (rotorFunctions.h:315-585, called by buildGrids for the levels
  * with Info.useRotors, gridBuilderFunctions.h:1612-1619): every level whose bit

@@ -238,4 +238,13 @@ __host__ __device__ __forceinline__ int normal_index(int nx, int ny, int nz)
     return n;
 }
 
+// per open cell: bits 0-4 the normal's table index, bit 5 set where the cell's
+// flow is tracked, bits 6-31 the index of its boundary-condition record
+constexpr uint32_t kOpenTrackBit = 1u << 5;
+__host__ __device__ __forceinline__ uint32_t open_meta(int normal, int record, bool track)
+{
+    return (uint32_t)normal | (track ? kOpenTrackBit : 0u) | ((uint32_t)record << 6);
+}
+__host__ __device__ __forceinline__ uint32_t open_meta_record(uint32_t meta) { return meta >> 6; }
+
 } // namespace lbm3

@@ -789,6 +789,112 @@ struct StlView {
     int bx, by, bz;
 };
 
+// getOpenBC of every boundary cell of a level (rm[0]) and of its projections
+// onto the x, y and z boundary planes where the cell lies within `layers` of
+// them (rm[1..3], the periodic-overlap test, gridBuilderFunctions.h:1232-1274),
+// as indices into `recs`, the records those calls return (-1: no call).  From
+// the case's rule table (records: the rules, then the default), or from the
+// case's own hook through the open_bc callback (records: the distinct BCStructs
+// it returned, in order of first appearance).
+static int open_rule_maps(Sim &sim, BLevel &B, int L, const lbm3_case *dc, Scratch &sc, DBuf<int> (&rm)[4],
+                          std::vector<lbm3_bc> &recs)
+{
+    hipStream_t s = sim.stream;
+    const long long N = B.N;
+    const Views v = views(B);
+    const lbm3_info Il = B.I;
+    const int layers = 1 << L, lev = L;
+    const uint8_t *wall = B.wall.p;
+    for (DBuf<int> &a : rm) BCHK(a.alloc(N > 0 ? N : 1));
+    int *r0 = rm[0].p, *r1 = rm[1].p, *r2 = rm[2].p, *r3 = rm[3].p;
+    if (!sim.hooks.open_bc) {
+        for_n(N, s, [=] __device__(long long c) {
+            r0[c] = r1[c] = r2[c] = r3[c] = -1;
+            if (wall[c]) return;
+            const int i = v.i[c], j = v.j[c], k = v.k[c];
+            const int X = Il.cellCountX, Y = Il.cellCountY, Z = Il.cellCountZ;
+            if (i == 0 || i == X - 1 || j == 0 || j == Y - 1 || k == 0 || k == Z - 1) r0[c] = open_rule(*dc, lev, X, Y, Z, i, j, k);
+            if (i < layers || i >= X - layers) r1[c] = open_rule(*dc, lev, X, Y, Z, i < layers ? 0 : X - 1, j, k);
+            if (j < layers || j >= Y - layers) r2[c] = open_rule(*dc, lev, X, Y, Z, i, j < layers ? 0 : Y - 1, k);
+            if (k < layers || k >= Z - layers) r3[c] = open_rule(*dc, lev, X, Y, Z, i, j, k < layers ? 0 : Z - 1);
+        });
+        const lbm3_case &C = sim.C;
+        recs.clear();
+        for (int r = 0; r < C.nOpenRules; ++r) recs.push_back(C.openRules[r].bc);
+        recs.push_back(C.openDefault);
+        return 0;
+    }
+    // queries: for each of the four calls, the cells that make it, compacted
+    DBuf<uint8_t> fl;
+    BCHK(fl.alloc(N > 0 ? N : 1));
+    DBuf<int> lists[4];
+    long long count[4], total = 0;
+    for (int t = 0; t < 4; ++t) {
+        uint8_t *f = fl.p;
+        for_n(N, s, [=] __device__(long long c) {
+            bool q = false;
+            if (!wall[c]) {
+                const int i = v.i[c], j = v.j[c], k = v.k[c];
+                const int X = Il.cellCountX, Y = Il.cellCountY, Z = Il.cellCountZ;
+                if (t == 0) q = i == 0 || i == X - 1 || j == 0 || j == Y - 1 || k == 0 || k == Z - 1;
+                else if (t == 1) q = i < layers || i >= X - layers;
+                else if (t == 2) q = j < layers || j >= Y - layers;
+                else q = k < layers || k >= Z - layers;
+            }
+            f[c] = q;
+        });
+        count[t] = compact(sc, fl.p, N, lists[t], s);
+        if (count[t] < 0) return LBM3_E_HIP;
+        total += count[t];
+    }
+    recs.clear();
+    for (int t = 0; t < 4; ++t) for_n(N, s, [=] __device__(long long c) { (t == 0 ? r0 : t == 1 ? r1 : t == 2 ? r2 : r3)[c] = -1; });
+    if (total == 0) return 0;
+    if (total > (1LL << 31) - 1) return set_error(LBM3_E_MEMORY, "too many open-boundary hook calls on one level");
+    DBuf<int> qi, qj, qk;
+    BCHK(qi.alloc(total));
+    BCHK(qj.alloc(total));
+    BCHK(qk.alloc(total));
+    long long off = 0;
+    for (int t = 0; t < 4; ++t) {
+        const int *lc = lists[t].p;
+        int *oi = qi.p + off, *oj = qj.p + off, *ok = qk.p + off;
+        for_n(count[t], s, [=] __device__(long long x) {
+            const int c = lc[x];
+            int i = v.i[c], j = v.j[c], k = v.k[c];
+            const int X = Il.cellCountX, Y = Il.cellCountY, Z = Il.cellCountZ;
+            if (t == 1) i = i < layers ? 0 : X - 1;
+            else if (t == 2) j = j < layers ? 0 : Y - 1;
+            else if (t == 3) k = k < layers ? 0 : Z - 1;
+            oi[x] = i; oj[x] = j; ok[x] = k;
+        });
+        off += count[t];
+    }
+    DBuf<lbm3_bc> out;
+    BCHK(out.alloc(total));
+    lbm3_info info = Il;
+    if (sim.hooks.open_bc(sim.hooks.user, &info, qi.p, qj.p, qk.p, (int)total, out.p, (void *)s) != 0)
+        return set_error(LBM3_E_ARG, "the getOpenBC hook callback failed");
+    BCHK(hipGetLastError());
+    std::vector<lbm3_bc> host((size_t)total);
+    BCHK(hipMemcpyAsync(host.data(), out.p, sizeof(lbm3_bc) * host.size(), hipMemcpyDeviceToHost, s));
+    BCHK(hipStreamSynchronize(s));
+    std::vector<int> idx((size_t)total);
+    dedupe_bc(host, idx, recs);
+    DBuf<int> di;
+    BCHK(di.alloc(total));
+    BCHK(hipMemcpyAsync(di.p, idx.data(), sizeof(int) * idx.size(), hipMemcpyHostToDevice, s));
+    off = 0;
+    for (int t = 0; t < 4; ++t) {
+        const int *lc = lists[t].p, *ix = di.p + off;
+        int *dst = t == 0 ? r0 : t == 1 ? r1 : t == 2 ? r2 : r3;
+        for_n(count[t], s, [=] __device__(long long x) { dst[lc[x]] = ix[x]; });
+        off += count[t];
+    }
+    BCHK(hipStreamSynchronize(s));   // the host vectors above outlive the copies
+    return 0;
+}
+
 int walls_and_grid(Sim &sim, std::vector<BLevel> &G, std::vector<DevSTL> &stl, int nBodies, const lbm3_case *dc, Scratch &sc,
                    long long *fluidUpdates)
 {
@@ -1083,21 +1189,26 @@ int walls_and_grid(Sim &sim, std::vector<BLevel> &G, std::vector<DevSTL> &stl, i
                 for (int p = 0; p < 7; ++p) lk[(size_t)p * W + w] = packed[p];
             });
         }
+        // getOpenBC at the boundary cells and their boundary projections
+        DBuf<int> rm[4];
+        DBuf<lbm3_bc> drec;
+        if (const int rc = open_rule_maps(sim, B, L, dc, sc, rm, l.openBC)) return rc;
+        BCHK(drec.alloc(l.openBC.empty() ? 1 : (long long)l.openBC.size()));
+        if (!l.openBC.empty())
+            BCHK(hipMemcpyAsync(drec.p, l.openBC.data(), sizeof(lbm3_bc) * l.openBC.size(), hipMemcpyHostToDevice, s));
+        const lbm3_bc *rec = drec.p;
+        const int *rSelf = rm[0].p, *rPx = rm[1].p, *rPy = rm[2].p, *rPz = rm[3].p;
         // periodic overlap: no force tracking near periodic boundaries (:1232-1274)
         {
             int *wm = l.wallMap.p;
             uint32_t *wd = l.wallData.p;
             const uint8_t *wall = B.wall.p;
-            const lbm3_info Il = B.I;
-            const int layers = 1 << L, lev = L;
             for_n(N, s, [=] __device__(long long c) {
                 if (wall[c]) return;
-                const int i = v.i[c], j = v.j[c], k = v.k[c];
-                const int X = Il.cellCountX, Y = Il.cellCountY, Z = Il.cellCountZ;
                 bool per = false;
-                if (i < layers || i >= X - layers) per |= open_bc(*dc, open_rule(*dc, lev, X, Y, Z, i < layers ? 0 : X - 1, j, k)).periodic != 0;
-                if (j < layers || j >= Y - layers) per |= open_bc(*dc, open_rule(*dc, lev, X, Y, Z, i, j < layers ? 0 : Y - 1, k)).periodic != 0;
-                if (k < layers || k >= Z - layers) per |= open_bc(*dc, open_rule(*dc, lev, X, Y, Z, i, j, k < layers ? 0 : Z - 1)).periodic != 0;
+                if (rPx[c] >= 0) per |= rec[rPx[c]].periodic != 0;
+                if (rPy[c] >= 0) per |= rec[rPy[c]].periodic != 0;
+                if (rPz[c] >= 0) per |= rec[rPz[c]].periodic != 0;
                 if (!per) return;
                 const int m = wm[c];
                 if (m >= 0) wd[m] |= 1u;
@@ -1116,16 +1227,13 @@ int walls_and_grid(Sim &sim, std::vector<BLevel> &G, std::vector<DevSTL> &stl, i
             int *bi = bcid.p, *ru = rule.p, *m = mx.p;
             const uint8_t *wall = B.wall.p;
             const lbm3_info Il = B.I;
-            const int lev = L;
             for_n(N, s, [=] __device__(long long c) {
                 bi[c] = -1;
                 ru[c] = -1;
                 if (wall[c]) return;
-                const int i = v.i[c], j = v.j[c], k = v.k[c];
-                const int X = Il.cellCountX, Y = Il.cellCountY, Z = Il.cellCountZ;
-                if (!(i == 0 || i == X - 1 || j == 0 || j == Y - 1 || k == 0 || k == Z - 1)) return;
-                const int r = open_rule(*dc, lev, X, Y, Z, i, j, k);
-                const lbm3_bc &b = open_bc(*dc, r);
+                const int r = rSelf[c];
+                if (r < 0) return;
+                const lbm3_bc &b = rec[r];
                 ru[c] = r;
                 bi[c] = b.periodic ? (b.periodic ? -2 : -1) : b.openBCID;
                 if (!b.periodic) atomicMax(m, b.openBCID);
@@ -1166,7 +1274,7 @@ int walls_and_grid(Sim &sim, std::vector<BLevel> &G, std::vector<DevSTL> &stl, i
                 const int nx = i == 0 ? -1 : (i == X - 1 ? 1 : 0), ny = j == 0 ? -1 : (j == Y - 1 ? 1 : 0), nz = k == 0 ? -1 : (k == Z - 1 ? 1 : 0);
                 const int faces = (nx != 0) + (ny != 0) + (nz != 0);
                 const int r = ru[c];
-                const lbm3_bc &b = open_bc(*dc, r);
+                const lbm3_bc &b = rec[r];
                 const bool edge = b.dirichletU && faces == 2;
                 int pref = 0;
                 if (edge) {
@@ -1193,7 +1301,7 @@ int walls_and_grid(Sim &sim, std::vector<BLevel> &G, std::vector<DevSTL> &stl, i
                 const int wm = wmap[c];
                 bool track = wm != -2;
                 if (wm >= 0 && (wdat[wm] & 1u)) track = false;
-                meta[x] = (uint32_t)normal_index(nx, ny, nz) | ((uint32_t)r << 5) | ((track ? 1u : 0u) << 13);
+                meta[x] = open_meta(normal_index(nx, ny, nz), r, track);
             });
             if (read_one(bad.p, s) > 0) return set_error(LBM3_E_BUILD, "Building open BC failed: no valid source for some cells.");
             // periodic cells and sources (:1486-1564)
@@ -1536,9 +1644,20 @@ int gpu_build(Sim &sim, const float *tri, const int *triCounts, int nBodies, lbm
                 if (spread(B, deep, bp, s)) return LBM3_E_HIP;
             }
             const lbm3_info Il = B.I;
+            // getRefinementModifier: the case's own hook (both answers per cell,
+            // for an incoming marker false and true) or its rule table
+            DBuf<uint8_t> hookOut;
+            if (sim.hooks.refinement) {
+                BCHK(hookOut.alloc(B.N > 0 ? B.N : 1));
+                lbm3_info info = Il;
+                if (sim.hooks.refinement(sim.hooks.user, &info, v.i, v.j, v.k, B.N, hookOut.p, (void *)s) != 0)
+                    return set_error(LBM3_E_ARG, "the getRefinementModifier hook callback failed");
+                BCHK(hipGetLastError());
+            }
+            const uint8_t *ho = hookOut.p;
             for_n(B.N, s, [=] __device__(long long c) {
                 const float x = (float)v.i[c] * Il.res + Il.ox, y = (float)v.j[c] * Il.res + Il.oy, z = (float)v.k[c] * Il.res + Il.oz;
-                bool d = refine_hook(*dc, L0, x, y, z, deep[c] != 0);
+                bool d = ho ? ((ho[c] >> (deep[c] ? 1 : 0)) & 1u) != 0 : refine_hook(*dc, L0, x, y, z, deep[c] != 0);
                 if (L0 > 0) d = d && !pif[c];
                 deep[c] = d && keep[c];
             });

@@ -73,6 +73,52 @@ static int refresh_bulk_mask(Sim &s, Level &l)
     return check_hip(hipGetLastError(), "bulk mask");
 }
 
+// the open-boundary records the level's open cells index: the level's own list
+// (built from the rule table or the case's getOpenBC hook), else the rule table
+static int upload_open_records(Sim &s, Level &l)
+{
+    std::vector<OpenBCRecord> rec;
+    if (!l.openBC.empty()) {
+        for (const lbm3_bc &b : l.openBC) rec.push_back(open_record(b));
+    } else {
+        for (int r = 0; r <= s.C.nOpenRules; ++r) rec.push_back(open_record(open_bc(s.C, r)));
+    }
+    for (OpenBCRecord &r : rec) {
+        r.impedance = s.C.openScheme == LBM3_OPEN_IMPEDANCE;
+        r.tolerance = s.C.rhoReflectionTolerance;
+    }
+    if (upload(l.openRecords, rec.data(), (long long)rec.size(), s.stream)) return LBM3_E_HIP;
+    return 0;
+}
+
+int hook_local_bc(Sim &s, int level, LocalBC &bulk, LocalBC (&wb)[32]);   // casehooks.hip
+int hook_initial_fields(Sim &s, int level, DBuf<float> &fields4);
+
+// getLocalBC per (level, wall id): the rule table, or the case's own hook
+static int setup_local_bc(Sim &s, int level)
+{
+    Level &l = s.L[level];
+    LocalBC wb[32];
+    l.wallVel.release();
+    if (s.hooks.local_bc) {
+        if (const int rc = hook_local_bc(s, level, l.bulk, wb)) return rc;
+    } else {
+        l.bulk = local_bc(s.C, level, -1);
+        for (int w = 0; w < 32; ++w) wb[w] = local_bc(s.C, level, w);
+    }
+    set_relax(l.bulk, double(l.info.nu));
+    for (int w = 0; w < 32; ++w) set_relax(wb[w], double(l.info.nu));
+    if (upload(l.wallBC, wb, 32, s.stream)) return LBM3_E_HIP;
+    return 0;
+}
+
+static int settle_open(Sim &s, int L);
+
+// entry points of casehooks.hip into this file
+int upload_open_records_of(Sim &s, Level &l) { return upload_open_records(s, l); }
+int settle_open_level(Sim &s, int level) { return settle_open(s, level); }
+int setup_local_bc_of(Sim &s, int level) { return setup_local_bc(s, level); }
+
 // derived per-level tables: local BC, open-BC records, force accumulators, populations
 int finish_level_setup(Sim &s, int level)
 {
@@ -80,14 +126,7 @@ int finish_level_setup(Sim &s, int level)
     l.geom = LevelGeom{l.info.res, l.info.ox, l.info.oy, l.info.oz, l.info.dtPhys, l.info.cellCountX, l.info.cellCountY,
                        l.info.cellCountZ};
     if (l.N > (1LL << 31) - 1 - LBM3_TILE) return set_error(LBM3_E_MEMORY, "a level exceeds 2^31 cells");
-    l.bulk = local_bc(s.C, level, -1);
-    set_relax(l.bulk, double(l.info.nu));
-    LocalBC wb[32];
-    for (int w = 0; w < 32; ++w) {
-        wb[w] = local_bc(s.C, level, w);
-        set_relax(wb[w], double(l.info.nu));
-    }
-    if (upload(l.wallBC, wb, 32, s.stream)) return LBM3_E_HIP;
+    if (const int rc = setup_local_bc(s, level)) return rc;
     HIPCHK(l.cellRow.alloc(l.N > 0 ? l.N : 1));
     HIPCHK(l.rowTab.alloc(l.nRows > 0 ? l.nRows : 1));
     launch_fast_topo(topo_of(l), l.wallMap.p, nullptr, nullptr, l.cellRow.p, l.rowTab.p, s.stream);
@@ -104,16 +143,11 @@ int finish_level_setup(Sim &s, int level)
     HIPCHK(l.openNb.alloc(2LL * (l.nOpen > 0 ? l.nOpen : 1)));
     launch_open_nb(topo_of(l), l.nOpen, l.openCell.p, l.openNb.p, s.stream);
     HIPCHK(hipGetLastError());
-    std::vector<OpenBCRecord> rec(s.C.nOpenRules + 1);
-    for (int r = 0; r <= s.C.nOpenRules; ++r) {
-        rec[r] = open_record(open_bc(s.C, r));
-        rec[r].impedance = s.C.openScheme == LBM3_OPEN_IMPEDANCE;
-        rec[r].tolerance = s.C.rhoReflectionTolerance;
-    }
-    if (upload(l.openRecords, rec.data(), (long long)rec.size(), s.stream)) return LBM3_E_HIP;
+    if (const int rc = upload_open_records(s, l)) return rc;
+    // radial profiles of the rule table (the case's own hook returns final values)
     OpenProfiles prof{};
     bool profiled = false;
-    for (int r = 0; r < s.C.nOpenRules; ++r) {
+    for (int r = 0; r < s.C.nOpenRules && !s.hooks.open_bc; ++r) {
         prof.p[r] = s.C.openRules[r].profile;
         profiled = profiled || prof.p[r].radius > 0.f;
     }
@@ -210,12 +244,13 @@ static int prepare_open_fusion(Sim &s, int L)
         HIPCHK(hipMemcpyAsync(cell.data(), l.openCell.p, sizeof(int) * cell.size(), hipMemcpyDeviceToHost, s.stream));
         HIPCHK(hipMemcpyAsync(meta.data(), l.openMeta.p, sizeof(uint32_t) * meta.size(), hipMemcpyDeviceToHost, s.stream));
         HIPCHK(hipStreamSynchronize(s.stream));
-        const int G = open_group_cells((int)storage_bytes(s.storage));
+        const int G = s.tune.openGroup ? s.tune.openGroup : open_group_cells((int)storage_bytes(s.storage));
+        l.openGroupCells = G;
         std::vector<std::pair<int, int>> first((size_t)(l.N / G + 1), {-1, 0});   // {first open cell, key}
         for (int x = 0; x < l.nOpen; ++x) {
             const int g = cell[(size_t)x] / G;
             if (first[(size_t)g].first < 0 || cell[(size_t)x] < first[(size_t)g].first)
-                first[(size_t)g] = {cell[(size_t)x], int(meta[(size_t)x] & 0x1FFFu)};
+                first[(size_t)g] = {cell[(size_t)x], int(meta[(size_t)x] & ~kOpenTrackBit)};
         }
         std::vector<std::pair<int, int>> order;
         for (size_t g = 0; g < first.size(); ++g)
@@ -326,9 +361,9 @@ template <class S> static void step_level(Sim &s, int level)
         // step's boundary condition when it is still pending
         const OpenArgs<S> o = open_args<S>(s, l, l.info.iterationsFinished - 1);
         a.open = {l.openGroups.p, l.nOpenGroups, l.openSettled ? 0 : 1, o.meta, o.records, o.dRhoRef, o.uNormalRef, o.dRhoCum,
-                  o.uNormalCum, o.iterationsFinished, o.scale};
+                  o.uNormalCum, o.iterationsFinished, o.scale, l.openGroupCells};
     }
-    if (timed || (l.W == 0 && a.open.count == 0 && a.rotorCellCount == 0)) {
+    if (timed || !s.tune.overlap || (l.W == 0 && a.open.count == 0 && a.rotorCellCount == 0)) {
         // in order on one stream, so that each kernel's events bracket it alone
         launch_collide_stream<S>(a, flip, s.stream);
         if (timed) (void)hipEventRecord(s.ev[s.evUsed][1], s.stream);
@@ -399,6 +434,11 @@ template <class S> static int init_level(Sim &s, int level, const float *fieldsD
 
 static int init_level_any(Sim &s, int level, const float *fieldsDev)
 {
+    DBuf<float> hooked;
+    if (!fieldsDev && s.hooks.initial_condition) {
+        if (const int rc = hook_initial_fields(s, level, hooked)) return rc;
+        fieldsDev = hooked.p;
+    }
     switch (s.storage) {
     case LBM3_STORE_F16: return init_level<__half>(s, level, fieldsDev);
     case LBM3_STORE_BF16: return init_level<__hip_bfloat16>(s, level, fieldsDev);
@@ -660,6 +700,9 @@ int lbm3_import_level(lbm3_sim *h, int level, const lbm3_level_arrays *a)
     Level &l = s.L[level];
     hipStream_t st = s.stream;
     if (const int rc = validate_level(*a)) return rc;
+    if (s.hooks.open_bc || s.hooks.local_bc)
+        return set_error(LBM3_E_STATE, "imported levels take the case's rule tables, not hook callbacks");
+    l.openBC.clear();                 // records of the rule table
     l.info = a->info;
     l.N = a->info.cellCount;
     l.nRows = a->nRows;
@@ -691,7 +734,7 @@ int lbm3_import_level(lbm3_sim *h, int level, const lbm3_level_arrays *a)
         const int wm = a->wallMap[cell];
         bool track = wm != -2;
         if (wm >= 0 && (a->wallData[wm] & 1u)) track = false;
-        meta[x] = (uint32_t)normal_index(nx, ny, nz) | ((uint32_t)rule << 5) | ((track ? 1u : 0u) << 13);
+        meta[x] = open_meta(normal_index(nx, ny, nz), rule, track);
     }
     if (upload(l.openCell, a->openIndex, l.nOpen, st) || upload(l.openSource, a->openSource, l.nOpen, st) ||
         upload(l.openMeta, meta.data(), l.nOpen, st))
@@ -783,6 +826,162 @@ int lbm3_set_watchdog(lbm3_sim *h, int period)
     if (!h || period < 0) return set_error(LBM3_E_ARG, "bad watchdog period");
     h->s.watchdog = period;
     return 0;
+}
+
+} // extern "C"
+
+namespace lbm3 {
+
+// the time-step state of every level, kept on the device so that timing runs
+// can be undone: the buffers a checkpoint holds, the levels' step counters and
+// parities, rotor angles and the pending-condition flags
+struct Snapshot {
+    std::vector<DBuf<unsigned char>> bufs;
+    std::vector<lbm3_info> infos;
+    std::vector<std::vector<double>> angles;
+    std::vector<char> settled;
+};
+
+static int take_snapshot(Sim &s, Snapshot &z)
+{
+    z = Snapshot{};
+    for (int L = 0; L < s.levels; ++L) {
+        Level &l = s.L[L];
+        for (const auto &b : checkpoint_buffers(l, s.storage)) {
+            z.bufs.emplace_back();
+            if (b.second == 0 || !b.first) continue;
+            HIPCHK(z.bufs.back().alloc((long long)b.second));
+            HIPCHK(hipMemcpyAsync(z.bufs.back().p, b.first, b.second, hipMemcpyDeviceToDevice, s.stream));
+        }
+        z.infos.push_back(l.info);
+        std::vector<double> a;
+        for (const auto &r : l.rotors) a.push_back(r.radiansElapsed);
+        z.angles.push_back(a);
+        z.settled.push_back(l.openSettled ? 1 : 0);
+    }
+    return check_hip(hipStreamSynchronize(s.stream), "snapshot");
+}
+
+static int restore_snapshot(Sim &s, Snapshot &z)
+{
+    HIPCHK(hipStreamSynchronize(s.aux));
+    size_t k = 0;
+    for (int L = 0; L < s.levels; ++L) {
+        Level &l = s.L[L];
+        for (const auto &b : checkpoint_buffers(l, s.storage)) {
+            DBuf<unsigned char> &d = z.bufs[k++];
+            if (b.second == 0 || !b.first) continue;
+            HIPCHK(hipMemcpyAsync(b.first, d.p, b.second, hipMemcpyDeviceToDevice, s.stream));
+        }
+        l.info.iterationsFinished = z.infos[(size_t)L].iterationsFinished;
+        l.info.iterationsSinceTrackerReset = z.infos[(size_t)L].iterationsSinceTrackerReset;
+        l.info.esotwistFlipper = z.infos[(size_t)L].esotwistFlipper;
+        for (size_t r = 0; r < l.rotors.size(); ++r) l.rotors[r].radiansElapsed = z.angles[(size_t)L][r];
+        l.openSettled = z.settled[(size_t)L] != 0;
+    }
+    return check_hip(hipStreamSynchronize(s.stream), "snapshot restore");
+}
+
+// build every level's open-cell groups again for the current Sim::Tune width
+static int regroup(Sim &s)
+{
+    for (int L = 0; L < s.levels; ++L) {
+        Level &l = s.L[L];
+        if (!l.fusedOpen) continue;
+        if (const int rc = settle_open(s, L)) return rc;
+        l.openFusionChecked = false;
+        if (const int rc = prepare_open_fusion(s, L)) return rc;
+    }
+    return 0;
+}
+
+} // namespace lbm3
+
+extern "C" {
+
+int lbm3_autotune(lbm3_sim *h, int steps, double *out, int cap, int *nOut)
+{
+    if (!h || steps <= 0 || (cap > 0 && !out)) return set_error(LBM3_E_ARG, "bad autotune arguments");
+    Sim &s = h->s;
+    if (const int rc = check_run(s, 0)) return rc;
+    for (int L = 0; L < s.levels; ++L)
+        if (const int rc = settle_open(s, L)) return rc;
+    Snapshot z;
+    if (const int rc = take_snapshot(s, z)) return rc;
+    bool fused = false;
+    for (int L = 0; L < s.levels; ++L) fused = fused || s.L[L].fusedOpen;
+    const int storageGroup = open_group_cells((int)storage_bytes(s.storage));
+    std::vector<int> groups = {storageGroup};
+    if (fused) groups.push_back(storageGroup == 8 ? 16 : 8);
+    const int ldsChoices[4] = {0, 24576, 40960, 65536};
+    const Sim::Tune start = s.tune;
+    Sim::Tune best = start;
+    double bestMs = 1e300;
+    int rows = 0;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    int rc = 0;
+    for (int g : groups) {
+        s.tune.openGroup = g;
+        if ((rc = regroup(s))) break;
+        for (int lds : ldsChoices)
+            for (int remap = 0; remap < 2 && !rc; ++remap)
+                for (int overlap = 1; overlap >= 0 && !rc; --overlap) {
+                    s.tune.bulkLds = lds;
+                    s.tune.xcdRemap = remap;
+                    s.tune.overlap = overlap;
+                    update_all(s, 0);                     // one untimed step in this configuration
+                    if ((rc = check_hip(hipEventRecord(e0, s.stream), "event"))) break;
+                    for (int k = 0; k < steps; ++k) update_all(s, 0);
+                    if ((rc = check_hip(hipEventRecord(e1, s.stream), "event"))) break;
+                    if ((rc = check_hip(hipEventSynchronize(e1), "autotune step"))) break;
+                    float ms = 0.f;
+                    if ((rc = check_hip(hipEventElapsedTime(&ms, e0, e1), "event time"))) break;
+                    const double per = (double)ms / steps;
+                    if (rows < cap) {
+                        double *o = out + 5 * (size_t)rows;
+                        o[0] = lds; o[1] = remap; o[2] = overlap; o[3] = g; o[4] = per;
+                    }
+                    ++rows;
+                    if (per < bestMs) { bestMs = per; best = s.tune; }
+                }
+        if (rc) break;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) {
+        s.tune = start;
+        (void)regroup(s);
+        (void)restore_snapshot(s, z);
+        return rc;
+    }
+    s.tune = best;
+    if ((rc = regroup(s))) return rc;
+    if (nOut) *nOut = rows;
+    return restore_snapshot(s, z);
+}
+
+int lbm3_get_tune(lbm3_sim *h, int *out4)
+{
+    if (!h || !out4) return set_error(LBM3_E_ARG, "null argument");
+    const Sim::Tune &t = h->s.tune;
+    out4[0] = t.bulkLds; out4[1] = t.xcdRemap; out4[2] = t.overlap;
+    out4[3] = t.openGroup ? t.openGroup : open_group_cells((int)storage_bytes(h->s.storage));
+    return 0;
+}
+
+int lbm3_set_tune(lbm3_sim *h, const int *in4)
+{
+    if (!h || !in4) return set_error(LBM3_E_ARG, "null argument");
+    if (in4[0] < 0 || in4[0] > 65536 || (in4[3] != 8 && in4[3] != 16))
+        return set_error(LBM3_E_ARG, "bulk LDS in 0..65536 bytes, open groups of 8 or 16 cells");
+    Sim &s = h->s;
+    s.tune.bulkLds = in4[0];
+    s.tune.xcdRemap = in4[1] != 0;
+    s.tune.overlap = in4[2] != 0;
+    s.tune.openGroup = in4[3];
+    return regroup(s);
 }
 
 int lbm3_run_graph(lbm3_sim *h, int steps)
@@ -940,6 +1139,51 @@ int lbm3_upload_populations(lbm3_sim *h, int level, const void *in, long long by
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(s.stream));
     s.L[level].openSettled = true;   // an uploaded state is complete
+    return 0;
+}
+
+// the device buffer of a named per-level array: false if the name is not one
+static bool level_buffer(Level &l, const char *name, const void *&src, long long &n, size_t &es)
+{
+#define ARR(nm, buf, cnt) else if (!strcmp(name, nm)) { src = (buf).p; n = (cnt); es = sizeof(*(buf).p); }
+    if (false) {}
+    ARR("shifter", l.shifter, l.N) ARR("rowI", l.ri, l.nRows) ARR("rowJ", l.rj, l.nRows) ARR("rowK", l.rk, l.nRows)
+    ARR("rowJPlus", l.rjp, l.nRows) ARR("rowKPlus", l.rkp, l.nRows) ARR("rowJKPlus", l.rjkp, l.nRows)
+    ARR("wallMap", l.wallMap, l.N) ARR("wallIndex", l.wallIndex, l.W) ARR("wallData", l.wallData, l.W)
+    ARR("linkLength", l.linkLength, 7LL * l.W) ARR("periodicIndex", l.perIndex, l.nPeriodic)
+    ARR("periodicSource", l.perSource, l.nPeriodic) ARR("openIndex", l.openCell, l.nOpen)
+    ARR("openSource", l.openSource, l.nOpen) ARR("wallForce", l.gWall, 3LL * l.W)
+    else return false;
+#undef ARR
+    return true;
+}
+
+int lbm3_device_array(lbm3_sim *h, int level, const char *name, void **ptr, long long *count)
+{
+    if (!h || !name || !ptr || !count || level < 0 || level >= h->s.levels) return set_error(LBM3_E_ARG, "bad array arguments");
+    Sim &s = h->s;
+    Level &l = s.L[level];
+    if (!l.built) return set_error(LBM3_E_STATE, "level not built");
+    if (!strcmp(name, "f")) {
+        if (const int rc = settle_open(s, level)) return rc;
+        *ptr = l.f.p;
+        *count = 27LL * padded_cells(l.N);
+        return 0;
+    }
+    const void *src = nullptr;
+    long long n = 0;
+    size_t es = 4;
+    if (!level_buffer(l, name, src, n, es)) return set_error(LBM3_E_ARG, std::string("unknown array ") + name);
+    *ptr = const_cast<void *>(src);
+    *count = n;
+    return 0;
+}
+
+int lbm3_population_layout(lbm3_sim *h, int *tile, int *bytes)
+{
+    if (!h || !tile || !bytes) return set_error(LBM3_E_ARG, "null argument");
+    *tile = LBM3_TILE;
+    *bytes = (int)storage_bytes(h->s.storage);
     return 0;
 }
 

@@ -72,7 +72,8 @@ __device__ __forceinline__ void decode_fast(const FastTopo &t, int cell, int row
 struct LocalBC {
     float gx, gy, gz, limiter, nuMult;
     int useIBB;
-    int velMode;            // 0: wall moves with the fluid's own velocity, 1: fixed, 2: rotation
+    int velMode;            // 0: wall moves with the fluid's own velocity, 1: fixed, 2: rotation,
+                            // 3: per wall-adjacent cell (the level's wallVel array)
     float ux, uy, uz;
     float omega[3], origin[3];
     Relax<float> rf;        // relaxation constants for the level's viscosity
@@ -112,6 +113,7 @@ template <class T> struct FusedOpen {
     double *dRhoCum, *uNormalCum;
     int iter;                     // iteration the pending boundary condition belongs to
     const float *scale;           // per open-list entry: velocity profile factor, or null (none)
+    int groupCells;               // cells per group (the width the groups were built with)
 };
 
 // One rotor of a level (RotorStruct, types.h:206-219) as the kernels read it;
@@ -152,6 +154,7 @@ template <class S> struct StepArgs {
     float nu;
     LocalBC bulk;                 // cells that are not next to a wall
     const LocalBC *wallBC;        // [32] by wallID
+    const float *wallVel;         // [3][W] wall velocity per wall-adjacent cell (velMode 3) or null
     LevelGeom geom;
     Codec cd;                     // 16-bit population encoding
     FusedOpen<typename StoreTraits<S>::compute> open;
@@ -163,6 +166,8 @@ template <class S> struct StepArgs {
     const int *rotorCells;        // [rotorCellCount] cells with their own kernel (record -4)
     int rotorCellCount;
     double rotorCS[2 * LBM3_MAX_ROTORS];   // cos, sin of each rotor's angle this step, in the compute type
+    int xcdRemap;                 // bulk blocks dealt so that each XCD sweeps one contiguous part (else round-robin)
+    int bulkLds;                  // dynamic LDS bytes per bulk block (host side: caps resident bulk blocks per CU)
 };
 
 // Open boundary cells of one level, all BCIDs concatenated (BCID order, then cell order)
@@ -172,7 +177,7 @@ template <class S> struct OpenArgs {
     int count;
     const int *cell;
     const int4 *nb;               // [2*count] the 8 neighbour cells of each open cell (setup-time decode)
-    const uint32_t *meta;         // bits 0-4: normal index, 5-12: record, 13: track flow
+    const uint32_t *meta;         // open_meta: normal index, track flow, record
     const OpenBCRecord *records;
     typename StoreTraits<S>::compute *dRhoRef, *uNormalRef;  // reference state, compute precision
     double *dRhoCum, *uNormalCum; // flow accumulators or null
@@ -191,9 +196,19 @@ template <class S> struct OpenArgs {
 #define LBM3_XCD_REMAP 0
 #endif
 
-__device__ __forceinline__ int xcd_block(int b, int n)
+// Dynamic LDS per bulk block, unused by the kernel: a cap on resident bulk
+// blocks per CU (160 KiB / 40 KiB = 4, i.e. 16 of the 32 wave slots a CU has
+// at this kernel's register use).  Measured on the 256x128x128 sphere: the
+// bulk kernel is 1.5% faster than at full occupancy, and the free slots let
+// the wall-adjacent kernel on the second stream run beside it instead of
+// after it (the whole step 7% faster).  The default of Sim::Tune::bulkLds.
+#ifndef LBM3_BULK_LDS
+#define LBM3_BULK_LDS 40960
+#endif
+
+__device__ __forceinline__ int xcd_block(int b, int n, int remap)
 {
-    if (!LBM3_XCD_REMAP) return b;
+    if (!remap) return b;
     const int per = n / 8;
     if (b >= per * 8) return b;
     return (b % 8) * per + b / 8;

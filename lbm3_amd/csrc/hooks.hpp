@@ -2,6 +2,10 @@
 // getInitialCondition, getOpenBC, getLocalBC) evaluated from the rule tables
 // of lbm3_case.  Host and device.
 #pragma once
+#include <array>
+#include <cstring>
+#include <map>
+#include <vector>
 #include "../../include/lbm3mi_types.h"
 #include "device.hpp"
 
@@ -128,6 +132,31 @@ __host__ __device__ inline bool refine_hook(const lbm3_case &c, int level, float
         if (R.mode == LBM3_REFINE_REMOVE && in) marker = false;
     }
     return marker;
+}
+
+// distinct BCStructs of `calls` in order of first appearance (recs, appended)
+// and the record of every call (idx); equal means bitwise equal
+inline void dedupe_bc(const std::vector<lbm3_bc> &calls, std::vector<int> &idx, std::vector<lbm3_bc> &recs)
+{
+    using Key = std::array<uint32_t, sizeof(lbm3_bc) / 4>;
+    static_assert(sizeof(lbm3_bc) % 4 == 0, "lbm3_bc holds 4-byte fields only");
+    std::map<Key, int> seen;
+    for (size_t r = 0; r < recs.size(); ++r) {
+        Key k;
+        std::memcpy(k.data(), &recs[r], sizeof(lbm3_bc));
+        seen.emplace(k, (int)r);
+    }
+    idx.resize(calls.size());
+    for (size_t x = 0; x < calls.size(); ++x) {
+        Key k;
+        std::memcpy(k.data(), &calls[x], sizeof(lbm3_bc));
+        auto it = seen.find(k);
+        if (it == seen.end()) {
+            it = seen.emplace(k, (int)recs.size()).first;
+            recs.push_back(calls[x]);
+        }
+        idx[x] = it->second;
+    }
 }
 
 } // namespace lbm3

@@ -97,6 +97,8 @@ template <class T> struct IfArgs {
     const int *wallMapC;
     const uint32_t *wallDataC;
     const LocalBC *wallBCC;        // [32] local BC of the coarse level by wall id
+    const float *wallVelC;         // [3][WC] per-cell wall velocity of the coarse level (velMode 3) or null
+    int WC;
     LevelGeom geomC;
     Topology topoC;
     Codec cd;
@@ -220,6 +222,10 @@ __global__ __launch_bounds__(256) void k_leftovers(IfArgs<typename StoreTraits<S
             uw[0] = T((b.omega[1] * pz - b.omega[2] * py) * s);
             uw[1] = T((b.omega[2] * px - b.omega[0] * pz) * s);
             uw[2] = T((b.omega[0] * py - b.omega[1] * px) * s);
+        } else if (b.velMode == 3) {
+            uw[0] = T(a.wallVelC[wm]);
+            uw[1] = T(a.wallVelC[(size_t)a.WC + wm]);
+            uw[2] = T(a.wallVelC[2 * (size_t)a.WC + wm]);
         }
     }
     CellValues<T> r = c0;
@@ -312,6 +318,8 @@ template <class S> void interface_typed(Sim &s, int coarse)
     a.wallMapC = c.wallMap.p;
     a.wallDataC = c.wallData.p;
     a.wallBCC = c.wallBC.p;
+    a.wallVelC = c.wallVel.p;
+    a.WC = c.W;
     a.geomC = c.geom;
     a.topoC = Topology{c.N, c.shifter.p, c.ri.p, c.rj.p, c.rk.p, c.rjp.p, c.rkp.p, c.rjkp.p};
     a.cd = s.codec;

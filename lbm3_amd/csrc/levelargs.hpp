@@ -24,6 +24,9 @@ template <class S> inline StepArgs<S> step_args(Sim &s, Level &l)
     a.nu = l.info.nu;
     a.bulk = l.bulk;
     a.wallBC = l.wallBC.p;
+    a.wallVel = l.wallVel.p;
+    a.xcdRemap = s.tune.xcdRemap;
+    a.bulkLds = s.tune.bulkLds;
     a.geom = l.geom;
     a.cd = s.codec;
     a.open = FusedOpen<typename StoreTraits<S>::compute>{};

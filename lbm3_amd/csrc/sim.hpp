@@ -83,8 +83,13 @@ struct Level {
     bool openSettled = true;         // no boundary condition pending (after init, upload or settle)
     DBuf<int> openGroups;            // aligned cell groups holding open cells (k_open_groups)
     int nOpenGroups = 0;
+    int openGroupCells = 8;          // cells per group of openGroups
     LocalBC bulk{};
     DBuf<LocalBC> wallBC;            // [32]
+    DBuf<float> wallVel;             // [3][W] per-cell wall velocity of velMode 3 wall ids (else empty)
+    // open-boundary condition records the open cells index (open_meta); empty:
+    // the case's rule table and default (imported levels)
+    std::vector<lbm3_bc> openBC;
     DBuf<unsigned char> f;           // [27][N] storage type
     // rotors of the level (buildRotors), device arrays and host-side motion
     struct Rotor {
@@ -117,6 +122,14 @@ struct Sim {
     hipStream_t stream = nullptr;    // every call's work is ordered on this stream
     hipStream_t aux = nullptr;       // wall-adjacent kernel beside the bulk kernel, joined back each step
     hipEvent_t fork = nullptr, join = nullptr;
+    lbm3_hooks hooks{};              // case hooks evaluated by the case (null callbacks: rule tables)
+    // launch choices of the step (lbm3_autotune picks them by timing)
+    struct Tune {
+        int bulkLds = LBM3_BULK_LDS;  // dynamic LDS per bulk block: caps resident bulk blocks per CU
+        int xcdRemap = LBM3_XCD_REMAP;
+        int overlap = 1;              // wall / open / rotor kernels on the second stream beside the bulk kernel
+        int openGroup = 0;            // cells per open-boundary group (0: open_group_cells of the storage)
+    } tune;
     DBuf<double> partial;            // reduction scratch
     DBuf<double> scalar;
     int watchdog = 0;                // lbm3_run checks for non-finite populations every this many coarse steps (0: off)

@@ -111,7 +111,7 @@ __device__ __forceinline__ void open_fill(T (&f)[27], int x, uint32_t meta, cons
         known[q] = f[q];
         if (unknown_dir(nx, ny, nz, q)) f[q] = T(0);
     }
-    OpenBCRecord R = records[(meta >> 5) & 255u];
+    OpenBCRecord R = records[open_meta_record(meta)];
     if (scale) {                   // the rule's velocity profile at this cell
         const float sc = scale[x];
         R.ux *= sc; R.uy *= sc; R.uz *= sc;
@@ -137,7 +137,7 @@ __device__ __forceinline__ void open_fill(T (&f)[27], int x, uint32_t meta, cons
             uNormalRef[x] = fma(alpha, un - uRef, uRef);
         }
     }
-    if (dRhoCum && ((meta >> 13) & 1u)) {
+    if (dRhoCum && (meta & kOpenTrackBit)) {
         dRhoCum[x] += double(dRho);
         uNormalCum[x] += double(un);
     }
@@ -339,7 +339,7 @@ __device__ __forceinline__ void bulk_cell(const StepArgs<S> &a, int cell)
 template <class S, bool FLIP, int BULK, bool WIDE>
 __global__ __launch_bounds__(256) void k_collide_stream(StepArgs<S> a)
 {
-    const int cell = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+    const int cell = xcd_block(blockIdx.x, gridDim.x, a.xcdRemap) * 256 + threadIdx.x;
     if (cell >= a.fast.N) return;
     if (WIDE) bulk_cell<S, FLIP, BULK, uint64_t>(a, cell);
     else bulk_cell<S, FLIP, BULK, uint32_t>(a, cell);
@@ -489,6 +489,10 @@ __device__ __forceinline__ void wall_cell(const StepArgs<S> &a, int wm)
         wu[0] = T((b.omega[1] * z - b.omega[2] * y) * s);
         wu[1] = T((b.omega[2] * x - b.omega[0] * z) * s);
         wu[2] = T((b.omega[0] * y - b.omega[1] * x) * s);
+    } else if (velMode == 3) {
+        wu[0] = T(a.wallVel[wm]);
+        wu[1] = T(a.wallVel[(size_t)a.W + wm]);
+        wu[2] = T(a.wallVel[2 * (size_t)a.W + wm]);
     }
     Relax<T> R = relax_of<T>(b);
     if (a.nRotors > 0) {
@@ -816,7 +820,7 @@ __global__ __launch_bounds__(256) void k_open_stats(const double *dRhoCum, const
     __shared__ double lds[7 * 4];
     const int i = blockIdx.x * 256 + threadIdx.x;
     double v[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    if (i < count && ((meta[start + i] >> 13) & 1u)) {
+    if (i < count && (meta[start + i] & kOpenTrackBit)) {
         const double dRho = dRhoCum[start + i] * invSpan, un = uNormalCum[start + i] * invSpan;
         const double rho = 1.0 + dRho;
         v[0] = un;
@@ -980,22 +984,13 @@ void launch_collide_wall(const StepArgs<S> &a, bool flip, hipStream_t s)
     else hipLaunchKernelGGL((k_collide_wall<S, false>), dim3(blocks_for(a.W)), dim3(256), 0, s, a);
 }
 
-// Dynamic LDS per bulk block, unused by the kernel: a cap on resident bulk
-// blocks per CU (160 KiB / 40 KiB = 4, i.e. 16 of the 32 wave slots a CU has
-// at this kernel's register use).  Measured on the 256x128x128 sphere: the
-// bulk kernel is 1.5% faster than at full occupancy, and the free slots let
-// the wall-adjacent kernel on the second stream run beside it instead of
-// after it (the whole step 7% faster).
-#ifndef LBM3_BULK_LDS
-#define LBM3_BULK_LDS 40960
-#endif
 
 template <class S>
 void launch_collide_stream(const StepArgs<S> &a, bool flip, hipStream_t s)
 {
     if (a.fast.N <= 0) return;
     const dim3 g(blocks_for(a.fast.N)), b(256);
-    const unsigned lds = LBM3_BULK_LDS;
+    const unsigned lds = (unsigned)a.bulkLds;
     const bool k17 = relax_of<typename StoreTraits<S>::compute>(a.bulk).k17 != 0;
     const bool wide = wide_level<S>(a.fast.N);
 #define LBM3_BULK_LAUNCH(F, K, W) hipLaunchKernelGGL((k_collide_stream<S, F, K, W>), g, b, lds, s, a)
@@ -1017,15 +1012,22 @@ void launch_rotor_cells(const StepArgs<S> &a, bool flip, hipStream_t s)
     else hipLaunchKernelGGL((k_rotor_cells<S, false>), dim3(blocks_for(a.rotorCellCount)), dim3(256), 0, s, a);
 }
 
-template <class S>
-void launch_open_cells(const StepArgs<S> &a, bool flip, hipStream_t s)
+template <class S, int G>
+void launch_open_groups(const StepArgs<S> &a, bool flip, hipStream_t s)
 {
-    if (a.open.count <= 0) return;
-    constexpr int G = open_group_cells(int(sizeof(S)));
     const dim3 g((unsigned)((a.open.count + 256 / G - 1) / (256 / G))), b(256);
     const bool k17 = relax_of<typename StoreTraits<S>::compute>(a.bulk).k17 != 0;
     if (flip) { if (k17) hipLaunchKernelGGL((k_open_groups<S, true, 1, G>), g, b, 0, s, a); else hipLaunchKernelGGL((k_open_groups<S, true, 2, G>), g, b, 0, s, a); }
     else { if (k17) hipLaunchKernelGGL((k_open_groups<S, false, 1, G>), g, b, 0, s, a); else hipLaunchKernelGGL((k_open_groups<S, false, 2, G>), g, b, 0, s, a); }
+}
+
+// the group widths the open-cell groups may be built with (Sim::Tune::openGroup)
+template <class S>
+void launch_open_cells(const StepArgs<S> &a, bool flip, hipStream_t s)
+{
+    if (a.open.count <= 0) return;
+    if (a.open.groupCells == 16) launch_open_groups<S, 16>(a, flip, s);
+    else launch_open_groups<S, 8>(a, flip, s);
 }
 
 // velocity profile factor of each open-list entry: its rule's radial profile
@@ -1039,7 +1041,7 @@ __global__ __launch_bounds__(256) void k_open_scale(Topology t, LevelGeom g, int
     decode(t, cell[x], nb, ijk);
     float px, py, pz;
     cell_xyz(g, ijk, px, py, pz);
-    scale[x] = profile_factor(P.p[(meta[x] >> 5) & 255u], px, py, pz);
+    scale[x] = profile_factor(P.p[open_meta_record(meta[x])], px, py, pz);
 }
 
 void launch_open_scale(const Topology &t, const LevelGeom &g, int count, const int *cell, const uint32_t *meta,

@@ -55,6 +55,14 @@ def load():
         "lbm3_kinetic_energy": (i32, [vp, C.POINTER(f64)]),
         "lbm3_mass": (i32, [vp, C.POINTER(f64)]),
         "lbm3_wall_forces": (i32, [vp, i32, C.POINTER(f64), i32, C.POINTER(f64)]),
+        "lbm3_autotune": (i32, [vp, i32, vp, i32, C.POINTER(i32)]),
+        "lbm3_get_tune": (i32, [vp, vp]),
+        "lbm3_set_tune": (i32, [vp, vp]),
+        "lbm3_set_hooks": (i32, [vp, vp]),
+        "lbm3_set_info_controls": (i32, [vp, i32, C.c_float, C.c_float, i32]),
+        "lbm3_refresh_hooks": (i32, [vp]),
+        "lbm3_device_array": (i32, [vp, i32, C.c_char_p, C.POINTER(vp), C.POINTER(C.c_longlong)]),
+        "lbm3_population_layout": (i32, [vp, C.POINTER(i32), C.POINTER(i32)]),
         "lbm3_reset_wall_forces": (i32, [vp]),
         "lbm3_open_boundary_stats": (i32, [vp, i32, C.POINTER(f64)]),
         "lbm3_reset_open_boundary_stats": (i32, [vp]),
@@ -280,6 +288,25 @@ class Sim:
         out = (C.c_double * 2)()
         check(load().lbm3_mass(self.h, out))
         return out[0], out[1]
+
+    def autotune(self, steps=10):
+        """lbm3_autotune: time every launch choice over `steps` coarse steps,
+        keep the fastest, leave the state as it was.  Returns (chosen
+        {bulk_lds, xcd_remap, overlap, open_group}, [(lds, remap, overlap,
+        group, ms per step), ...])."""
+        out = np.zeros((64, 5))
+        n = C.c_int()
+        check(load().lbm3_autotune(self.h, steps, out.ctypes.data, 64, C.byref(n)))
+        return self.tune(), [tuple(r) for r in out[:min(n.value, 64)]]
+
+    def tune(self):
+        t = np.zeros(4, np.int32)
+        check(load().lbm3_get_tune(self.h, t.ctypes.data))
+        return dict(zip(("bulk_lds", "xcd_remap", "overlap", "open_group"), (int(x) for x in t)))
+
+    def set_tune(self, bulk_lds, xcd_remap, overlap, open_group):
+        t = np.array([bulk_lds, xcd_remap, overlap, open_group], np.int32)
+        check(load().lbm3_set_tune(self.h, t.ctypes.data))
 
     def wall_forces(self, level=0, origin=(0.0, 0.0, 0.0), bodies=1):
         o = (C.c_double * 3)(*origin)

@@ -142,6 +142,7 @@ int orc_stl_edge_counts(const float *tri9, int n, float resGlobal, int *out, int
 orc_sim *orc_sim_create(const lbm3_case *C, int isDouble);
 int orc_build_grids(orc_sim *S, const orc_stl *statics, int nStatic, lbm3_bounds Domain);
 void orc_apply_initial_condition(orc_sim *S, int level);
+int orc_apply_initial_fields(orc_sim *S, int level, const float *fields4);
 void orc_update_all_grids(orc_sim *S, int level);
 void orc_update_single_grid(orc_sim *S, int level);
 void orc_update_interface(orc_sim *S, int coarse);

@@ -65,8 +65,17 @@ orc_sim *orc_sim_create(const lbm3_case *C, int isDouble)
 
 void orc_apply_initial_condition(orc_sim *S, int L)
 {
-    if (S->isDouble) orc_init_level_f64(S, L);
-    else orc_init_level_f32(S, L);
+    if (S->isDouble) orc_init_level_f64(S, L, NULL);
+    else orc_init_level_f32(S, L, NULL);
+}
+
+/* applyInitialCondition with the hook's per-cell (dRho, ux, uy, uz) given */
+int orc_apply_initial_fields(orc_sim *S, int L, const float *fields4)
+{
+    if (L < 0 || L >= S->levels || !fields4) return -1;
+    if (S->isDouble) orc_init_level_f64(S, L, fields4);
+    else orc_init_level_f32(S, L, fields4);
+    return 0;
 }
 
 void orc_update_single_grid(orc_sim *S, int L)

@@ -92,8 +92,9 @@ static void NAME(hook_init)(const lbm3_case *C, const lbm3_info *I, int i, int j
 
 static const lbm3_bc NAME(BC_DEFAULT) = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, -1, 0, 1, 0, 0, 0, 1, 7.f, 0.05f, 1.f};
 
-/* applyInitialCondition, applyInitialCondition.h:8-84 */
-void NAME(orc_init_level)(orc_sim *S, int L)
+/* applyInitialCondition, applyInitialCondition.h:8-84; fields4 (dRho, ux, uy,
+   uz per cell, cell order) stands for the getInitialCondition hook when given */
+void NAME(orc_init_level)(orc_sim *S, int L, const float *fields4)
 {
     orc_grid *g = &S->g[L];
     const int N = g->Info.cellCount;
@@ -102,7 +103,12 @@ void NAME(orc_init_level)(orc_sim *S, int L)
         int ijk[3], nb[8];
         NAME(decode)(g, c, ijk, nb);
         NAME(bcr) BC; NAME(bc_from)(&BC, &NAME(BC_DEFAULT));
-        NAME(hook_init)(&S->C, &g->Info, ijk[0], ijk[1], ijk[2], &BC);
+        if (fields4) {
+            BC.dRho = fields4[4 * (size_t)c]; BC.ux = fields4[4 * (size_t)c + 1];
+            BC.uy = fields4[4 * (size_t)c + 2]; BC.uz = fields4[4 * (size_t)c + 3];
+        } else {
+            NAME(hook_init)(&S->C, &g->Info, ijk[0], ijk[1], ijk[2], &BC);
+        }
         REAL f[27];
         NAME(orc_feq)(BC.dRho, BC.ux, BC.uy, BC.uz, f);
         for (int q = 0; q < 27; ++q) { int cc, s; NAME(pre_index)(nb, flip, q, &cc, &s); F_AT(g, s, cc) = f[q]; }

@@ -42,6 +42,7 @@ def lib():
         L.orc_kinetic_energy.argtypes = [C.c_void_p]
         L.orc_run.argtypes = [C.c_void_p, C.c_int]
         L.orc_init_all.argtypes = [C.c_void_p]
+        L.orc_apply_initial_fields.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
         L.orc_sim_free.argtypes = [C.c_void_p]
         L.orc_update_single_grid.argtypes = [C.c_void_p, C.c_int]
         L.orc_collide_batch_f32.argtypes = [C.c_void_p, C.c_int, C.POINTER(T.BC), C.c_float]
@@ -172,6 +173,12 @@ class OracleSim:
 
     def init(self):
         lib().orc_init_all(self.h)
+
+    def init_fields(self, level, fields):
+        """applyInitialCondition on one level with per-cell (dRho, ux, uy, uz)."""
+        f = np.ascontiguousarray(fields, np.float32).reshape(-1, 4)
+        assert len(f) == self.info(level).cellCount
+        assert lib().orc_apply_initial_fields(self.h, level, f.ctypes.data) == 0
 
     def run(self, steps):
         lib().orc_run(self.h, steps)

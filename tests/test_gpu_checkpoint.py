@@ -85,6 +85,7 @@ def test_fresh_restart_with_a_fused_open_level(tmp_path, storage):
     pending and loaded into a freshly built simulation that has never run;
     the continued runs agree bit for bit."""
     case, dom, tris = cases.sphere(48, 24, 24, diameter=6.0, center=(14.0, 12.0, 12.0), segments=(16, 32))
+    case.trackOpenBoundaries = 1
     a = lib.Sim(case, storage)
     a.build(dom, tris)
     a.init()

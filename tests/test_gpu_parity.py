@@ -142,8 +142,10 @@ def test_sphere_k17_100_steps_f32_within_fp32_sensitivity():
     r, u = _compare(o, g)
     m64, mo = o64.macros(), o.macros()
     sens = rel_max(mo[:, 1:], m64[:, 1:])
-    print("K17 sphere: GPU vs oracle FP32 %.2e, oracle FP32 vs FP64 %.2e" % (u, sens))
+    err64 = rel_max(g.macros()[:, 1:], m64[:, 1:])
+    print("K17 sphere: GPU vs oracle FP32 %.2e; FP32 vs FP64: oracle %.2e, GPU %.2e" % (u, sens, err64))
     assert r < 1e-5 and u < 3 * sens + 1e-6, (r, u, sens)
+    assert err64 < 1.5 * sens + 1e-6, (err64, sens)
 
 
 @pytest.mark.parametrize("k15", [True, False])
@@ -182,9 +184,12 @@ def test_channel_periodic_and_moment_walls_f32_within_fp32_sensitivity():
         x.run(100)
     g.sync()
     r, u = _compare(o, g)
-    sens = rel_max(o.macros()[:, 1:], o64.macros()[:, 1:])
-    print("channel: GPU vs oracle FP32 %.2e, oracle FP32 vs FP64 %.2e" % (u, sens))
+    m64 = o64.macros()
+    sens = rel_max(o.macros()[:, 1:], m64[:, 1:])
+    err64 = rel_max(g.macros()[:, 1:], m64[:, 1:])
+    print("channel: GPU vs oracle FP32 %.2e; FP32 vs FP64: oracle %.2e, GPU %.2e" % (u, sens, err64))
     assert r < 1e-5 and u < 3 * sens + 1e-6, (r, u, sens)
+    assert err64 < 1.5 * sens + 1e-6, (err64, sens)
 
 
 # ---------------------------------------------------------------- multilevel (interface transfer)
@@ -251,8 +256,11 @@ def test_multilevel_f32(name):
     g.sync()
     r, u = _compare_all(o, g)
     sens = max(rel_max(o.macros(L)[:, 1:], o64.macros(L)[:, 1:]) for L in range(o.levels))
-    print("%s FP32 after 50 coarse steps: rel max rho %.2e, u %.2e (oracle FP32 vs FP64 %.2e)" % (name, r, u, sens))
+    err64 = max(rel_max(g.macros(L)[:, 1:], o64.macros(L)[:, 1:]) for L in range(o.levels))
+    print("%s FP32 after 50 coarse steps: rel max rho %.2e, u %.2e; FP32 vs FP64: oracle %.2e, GPU %.2e"
+          % (name, r, u, sens, err64))
     assert r < 1e-5 and u < max(1e-5, 3 * sens + 1e-6), (r, u, sens)
+    assert err64 < 1.5 * sens + 1e-6, (err64, sens)
 
 
 # ---------------------------------------------------------------- 16-bit storage accuracy
@@ -276,3 +284,24 @@ def test_16bit_storage_velocity_error_vs_fp64(storage, bar):
     err = float(np.linalg.norm(a - b) / np.linalg.norm(b))
     print("%s sphere 64x32x32, 1000 steps: rel L2 u error vs FP64 %.2e" % (storage, err))
     assert err < bar, err
+
+
+def test_16bit_multilevel_velocity_error_vs_fp64():
+    """FP16 shifted storage on the three-level sphere: relative L2 velocity
+    error of every level against the FP64 run of the same kernels after 400
+    coarse steps (1600 finest sub-steps)."""
+    case, dom, tris = _small_sphere3()
+    runs = {}
+    for st in ("f16", "f64"):
+        g = lib.Sim(case, st)
+        g.build(dom, tris)
+        g.init()
+        g.run(400)
+        g.sync()
+        runs[st] = [(g.macros(L)[:, 1:].astype(np.float64), g.array("wallMap", L) != -3) for L in range(3)]
+        g.close()
+    errs = []
+    for (u16, fluid), (u64, _) in zip(runs["f16"], runs["f64"]):
+        errs.append(float(np.linalg.norm((u16 - u64)[fluid]) / np.linalg.norm(u64[fluid])))
+    print("FP16 three-level sphere, 400 coarse steps: rel L2 u vs FP64 per level %s" % ["%.2e" % e for e in errs])
+    assert max(errs) < 1e-3, errs

@@ -167,3 +167,26 @@ def test_rotor_duct_two_levels_f64():
     fo, fg = o.rotor_forces(1, 0), g.rotor_forces(1, 0)
     assert np.max(np.abs(fg - fo)) < 1e-9 * np.max(np.abs(fo))
     g.close()
+
+
+def test_seed_rotor_of_the_duct_config_bit_exact():
+    """The rotor-in-a-duct configuration's own rotor (seed 0: 6 blades, about
+    200k triangles) voxelized on the GPU on both levels of a reduced duct
+    (96x48x48 coarse, 2 levels): rotor map, index and corner-count arrays equal
+    the oracle's bit for bit, as do the duct's flags."""
+    case, dom, statics, rotor, info = cases.rotor_duct(96, 48, 48, seed=0, levels=2)
+    assert info["triangles"] > 190000
+    o = OracleSim(case, dom, statics, double=False)
+    o.build_rotors([rotor])
+    g = lib.Sim(case, "f32")
+    g.build(dom, statics)
+    g.build_rotors([rotor])
+    for L in range(2):
+        for name in ("wallMap", "wallData", "linkLength"):
+            assert np.array_equal(g.array(name, L), o.array(name, L)), (L, name)
+        for name in ("rotor0Map", "rotor0Index", "rotor0Interp"):
+            assert np.array_equal(g.array(name, L), o.array(name, L)), (L, name)
+        assert o.rotor_info(L, 0)[11] > 0
+    print("seed-0 rotor: %d triangles, marked blocks per level %s" %
+          (info["triangles"], [int(o.rotor_info(L, 0)[11]) for L in range(2)]))
+    g.close()
