@@ -69,15 +69,15 @@ def measure(storage, steps, warmup, k17=False):
         "boundary_kernels_us": wall_ms / launches * 1e3,
         "kernel_mlups": bulk_cells / avg_kernel_s / 1e6,
         "bulk_cells": bulk_cells, "wall_adjacent": wall_adjacent, "open_cells": kinds["open"],
-        "open_group_cells": kinds["open_group"],
+        "open_group_cells": kinds["open_group"], "open_cells_in_bulk_kernel": kinds["open_in_bulk"],
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None},
         "cells": info.cellCount, "walls": walls, "fluid_updates_per_step": fluid_updates,
         "bytes_per_launch": bytes_per_launch, "build_s": round(build_s, 3),
         "steps_done": warmup + 2 * steps,
         "tune": {"chosen": tuned, "candidates": len(table),
-                 "ms_per_step_best": round(min(r[4] for r in table), 4),
-                 "ms_per_step_worst": round(max(r[4] for r in table), 4)},
+                 "ms_per_step_best": round(min(r[5] for r in table), 4),
+                 "ms_per_step_worst": round(max(r[5] for r in table), 4)},
     }
     u = sim.macros(0)[:, 1:].astype(np.float64)
     u[sim.array("wallMap") == -3] = np.nan     # solid cells hold no flow
@@ -140,11 +140,12 @@ def measure_multilevel(storage, steps=200, warmup=20):
     build_s = time.time() - t0
     sim.init()
     sim.run(warmup)
+    tuned, _ = sim.autotune(steps=5)
     sim.sync()
     total_ms = sim.run_timed(steps)[0]
     out = {"storage": storage, "mlups": round(fu * steps / (total_ms / 1e3) / 1e6, 1),
            "ms_per_coarse_step": round(total_ms / steps, 4), "fluid_updates_per_coarse_step": fu,
-           "cells_per_level": [sim.info(L).cellCount for L in range(3)], "build_s": round(build_s, 3)}
+           "cells_per_level": [sim.info(L).cellCount for L in range(3)], "build_s": round(build_s, 3), "tune": tuned}
     sim.close()
     return out
 
@@ -193,6 +194,7 @@ def measure_rotor(storage, windows=2, window=100, warmup=20):
     build_s = time.time() - t0
     sim.init()
     sim.run(warmup)
+    tuned, _ = sim.autotune(steps=5)
     sim.sync()
     torque, ms = [], 0.0
     for _ in range(windows):
@@ -204,7 +206,7 @@ def measure_rotor(storage, windows=2, window=100, warmup=20):
            "ms_per_coarse_step": round(ms / (windows * window), 4), "fluid_updates_per_coarse_step": fu,
            "rotor": {"blades": ri["blades"], "twist_deg": round(ri["twist_deg"], 2), "triangles": ri["triangles"],
                      "rotor_cells_fine": kinds["rotor"]},
-           "torque_x_per_window": torque, "window_coarse_steps": window, "build_s": round(build_s, 3)}
+           "torque_x_per_window": torque, "window_coarse_steps": window, "build_s": round(build_s, 3), "tune": tuned}
     sim.close()
     return out
 
@@ -423,7 +425,9 @@ def main():
         "kernel_mlups": round(main_run["kernel_mlups"], 1),
         "bulk_cells": main_run["bulk_cells"], "wall_adjacent": main_run["wall_adjacent"],
         "open_cells": main_run["open_cells"], "open_group_cells": main_run["open_group_cells"],
+        "open_cells_in_bulk_kernel": main_run["open_cells_in_bulk_kernel"],
         "variants": {k: {"mlups": round(v["mlups"], 1), "ms_per_step": round(v["ms_per_step"], 4),
+                         "tune": v["tune"]["chosen"], "bulk_cells": v["bulk_cells"],
                          "kernel_us": round(v["kernel_us"], 2), "boundary_kernels_us": round(v["boundary_kernels_us"], 2),
                          "roofline": v["roofline"],
                          "speedup_vs_f32": round(v["mlups"] / main_run["mlups"], 3)} for k, v in variants.items()},

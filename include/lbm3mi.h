@@ -116,28 +116,40 @@ int lbm3_set_watchdog(lbm3_sim *s, int period);
  * sequence of two coarse steps (every level's esotwist parity back where it
  * was) is captured once per call and replayed steps / 2 times, an odd last
  * step launched directly; returns when the work is done.  Results are those of
- * lbm3_run bit for bit.  For simulations without open boundaries or rotors
- * (their kernels take per-step arguments: LBM3_E_STATE otherwise), e.g.
- * periodic boxes, channels and multilevel Taylor-Green grids, where many
- * short launches per coarse step make launch overhead count. */
+ * lbm3_run bit for bit.  The values that change from replay to replay -- the
+ * iteration an open-boundary condition belongs to, whether a deferred one is
+ * pending, the rotors' angles -- are read by the captured kernels from a small
+ * device table refilled before each replay. */
 int lbm3_run_graph(lbm3_sim *s, int steps);
 int lbm3_synchronize(lbm3_sim *s);
 /* Autotuner of the step's launch choices (no counterpart in the reference):
  * times `steps` whole coarse steps (after one untimed step) for every
  * combination of the bulk kernel's occupancy cap (dynamic LDS per block 0,
  * 24, 40 or 64 KiB), its block order (round-robin over the XCDs or one
- * contiguous part per XCD), the side kernels' placement (second stream beside
- * the bulk kernel or in order after it) and, where a level completes its open
- * boundary in the open-cell kernel, that kernel's group width (8 or 16 cells),
- * keeps the fastest, and restores the simulation's state as it was (the
- * populations and every step counter; a pending open-boundary condition is
- * applied first).  out[5 r + {LDS bytes, XCD order, side stream, group cells,
- * ms per step}] for the first `cap` candidates; *nOut the candidates timed. */
+ * contiguous part per XCD), the side kernels' placement (in order after the
+ * bulk kernel: 0, on a second stream beside it: 1, and with several levels
+ * also each finer level's sub-cycle on a stream of its own beside its coarser
+ * level's step: 2) and, where a level completes its open boundary when it next
+ * loads the open cells, the width of the aligned cell groups the open-cell
+ * kernel takes (8 or 16 cells; 1: no groups, the bulk kernel completes and
+ * steps the open cells itself) and whether the cells on the x faces are
+ * instead stepped by the bulk kernel and completed by a separate pass after
+ * streaming; keeps the fastest and restores the
+ * simulation's state as it was (the populations and every step counter; a
+ * pending open-boundary condition stays pending unless the chosen open-cell
+ * layout differs, which completes it first).  out[6 r + {LDS bytes,
+ * XCD order, side stream, group cells, x faces separate, ms per step}] for the
+ * first `cap` candidates; *nOut the candidates timed. */
 int lbm3_autotune(lbm3_sim *s, int steps, double *out, int cap, int *nOut);
-/* The launch choices in use: {bulk LDS bytes, XCD order, side stream, open
- * group cells}; lbm3_set_tune sets them (LDS 0..65536, group 8 or 16). */
-int lbm3_get_tune(lbm3_sim *s, int *out4);
-int lbm3_set_tune(lbm3_sim *s, const int *in4);
+/* The launch choices in use: {bulk LDS bytes, XCD order, side streams, open
+ * group cells, x faces separate}; lbm3_set_tune sets them (LDS 0..65536, side
+ * streams 0..2, group 1, 8 or 16; a new group width or x-face choice
+ * completes a pending open-boundary condition first).  Every choice gives the
+ * same results bit for bit, except with 16-bit storage: that completion, and
+ * every step's separate pass for the x faces, store the completed populations
+ * rounded to the storage type. */
+int lbm3_get_tune(lbm3_sim *s, int *out5);
+int lbm3_set_tune(lbm3_sim *s, const int *in5);
 
 /* Timed run: `steps` coarse steps between HIP events on the sim's stream.
  * out[0] = total ms; with kernelEvents != 0 also out[1] = summed ms of the

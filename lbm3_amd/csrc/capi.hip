@@ -69,7 +69,8 @@ static int refresh_bulk_mask(Sim &s, Level &l)
     const long long blocks = (l.N + 255) / 256;
     if (l.tileRow.n < blocks) HIPCHK(l.tileRow.alloc(blocks > 0 ? blocks : 1));
     if (l.bulkSkip.n < 4 * blocks) HIPCHK(l.bulkSkip.alloc(blocks > 0 ? 4 * blocks : 4));
-    launch_bulk_mask(l.cellRow.p, l.N, l.bulkSkip.p, l.tileRow.p, s.stream);
+    if (l.openInBulk && l.openBits.n < 4 * blocks) HIPCHK(l.openBits.alloc(blocks > 0 ? 4 * blocks : 4));
+    launch_bulk_mask(l.cellRow.p, l.N, l.bulkSkip.p, l.tileRow.p, l.openInBulk ? l.openBits.p : nullptr, s.stream);
     return check_hip(hipGetLastError(), "bulk mask");
 }
 
@@ -139,6 +140,7 @@ int finish_level_setup(Sim &s, int level)
     l.rotorCellCount = 0;
     l.openFusionChecked = false;
     l.fusedOpen = false;
+    l.openInBulk = false;
     l.openSettled = true;
     HIPCHK(l.openNb.alloc(2LL * (l.nOpen > 0 ? l.nOpen : 1)));
     launch_open_nb(topo_of(l), l.nOpen, l.openCell.p, l.openNb.p, s.stream);
@@ -191,6 +193,7 @@ static int prepare_open_fusion(Sim &s, int L)
     if (const int rc = settle_open(s, L)) return rc;   // a condition pending under the old records
     l.openFusionChecked = true;
     l.fusedOpen = false;
+    l.openInBulk = false;
     l.openSettled = true;
     l.rotorCellCount = 0;
     const uint8_t *cand = nullptr;
@@ -233,10 +236,41 @@ static int prepare_open_fusion(Sim &s, int L)
         // the open-cell kernel uses 32-bit buffer offsets
         fuse = conflicts == 0 && !wide_of(s, l);
     }
+    l.nFusedEntries = l.nSepEntries = 0;
+    l.fusedEntries.release();
+    l.sepEntries.release();
+    if (fuse && s.tune.openSplit == 1) {
+        // cells on the x faces (one per row end: every access of theirs is a
+        // line of its own) leave the open-cell kernel: the bulk kernel steps
+        // them and a pass after streaming completes them
+        std::vector<uint32_t> meta((size_t)l.nOpen);
+        HIPCHK(hipMemcpyAsync(meta.data(), l.openMeta.p, sizeof(uint32_t) * meta.size(), hipMemcpyDeviceToHost, s.stream));
+        HIPCHK(hipStreamSynchronize(s.stream));
+        std::vector<int> fusedList, sepList;
+        for (int x = 0; x < l.nOpen; ++x)
+            (LBM3_BT_NORMAL[meta[(size_t)x] & 31u][0] != 0 ? sepList : fusedList).push_back(x);
+        if (!sepList.empty()) {
+            if (upload(l.fusedEntries, fusedList.data(), (long long)fusedList.size(), s.stream) ||
+                upload(l.sepEntries, sepList.data(), (long long)sepList.size(), s.stream))
+                return LBM3_E_HIP;
+            l.nFusedEntries = (int)fusedList.size();
+            l.nSepEntries = (int)sepList.size();
+            HIPCHK(hipMemsetAsync(openOf.p, 0xFF, sizeof(int) * (size_t)l.N, s.stream));
+            launch_scatter_entries(l.openCell.p, l.fusedEntries.p, l.nFusedEntries, openOf.p, s.stream);
+        }
+    }
     launch_fast_topo(topo_of(l), l.wallMap.p, fuse ? openOf.p : nullptr, cand, l.cellRow.p, l.rowTab.p, s.stream);
     HIPCHK(hipGetLastError());
     l.nOpenGroups = 0;
-    if (fuse) {
+    int G = s.tune.openGroup ? s.tune.openGroup : open_group_cells((int)storage_bytes(s.storage));
+    l.openInBulk = fuse && G == 1;   // the bulk kernel completes and steps the open cells
+    // the bulk kernel's open cells but those on the x faces, which stay in
+    // groups of the open-cell kernel (one x-face cell per row end would hold a
+    // bulk wave for one lane; eight in a group share one wave's condition code)
+    const bool xGroups = l.openInBulk && s.tune.openSplit == 2;
+    if (xGroups) G = open_group_cells((int)storage_bytes(s.storage));
+    l.openGroupCells = G;
+    if (fuse && (!l.openInBulk || xGroups)) {
         // aligned groups of cells holding open cells, sorted by the boundary
         // condition (normal and record) of their first open cell, then by group
         std::vector<int> cell((size_t)l.nOpen);
@@ -244,10 +278,17 @@ static int prepare_open_fusion(Sim &s, int L)
         HIPCHK(hipMemcpyAsync(cell.data(), l.openCell.p, sizeof(int) * cell.size(), hipMemcpyDeviceToHost, s.stream));
         HIPCHK(hipMemcpyAsync(meta.data(), l.openMeta.p, sizeof(uint32_t) * meta.size(), hipMemcpyDeviceToHost, s.stream));
         HIPCHK(hipStreamSynchronize(s.stream));
-        const int G = s.tune.openGroup ? s.tune.openGroup : open_group_cells((int)storage_bytes(s.storage));
-        l.openGroupCells = G;
+        std::vector<char> inKernel((size_t)l.nOpen, 1);   // entries the open-cell kernel completes
+        if (l.nSepEntries) {
+            std::vector<int> sep((size_t)l.nSepEntries);
+            HIPCHK(hipMemcpy(sep.data(), l.sepEntries.p, sizeof(int) * sep.size(), hipMemcpyDeviceToHost));
+            for (int x : sep) inKernel[(size_t)x] = 0;
+        }
+        if (xGroups)
+            for (int x = 0; x < l.nOpen; ++x) inKernel[(size_t)x] = LBM3_BT_NORMAL[meta[(size_t)x] & 31u][0] != 0;
         std::vector<std::pair<int, int>> first((size_t)(l.N / G + 1), {-1, 0});   // {first open cell, key}
         for (int x = 0; x < l.nOpen; ++x) {
+            if (!inKernel[(size_t)x]) continue;
             const int g = cell[(size_t)x] / G;
             if (first[(size_t)g].first < 0 || cell[(size_t)x] < first[(size_t)g].first)
                 first[(size_t)g] = {cell[(size_t)x], int(meta[(size_t)x] & ~kOpenTrackBit)};
@@ -275,6 +316,11 @@ static int prepare_open_fusion(Sim &s, int L)
         if (upload(l.rotorCells, cells.data(), (long long)cells.size(), s.stream)) return LBM3_E_HIP;
     }
     if (const int rc = refresh_bulk_mask(s, l)) return rc;
+    if (xGroups) {
+        // the grouped cells leave the bulk kernel: skip bits set, open bits clear
+        launch_group_bits(l.openGroups.p, l.nOpenGroups, G, l.bulkSkip.p, l.openBits.p, s.stream);
+        HIPCHK(hipGetLastError());
+    }
     HIPCHK(hipStreamSynchronize(s.stream));
     l.fusedOpen = fuse;
     return 0;
@@ -296,7 +342,12 @@ int upload_rotors(Sim &s, int level)
 // apply a pending open boundary condition in place (before anything reads the level's state)
 template <class S> static void settle_typed(Sim &s, Level &l)
 {
-    launch_open<S>(open_args<S>(s, l, l.info.iterationsFinished - 1), l.info.esotwistFlipper != 0, s.stream);
+    OpenArgs<S> o = open_args<S>(s, l, l.info.iterationsFinished - 1);
+    if (l.fusedOpen && l.nSepEntries) {   // the x-face entries completed in their own step
+        o.list = l.fusedEntries.p;
+        o.count = l.nFusedEntries;
+    }
+    launch_open<S>(o, l.info.esotwistFlipper != 0, s.stream);
 }
 
 // A condition is pending after a fused step and after a checkpoint load that
@@ -363,6 +414,23 @@ template <class S> static void step_level(Sim &s, int level)
         a.open = {l.openGroups.p, l.nOpenGroups, l.openSettled ? 0 : 1, o.meta, o.records, o.dRhoRef, o.uNormalRef, o.dRhoCum,
                   o.uNormalCum, o.iterationsFinished, o.scale, l.openGroupCells};
     }
+    // the values a replayed graph reads from device memory instead
+    StepParam prm{};
+    prm.iter = l.info.iterationsFinished - 1;
+    prm.iterNow = l.info.iterationsFinished;
+    prm.pending = l.fusedOpen && !l.openSettled ? 1 : 0;
+    for (int r = 0; r < 2 * LBM3_MAX_ROTORS; ++r) prm.cs[r] = a.rotorCS[r];
+    if (s.dryRun) {
+        // bookkeeping only: the state a launch would leave
+        s.dryRun->push_back(prm);
+        l.info.esotwistFlipper = flip ? 0 : 1;
+        if (l.fusedOpen) l.openSettled = false;
+        l.info.iterationsSinceTrackerReset++;
+        l.info.iterationsFinished++;
+        return;
+    }
+    const StepParam *slot = s.captureTable ? s.captureTable + s.captureSlot++ : nullptr;
+    a.param = slot;
     if (timed || !s.tune.overlap || (l.W == 0 && a.open.count == 0 && a.rotorCellCount == 0)) {
         // in order on one stream, so that each kernel's events bracket it alone
         launch_collide_stream<S>(a, flip, s.stream);
@@ -376,20 +444,40 @@ template <class S> static void step_level(Sim &s, int level)
         }
     } else {
         // the kernels touch disjoint cells: run the small, latency-bound wall and
-        // open-cell kernels on the second stream beside the bulk kernel
+        // open-cell kernels on the second stream beside the bulk kernel, the
+        // rotor cells (often the larger part of a rotor's level) on a third
         (void)hipEventRecord(s.fork, s.stream);
         (void)hipStreamWaitEvent(s.aux, s.fork, 0);
+        const bool third = a.rotorCellCount > 0 && s.aux2;
+        if (third) {
+            (void)hipStreamWaitEvent(s.aux2, s.fork, 0);
+            launch_rotor_cells<S>(a, flip, s.aux2);
+            (void)hipEventRecord(s.join2, s.aux2);
+        }
         launch_collide_wall<S>(a, flip, s.aux);
         launch_open_cells<S>(a, flip, s.aux);
-        launch_rotor_cells<S>(a, flip, s.aux);
+        if (!third) launch_rotor_cells<S>(a, flip, s.aux);
         (void)hipEventRecord(s.join, s.aux);
         launch_collide_stream<S>(a, flip, s.stream);
         (void)hipStreamWaitEvent(s.stream, s.join, 0);
+        if (third) (void)hipStreamWaitEvent(s.stream, s.join2, 0);
     }
     const bool nflip = !flip;
     l.info.esotwistFlipper = nflip ? 1 : 0;
-    if (l.fusedOpen) l.openSettled = false;   // this step's boundary condition is applied by the next step's open-cell kernel
-    else launch_open<S>(open_args<S>(s, l, l.info.iterationsFinished), nflip, s.stream);
+    if (l.fusedOpen) {
+        l.openSettled = false;   // this step's boundary condition is applied by the next step's open-cell kernel
+        if (l.nSepEntries) {     // except on the x faces: completed now
+            OpenArgs<S> o = open_args<S>(s, l, l.info.iterationsFinished);
+            o.param = slot;
+            o.list = l.sepEntries.p;
+            o.count = l.nSepEntries;
+            launch_open<S>(o, nflip, s.stream);
+        }
+    } else {
+        OpenArgs<S> o = open_args<S>(s, l, l.info.iterationsFinished);
+        o.param = slot;
+        launch_open<S>(o, nflip, s.stream);
+    }
     launch_periodic<S>(reinterpret_cast<S *>(l.f.p), topo_of(l), l.nPeriodic, l.perIndex.p, l.perSource.p, nflip, s.stream);
     l.info.iterationsSinceTrackerReset++;
     l.info.iterationsFinished++;
@@ -407,12 +495,29 @@ static void step_level_any(Sim &s, int level)
 
 void update_interface(Sim &s, int coarse);    // interface.hip
 
+// updateAllGrids (updateGrid.h:388-397): the level's step, two sub-steps of
+// every finer level, the interface transfer.  With Sim::Tune::overlap 2 the
+// finer levels run on their own stream beside this level's step: they depend
+// on this stream's earlier work (the last transfer), and the transfer waits
+// for them.  s.stream is the current level's stream throughout.
 static void update_all(Sim &s, int level)
 {
+    const bool fork = !s.dryRun && s.tune.overlap >= 2 && level + 1 < s.levels && s.levelStream[level + 1];
+    const hipStream_t own = s.stream;
+    if (fork) {
+        (void)hipEventRecord(s.levelFork[level + 1], own);
+        (void)hipStreamWaitEvent(s.levelStream[level + 1], s.levelFork[level + 1], 0);
+    }
     step_level_any(s, level);
     if (level < s.levels - 1) {
+        if (fork) s.stream = s.levelStream[level + 1];
         for (int i = 0; i < 2; ++i) update_all(s, level + 1);
-        update_interface(s, level);
+        if (fork) {
+            (void)hipEventRecord(s.levelJoin[level + 1], s.stream);
+            s.stream = own;
+            (void)hipStreamWaitEvent(own, s.levelJoin[level + 1], 0);
+        }
+        if (!s.dryRun) update_interface(s, level);
     }
 }
 
@@ -505,12 +610,13 @@ static const char kCheckpointMagic[8] = {'L', 'B', 'M', '3', 'C', 'K', 'P', 'T'}
 static constexpr int kCheckpointVersion = 1;
 struct CheckpointLevel {
     long long N, W, nOpen, nRotors;
-    int openSettled, pad;
+    int openSettled;
+    int split;          // a pending condition covers the open-cell kernel's entries only (x faces completed)
 };
 
 static CheckpointLevel checkpoint_level(const Level &l)
 {
-    return {l.N, l.W, l.nOpen, (long long)l.rotors.size(), l.openSettled ? 1 : 0, 0};
+    return {l.N, l.W, l.nOpen, (long long)l.rotors.size(), l.openSettled ? 1 : 0, l.fusedOpen && l.nSepEntries ? 1 : 0};
 }
 
 // the device state a time step changes: populations, open-boundary reference
@@ -582,7 +688,21 @@ int lbm3_sim_create(const lbm3_case *c, int storage, lbm3_sim **out)
     if (!rc) rc = check_hip(hipStreamCreateWithFlags(&h->s.aux, hipStreamNonBlocking), "stream");
     if (!rc) rc = check_hip(hipEventCreateWithFlags(&h->s.fork, hipEventDisableTiming), "event");
     if (!rc) rc = check_hip(hipEventCreateWithFlags(&h->s.join, hipEventDisableTiming), "event");
+    if (!rc) rc = check_hip(hipStreamCreateWithFlags(&h->s.aux2, hipStreamNonBlocking), "stream");
+    if (!rc) rc = check_hip(hipEventCreateWithFlags(&h->s.join2, hipEventDisableTiming), "event");
+    for (int L = 1; L < h->s.levels && !rc; ++L) {
+        rc = check_hip(hipStreamCreateWithFlags(&h->s.levelStream[L], hipStreamNonBlocking), "stream");
+        if (!rc) rc = check_hip(hipEventCreateWithFlags(&h->s.levelFork[L], hipEventDisableTiming), "event");
+        if (!rc) rc = check_hip(hipEventCreateWithFlags(&h->s.levelJoin[L], hipEventDisableTiming), "event");
+    }
     if (rc) {
+        for (int L = 1; L < LBM3_MAX_LEVELS; ++L) {
+            if (h->s.levelJoin[L]) (void)hipEventDestroy(h->s.levelJoin[L]);
+            if (h->s.levelFork[L]) (void)hipEventDestroy(h->s.levelFork[L]);
+            if (h->s.levelStream[L]) (void)hipStreamDestroy(h->s.levelStream[L]);
+        }
+        if (h->s.join2) (void)hipEventDestroy(h->s.join2);
+        if (h->s.aux2) (void)hipStreamDestroy(h->s.aux2);
         if (h->s.join) (void)hipEventDestroy(h->s.join);
         if (h->s.fork) (void)hipEventDestroy(h->s.fork);
         if (h->s.aux) (void)hipStreamDestroy(h->s.aux);
@@ -599,6 +719,17 @@ int lbm3_sim_destroy(lbm3_sim *h)
     if (!h) return 0;
     (void)hipStreamSynchronize(h->s.stream);
     (void)hipStreamSynchronize(h->s.aux);
+    (void)hipStreamSynchronize(h->s.aux2);
+    (void)hipEventDestroy(h->s.join2);
+    (void)hipStreamDestroy(h->s.aux2);
+    for (int L = 1; L < LBM3_MAX_LEVELS; ++L) {
+        if (h->s.levelStream[L]) {
+            (void)hipStreamSynchronize(h->s.levelStream[L]);
+            (void)hipStreamDestroy(h->s.levelStream[L]);
+        }
+        if (h->s.levelFork[L]) (void)hipEventDestroy(h->s.levelFork[L]);
+        if (h->s.levelJoin[L]) (void)hipEventDestroy(h->s.levelJoin[L]);
+    }
     for (auto &e3 : h->s.ev)
         for (auto e : e3) (void)hipEventDestroy(e);
     (void)hipEventDestroy(h->s.fork);
@@ -865,6 +996,7 @@ static int take_snapshot(Sim &s, Snapshot &z)
 static int restore_snapshot(Sim &s, Snapshot &z)
 {
     HIPCHK(hipStreamSynchronize(s.aux));
+    HIPCHK(hipStreamSynchronize(s.aux2));
     size_t k = 0;
     for (int L = 0; L < s.levels; ++L) {
         Level &l = s.L[L];
@@ -882,12 +1014,19 @@ static int restore_snapshot(Sim &s, Snapshot &z)
     return check_hip(hipStreamSynchronize(s.stream), "snapshot restore");
 }
 
+// the open-cell layout a set of launch choices gives: group width and split
+static bool same_open_layout(const Sim &s, const Sim::Tune &a, const Sim::Tune &b)
+{
+    const int g = open_group_cells((int)storage_bytes(s.storage));
+    return (a.openGroup ? a.openGroup : g) == (b.openGroup ? b.openGroup : g) && a.openSplit == b.openSplit;
+}
+
 // build every level's open-cell groups again for the current Sim::Tune width
 static int regroup(Sim &s)
 {
     for (int L = 0; L < s.levels; ++L) {
         Level &l = s.L[L];
-        if (!l.fusedOpen) continue;
+        if (!l.fusedOpen || !l.built) continue;
         if (const int rc = settle_open(s, L)) return rc;
         l.openFusionChecked = false;
         if (const int rc = prepare_open_fusion(s, L)) return rc;
@@ -904,15 +1043,22 @@ int lbm3_autotune(lbm3_sim *h, int steps, double *out, int cap, int *nOut)
     if (!h || steps <= 0 || (cap > 0 && !out)) return set_error(LBM3_E_ARG, "bad autotune arguments");
     Sim &s = h->s;
     if (const int rc = check_run(s, 0)) return rc;
-    for (int L = 0; L < s.levels; ++L)
-        if (const int rc = settle_open(s, L)) return rc;
+    // the state as it is, a pending open condition included: restored in the
+    // layout it was taken in, so that an unchanged layout continues bit for bit
     Snapshot z;
     if (const int rc = take_snapshot(s, z)) return rc;
     bool fused = false;
     for (int L = 0; L < s.levels; ++L) fused = fused || s.L[L].fusedOpen;
     const int storageGroup = open_group_cells((int)storage_bytes(s.storage));
-    std::vector<int> groups = {storageGroup};
-    if (fused) groups.push_back(storageGroup == 8 ? 16 : 8);
+    // (group width, x faces in a separate pass) layouts of the open cells
+    std::vector<std::pair<int, int>> groups = {{storageGroup, 0}};
+    if (fused) {
+        groups.push_back({storageGroup == 8 ? 16 : 8, 0});
+        groups.push_back({1, 0});                     // the open cells in the bulk kernel
+        groups.push_back({1, 2});                     // ... but those on the x faces, in groups
+        groups.push_back({storageGroup, 1});
+        groups.push_back({storageGroup == 8 ? 16 : 8, 1});
+    }
     const int ldsChoices[4] = {0, 24576, 40960, 65536};
     const Sim::Tune start = s.tune;
     Sim::Tune best = start;
@@ -922,12 +1068,14 @@ int lbm3_autotune(lbm3_sim *h, int steps, double *out, int cap, int *nOut)
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
     int rc = 0;
-    for (int g : groups) {
+    for (const auto &gs : groups) {
+        const int g = gs.first;
         s.tune.openGroup = g;
+        s.tune.openSplit = gs.second;
         if ((rc = regroup(s))) break;
         for (int lds : ldsChoices)
             for (int remap = 0; remap < 2 && !rc; ++remap)
-                for (int overlap = 1; overlap >= 0 && !rc; --overlap) {
+                for (int overlap = s.levels > 1 ? 2 : 1; overlap >= 0 && !rc; --overlap) {
                     s.tune.bulkLds = lds;
                     s.tune.xcdRemap = remap;
                     s.tune.overlap = overlap;
@@ -940,8 +1088,8 @@ int lbm3_autotune(lbm3_sim *h, int steps, double *out, int cap, int *nOut)
                     if ((rc = check_hip(hipEventElapsedTime(&ms, e0, e1), "event time"))) break;
                     const double per = (double)ms / steps;
                     if (rows < cap) {
-                        double *o = out + 5 * (size_t)rows;
-                        o[0] = lds; o[1] = remap; o[2] = overlap; o[3] = g; o[4] = per;
+                        double *o = out + 6 * (size_t)rows;
+                        o[0] = lds; o[1] = remap; o[2] = overlap; o[3] = g; o[4] = gs.second; o[5] = per;
                     }
                     ++rows;
                     if (per < bestMs) { bestMs = per; best = s.tune; }
@@ -950,38 +1098,47 @@ int lbm3_autotune(lbm3_sim *h, int steps, double *out, int cap, int *nOut)
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    if (rc) {
-        s.tune = start;
-        (void)regroup(s);
-        (void)restore_snapshot(s, z);
-        return rc;
-    }
-    s.tune = best;
-    if ((rc = regroup(s))) return rc;
+    s.tune = start;
+    const int rg = regroup(s);
+    const int rs = restore_snapshot(s, z);
+    if (rc) return rc;
+    if (rg) return rg;
+    if (rs) return rs;
     if (nOut) *nOut = rows;
-    return restore_snapshot(s, z);
+    // another open-cell layout: the pending condition is completed in the old
+    // one first (regroup), which rounds it to the storage type once
+    const bool relayout = !same_open_layout(s, start, best);
+    s.tune = best;
+    return relayout ? regroup(s) : 0;
 }
 
-int lbm3_get_tune(lbm3_sim *h, int *out4)
+int lbm3_get_tune(lbm3_sim *h, int *out5)
 {
-    if (!h || !out4) return set_error(LBM3_E_ARG, "null argument");
+    if (!h || !out5) return set_error(LBM3_E_ARG, "null argument");
     const Sim::Tune &t = h->s.tune;
-    out4[0] = t.bulkLds; out4[1] = t.xcdRemap; out4[2] = t.overlap;
-    out4[3] = t.openGroup ? t.openGroup : open_group_cells((int)storage_bytes(h->s.storage));
+    out5[0] = t.bulkLds; out5[1] = t.xcdRemap; out5[2] = t.overlap;
+    out5[3] = t.openGroup ? t.openGroup : open_group_cells((int)storage_bytes(h->s.storage));
+    out5[4] = t.openSplit;
     return 0;
 }
 
-int lbm3_set_tune(lbm3_sim *h, const int *in4)
+int lbm3_set_tune(lbm3_sim *h, const int *in5)
 {
-    if (!h || !in4) return set_error(LBM3_E_ARG, "null argument");
-    if (in4[0] < 0 || in4[0] > 65536 || (in4[3] != 8 && in4[3] != 16))
-        return set_error(LBM3_E_ARG, "bulk LDS in 0..65536 bytes, open groups of 8 or 16 cells");
+    if (!h || !in5) return set_error(LBM3_E_ARG, "null argument");
+    if (in5[0] < 0 || in5[0] > 65536 || in5[2] < 0 || in5[2] > 2 || (in5[3] != 1 && in5[3] != 8 && in5[3] != 16) ||
+        in5[4] < 0 || in5[4] > 2)
+        return set_error(LBM3_E_ARG, "bulk LDS in 0..65536 bytes, side streams 0..2, open groups of 1 (in the bulk "
+                                     "kernel), 8 or 16 cells, x-face choice 0..2");
     Sim &s = h->s;
-    s.tune.bulkLds = in4[0];
-    s.tune.xcdRemap = in4[1] != 0;
-    s.tune.overlap = in4[2] != 0;
-    s.tune.openGroup = in4[3];
-    return regroup(s);
+    if (s.timing) return set_error(LBM3_E_STATE, "launch choices changed during a timed run");
+    const Sim::Tune was = s.tune;
+    s.tune.bulkLds = in5[0];
+    s.tune.xcdRemap = in5[1] != 0;
+    s.tune.overlap = in5[2];
+    s.tune.openGroup = in5[3];
+    s.tune.openSplit = in5[4];
+    // only a new open-cell layout completes a pending condition (regroup)
+    return same_open_layout(s, was, s.tune) ? 0 : regroup(s);
 }
 
 int lbm3_run_graph(lbm3_sim *h, int steps)
@@ -989,41 +1146,88 @@ int lbm3_run_graph(lbm3_sim *h, int steps)
     if (!h || steps < 0) return set_error(LBM3_E_ARG, "bad run arguments");
     Sim &s = h->s;
     if (const int rc = check_run(s, 0)) return rc;
-    for (int L = 0; L < s.levels; ++L)
-        if (s.L[L].nOpen > 0 || !s.L[L].rotors.empty())
-            return set_error(LBM3_E_STATE, "graph replay needs levels without open boundaries or rotors "
-                                           "(their kernels take arguments that change every step)");
     if (s.timing) return set_error(LBM3_E_STATE, "graph replay during a timed run");
     if (steps >= 2) {
-        // two coarse steps bring every level's esotwist parity back, so one
-        // captured pair replays with the same kernel arguments
-        lbm3_info before[LBM3_MAX_LEVELS];
-        for (int L = 0; L < s.levels; ++L) before[L] = s.L[L].info;
+        // Two coarse steps bring every level's esotwist parity back, so one
+        // captured pair replays with the same kernel arguments.  What changes
+        // from pair to pair -- the iteration an open-boundary condition belongs
+        // to, whether a deferred one is pending, the rotors' angles -- the
+        // kernels read from a device table with one slot per level step,
+        // refilled before each replay from a dry run of the host bookkeeping.
+        struct Host {
+            lbm3_info info;
+            bool settled;
+            std::vector<double> angles;
+        };
+        auto save = [&]() {
+            std::vector<Host> v((size_t)s.levels);
+            for (int L = 0; L < s.levels; ++L) {
+                v[(size_t)L] = {s.L[L].info, s.L[L].openSettled, {}};
+                for (const auto &r : s.L[L].rotors) v[(size_t)L].angles.push_back(r.radiansElapsed);
+            }
+            return v;
+        };
+        auto load_host = [&](const std::vector<Host> &v) {
+            for (int L = 0; L < s.levels; ++L) {
+                s.L[L].info = v[(size_t)L].info;
+                s.L[L].openSettled = v[(size_t)L].settled;
+                for (size_t r = 0; r < s.L[L].rotors.size(); ++r) s.L[L].rotors[r].radiansElapsed = v[(size_t)L].angles[r];
+            }
+        };
+        long long slots = 0;
+        for (int L = 0; L < s.levels; ++L) slots += 2LL << L;
+        DBuf<StepParam> table;
+        HIPCHK(table.alloc(slots));
+        const std::vector<Host> before = save();
         hipGraph_t graph = nullptr;
         HIPCHK(hipStreamBeginCapture(s.stream, hipStreamCaptureModeThreadLocal));
+        s.captureTable = table.p;
+        s.captureSlot = 0;
         update_all(s, 0);
         update_all(s, 0);
+        s.captureTable = nullptr;
         const hipError_t ec = hipStreamEndCapture(s.stream, &graph);
-        int dIter[LBM3_MAX_LEVELS], dReset[LBM3_MAX_LEVELS];
-        for (int L = 0; L < s.levels; ++L) {
-            dIter[L] = s.L[L].info.iterationsFinished - before[L].iterationsFinished;
-            dReset[L] = s.L[L].info.iterationsSinceTrackerReset - before[L].iterationsSinceTrackerReset;
-            s.L[L].info = before[L];          // nothing ran yet
-        }
+        const int captured = s.captureSlot;
+        load_host(before);                    // nothing ran yet
         if (ec != hipSuccess) {
             if (graph) (void)hipGraphDestroy(graph);
             return check_hip(ec, "step capture");
         }
+        if (captured != slots) {
+            (void)hipGraphDestroy(graph);
+            return set_error(LBM3_E_STATE, "step capture used an unexpected number of level steps");
+        }
         hipGraphExec_t exec = nullptr;
         hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-        for (int k = 0; e == hipSuccess && k < steps / 2; ++k) {
-            e = hipGraphLaunch(exec, s.stream);
-            for (int L = 0; L < s.levels && e == hipSuccess; ++L) {
-                s.L[L].info.iterationsFinished += dIter[L];
-                s.L[L].info.iterationsSinceTrackerReset += dReset[L];
-            }
+        // pinned staging of the table, two buffers used in turn: a buffer is
+        // refilled only after the copy that last read it has completed
+        StepParam *stage[2] = {nullptr, nullptr};
+        hipEvent_t copied[2] = {nullptr, nullptr};
+        for (int b = 0; b < 2 && e == hipSuccess; ++b) {
+            e = hipHostMalloc((void **)&stage[b], sizeof(StepParam) * (size_t)slots, hipHostMallocDefault);
+            if (e == hipSuccess) e = hipEventCreateWithFlags(&copied[b], hipEventDisableTiming);
         }
+        std::vector<StepParam> params;
+        for (int k = 0; e == hipSuccess && k < steps / 2; ++k) {
+            params.clear();
+            s.dryRun = &params;
+            update_all(s, 0);
+            update_all(s, 0);
+            s.dryRun = nullptr;
+            const int b = k & 1;
+            if (k >= 2) e = hipEventSynchronize(copied[b]);
+            if (e != hipSuccess) break;
+            memcpy(stage[b], params.data(), sizeof(StepParam) * params.size());
+            e = hipMemcpyAsync(table.p, stage[b], sizeof(StepParam) * params.size(), hipMemcpyHostToDevice, s.stream);
+            if (e == hipSuccess) e = hipEventRecord(copied[b], s.stream);
+            if (e == hipSuccess) e = hipGraphLaunch(exec, s.stream);
+        }
+        s.dryRun = nullptr;
         const hipError_t es = hipStreamSynchronize(s.stream);   // the graph is released after its last replay
+        for (int b = 0; b < 2; ++b) {
+            if (stage[b]) (void)hipHostFree(stage[b]);
+            if (copied[b]) (void)hipEventDestroy(copied[b]);
+        }
         if (exec) (void)hipGraphExecDestroy(exec);
         (void)hipGraphDestroy(graph);
         if (e != hipSuccess) return check_hip(e, "step graph replay");
@@ -1201,6 +1405,7 @@ int lbm3_download_array(lbm3_sim *h, int level, const char *name, void *out, lon
     ARR("rowJPlus", l.rjp, l.nRows) ARR("rowKPlus", l.rkp, l.nRows) ARR("rowJKPlus", l.rjkp, l.nRows)
     ARR("wallMap", l.wallMap, l.N) ARR("wallIndex", l.wallIndex, l.W) ARR("wallData", l.wallData, l.W)
     ARR("cellRow", l.cellRow, l.N)
+    ARR("bulkSkip", l.bulkSkip, (l.N + 63) / 64) ARR("openBits", l.openBits, l.openInBulk ? (l.N + 63) / 64 : 0)
     ARR("linkLength", l.linkLength, 7LL * l.W) ARR("periodicIndex", l.perIndex, l.nPeriodic)
     ARR("periodicSource", l.perSource, l.nPeriodic) ARR("openIndex", l.openCell, l.nOpen)
     ARR("openSource", l.openSource, l.nOpen) ARR("openMeta", l.openMeta, l.nOpen)
@@ -1596,7 +1801,7 @@ int lbm3_load_checkpoint(lbm3_sim *h, const char *path)
     // read and check every level's shape before anything on the device changes
     std::vector<lbm3_info> infos((size_t)s.levels);
     std::vector<long> at((size_t)s.levels);
-    std::vector<char> settled((size_t)s.levels);
+    std::vector<char> settled((size_t)s.levels), split((size_t)s.levels);
     for (int L = 0; L < s.levels; ++L) {
         Level &l = s.L[L];
         CheckpointLevel d;
@@ -1615,6 +1820,7 @@ int lbm3_load_checkpoint(lbm3_sim *h, const char *path)
         skip += (long)(sizeof(double) * l.rotors.size());
         if (fseek(fp, skip, SEEK_CUR) != 0) return fail("seek failed");
         settled[(size_t)L] = d.openSettled != 0;
+        split[(size_t)L] = d.split != 0;
     }
     // the file ends where the last level's data ends (a short file changes nothing)
     const long end = ftell(fp);
@@ -1627,6 +1833,16 @@ int lbm3_load_checkpoint(lbm3_sim *h, const char *path)
             s.L[L].openSettled = true;       // the state the load replaces needs no completion
             if (const int rc = prepare_open_fusion(s, L)) { fclose(fp); return rc; }
         }
+    // a condition saved as pending covers the entries the saving simulation
+    // left to its open-cell kernel: group the levels the same way
+    for (int L = 0; L < s.levels; ++L) {
+        Level &l = s.L[L];
+        if (settled[(size_t)L] || !l.fusedOpen || split[(size_t)L] == (l.nSepEntries > 0)) continue;
+        if (const int rc = settle_open(s, L)) { fclose(fp); return rc; }
+        s.tune.openSplit = split[(size_t)L] ? 1 : 0;
+        l.openFusionChecked = false;
+        if (const int rc = prepare_open_fusion(s, L)) { fclose(fp); return rc; }
+    }
     std::vector<unsigned char> stage;
     for (int L = 0; L < s.levels; ++L) {
         Level &l = s.L[L];

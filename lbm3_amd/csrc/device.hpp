@@ -50,6 +50,7 @@ struct FastTopo {
     const uint64_t *skip;  // [N / 64 rounded up to 4] bit c % 64 of word c / 64: not a bulk cell; null: read cellRow
     const int *tileRow;    // [N / 256 rounded up] row of cell 256 t
     int nRows;
+    const uint64_t *openBits;  // as skip: open cells the bulk kernel completes and steps itself; null: none
 };
 
 __device__ __forceinline__ void decode_fast(const FastTopo &t, int cell, int row, int (&nb)[8])
@@ -116,6 +117,18 @@ template <class T> struct FusedOpen {
     int groupCells;               // cells per group (the width the groups were built with)
 };
 
+// Per-step values of a step replayed from a captured HIP graph, whose kernel
+// arguments are fixed at capture: the iteration the open-boundary condition
+// belongs to, whether a deferred condition is pending, and each rotor's cosine
+// and sine.  A null pointer in the argument block: the arguments hold them.
+struct StepParam {
+    int iter;                     // iteration of a condition the open-cell kernel applies (pending from the last step)
+    int pending;
+    int iterNow;                  // iteration of this step's separate boundary pass
+    int pad;
+    double cs[2 * LBM3_MAX_ROTORS];
+};
+
 // One rotor of a level (RotorStruct, types.h:206-219) as the kernels read it;
 // the angle enters as the step's cosine and sine (StepArgs::rotorCS).
 struct RotorDev {
@@ -166,6 +179,7 @@ template <class S> struct StepArgs {
     const int *rotorCells;        // [rotorCellCount] cells with their own kernel (record -4)
     int rotorCellCount;
     double rotorCS[2 * LBM3_MAX_ROTORS];   // cos, sin of each rotor's angle this step, in the compute type
+    const StepParam *param;       // this step's values in device memory (graph replay) or null
     int xcdRemap;                 // bulk blocks dealt so that each XCD sweeps one contiguous part (else round-robin)
     int bulkLds;                  // dynamic LDS bytes per bulk block (host side: caps resident bulk blocks per CU)
 };
@@ -184,6 +198,8 @@ template <class S> struct OpenArgs {
     int iterationsFinished;
     Codec cd;
     const float *scale;           // per entry: velocity profile factor of its rule, or null (none)
+    const StepParam *param;       // this step's iteration in device memory (graph replay) or null
+    const int *list;              // the entries to complete ([count]), or null: entries 0..count-1
 };
 
 // Blocks are dispatched round-robin over the 8 XCDs.  With LBM3_XCD_REMAP=1

@@ -14,7 +14,8 @@ template <class S> inline StepArgs<S> step_args(Sim &s, Level &l)
     StepArgs<S> a;
     a.f = reinterpret_cast<S *>(l.f.p);
     a.topo = topo_of(l);
-    a.fast = FastTopo{l.N, l.cellRow.p, l.rowTab.p, l.bulkSkip.p, l.tileRow.p, l.nRows};
+    a.fast = FastTopo{l.N, l.cellRow.p, l.rowTab.p, l.bulkSkip.p, l.tileRow.p, l.nRows,
+                      l.openInBulk ? l.openBits.p : nullptr};
     a.wallMap = l.wallMap.p;
     a.wallIndex = l.wallIndex.p;
     a.wallData = l.wallData.p;
@@ -25,6 +26,7 @@ template <class S> inline StepArgs<S> step_args(Sim &s, Level &l)
     a.bulk = l.bulk;
     a.wallBC = l.wallBC.p;
     a.wallVel = l.wallVel.p;
+    a.param = nullptr;
     a.xcdRemap = s.tune.xcdRemap;
     a.bulkLds = s.tune.bulkLds;
     a.geom = l.geom;
@@ -58,6 +60,8 @@ template <class S> inline OpenArgs<S> open_args(Sim &s, Level &l, int iter)
     a.uNormalCum = s.C.trackOpenBoundaries ? l.openCum.p + (l.nOpen > 0 ? l.nOpen : 1) : nullptr;
     a.iterationsFinished = iter;
     a.cd = s.codec;
+    a.param = nullptr;
+    a.list = nullptr;
     return a;
 }
 

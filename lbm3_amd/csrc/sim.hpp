@@ -84,6 +84,15 @@ struct Level {
     DBuf<int> openGroups;            // aligned cell groups holding open cells (k_open_groups)
     int nOpenGroups = 0;
     int openGroupCells = 8;          // cells per group of openGroups
+    // group width 1: no groups, the bulk kernel completes and steps the open
+    // cells itself (openBits marks them)
+    bool openInBulk = false;
+    DBuf<uint64_t> openBits;
+    // with Sim::Tune::openSplit: the open-list entries the open-cell kernel
+    // completes (fused) and those on the x faces, completed by a pass after
+    // the step's streaming (separate)
+    DBuf<int> fusedEntries, sepEntries;
+    int nFusedEntries = 0, nSepEntries = 0;
     LocalBC bulk{};
     DBuf<LocalBC> wallBC;            // [32]
     DBuf<float> wallVel;             // [3][W] per-cell wall velocity of velMode 3 wall ids (else empty)
@@ -122,14 +131,34 @@ struct Sim {
     hipStream_t stream = nullptr;    // every call's work is ordered on this stream
     hipStream_t aux = nullptr;       // wall-adjacent kernel beside the bulk kernel, joined back each step
     hipEvent_t fork = nullptr, join = nullptr;
+    hipStream_t aux2 = nullptr;      // the rotor-cell kernel, beside the wall-adjacent one
+    hipEvent_t join2 = nullptr;
+    // Tune::overlap 2: level L >= 1 sub-cycles on levelStream[L], forked from
+    // the coarser level's stream before that level's step (the sub-cycle
+    // needs the last interface transfer, not the coarse step) and joined back
+    // before the next transfer
+    hipStream_t levelStream[LBM3_MAX_LEVELS] = {};
+    hipEvent_t levelFork[LBM3_MAX_LEVELS] = {}, levelJoin[LBM3_MAX_LEVELS] = {};
     lbm3_hooks hooks{};              // case hooks evaluated by the case (null callbacks: rule tables)
     // launch choices of the step (lbm3_autotune picks them by timing)
     struct Tune {
         int bulkLds = LBM3_BULK_LDS;  // dynamic LDS per bulk block: caps resident bulk blocks per CU
         int xcdRemap = LBM3_XCD_REMAP;
-        int overlap = 1;              // wall / open / rotor kernels on the second stream beside the bulk kernel
+        // 1: wall / open / rotor kernels on the second stream beside the bulk
+        // kernel; 2: also each finer level's sub-cycle on a stream of its own,
+        // beside its coarser level's step; 0: everything in order
+        int overlap = 1;
         int openGroup = 0;            // cells per open-boundary group (0: open_group_cells of the storage)
+        // cells on the x faces: 0 as the others; 1 stepped by the bulk kernel
+        // and completed by a pass after streaming; 2 (with openGroup 1) in
+        // groups of the open-cell kernel while the bulk kernel takes the others
+        int openSplit = 0;
     } tune;
+    // graph capture: each level step takes the next slot of this device table
+    // for the values that change between replays; a dry run only collects them
+    StepParam *captureTable = nullptr;
+    int captureSlot = 0;
+    std::vector<StepParam> *dryRun = nullptr;
     DBuf<double> partial;            // reduction scratch
     DBuf<double> scalar;
     int watchdog = 0;                // lbm3_run checks for non-finite populations every this many coarse steps (0: off)

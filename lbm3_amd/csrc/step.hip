@@ -15,11 +15,12 @@
 //  k_init, k_open_ref_init   initial condition (applyInitialCondition.h:8-84)
 //  k_macros, k_cell_sums, k_wall_force        observables, FP64 sums
 //
-// Populations are SoA planes f[q*N + cell] in the storage type S; each thread
-// owns one cell, keeps its 27 populations in registers and writes back into the
-// same 27 slots it read (esotwist), so the kernel is race free without any
+// Populations are stored in tiles of LBM3_TILE cells, each tile holding its 27
+// slot runs one after another (storage.hpp pop_index); each thread owns one
+// cell, keeps its 27 populations in registers and writes back into the same 27
+// slots it read (esotwist), so the kernel is race free without any
 // synchronisation.  The esotwist parity is a template parameter, which makes
-// every plane offset a compile-time multiple of N.
+// every slot run's offset within a tile a compile-time constant.
 #include "step.hpp"
 #include "hooks.hpp"
 
@@ -225,7 +226,8 @@ __device__ __forceinline__ void rotors_apply(const StepArgs<S> &a, int cell, T d
     T cum = T(0);
     for (int r = 0; r < a.nRotors; ++r) {
         const RotorDev &Rt = a.rotors[r];
-        const T c = T(a.rotorCS[2 * r]), s = T(a.rotorCS[2 * r + 1]);
+        const double *cs = a.param ? a.param->cs : a.rotorCS;
+        const T c = T(cs[2 * r]), s = T(cs[2 * r + 1]);
         int rm, cI[3];
         T w[3], fr;
         if (!rotor_locate<T>(Rt, c, s, x, y, z, rm, cI, w, fr) || fr == T(0)) continue;
@@ -280,7 +282,24 @@ __device__ __forceinline__ void rotors_apply(const StepArgs<S> &a, int cell, T d
 #define LBM3_BUF16 0
 #endif
 
-template <class S, bool FLIP, int BULK, class Off>
+// OPEN: the level's open cells are in the bulk kernel too (FastTopo::openBits):
+// such a cell first completes the populations the previous step left unknown,
+// as k_open_groups does, then collides and streams like its neighbours
+template <class S, bool OPEN>
+__device__ __forceinline__ void open_in_bulk(const StepArgs<S> &a, int cell, typename StoreTraits<S>::compute (&f)[27])
+{
+    if constexpr (OPEN) {
+        using T = typename StoreTraits<S>::compute;
+        const uint64_t m = a.fast.openBits[__builtin_amdgcn_readfirstlane(cell >> 6)];
+        if (!((m >> (cell & 63)) & 1ull)) return;
+        if (!(a.param ? a.param->pending : a.open.pending)) return;
+        const int x = -(a.fast.cellRow[cell].x + 8);
+        open_dispatch<T>(f, x, a.open.meta[x], a.open.records, a.open.dRhoRef, a.open.uNormalRef, a.open.dRhoCum,
+                         a.open.uNormalCum, a.param ? a.param->iter : a.open.iter, a.open.scale);
+    }
+}
+
+template <class S, bool FLIP, int BULK, class Off, bool OPEN = false>
 __device__ __forceinline__ void bulk_cell(const StepArgs<S> &a, int cell)
 {
     using T = typename StoreTraits<S>::compute;
@@ -314,6 +333,7 @@ __device__ __forceinline__ void bulk_cell(const StepArgs<S> &a, int cell)
 #pragma unroll
         for (int q = 0; q < 27; ++q)
             f[q] = StoreTraits<S>::bload(r, off[pre_nb(q)], pre_plane(q, FLIP) * LBM3_TILE * int(sizeof(S)), q, a.cd);
+        open_in_bulk<S, OPEN>(a, cell, f);
         collide<T, BULK>(f, relax_of<T>(a.bulk));
         if constexpr (sizeof(S) >= 4 || LBM3_BUF16 == 1) {
 #pragma unroll
@@ -328,6 +348,7 @@ __device__ __forceinline__ void bulk_cell(const StepArgs<S> &a, int cell)
 #pragma unroll
         for (int q = 0; q < 27; ++q)
             f[q] = StoreTraits<S>::load(at(a.f + pre_plane(q, FLIP) * LBM3_TILE, off[pre_nb(q)]), q, a.cd);
+        open_in_bulk<S, OPEN>(a, cell, f);
         collide<T, BULK>(f, relax_of<T>(a.bulk));
 #pragma unroll
         for (int q = 0; q < 27; ++q)
@@ -335,14 +356,15 @@ __device__ __forceinline__ void bulk_cell(const StepArgs<S> &a, int cell)
     }
 }
 
-// WIDE: the level's population array exceeds 4 GiB (64-bit lane offsets)
-template <class S, bool FLIP, int BULK, bool WIDE>
+// WIDE: the level's population array exceeds 4 GiB (64-bit lane offsets);
+// OPEN: the open cells are stepped here too (never with WIDE)
+template <class S, bool FLIP, int BULK, bool WIDE, bool OPEN = false>
 __global__ __launch_bounds__(256) void k_collide_stream(StepArgs<S> a)
 {
     const int cell = xcd_block(blockIdx.x, gridDim.x, a.xcdRemap) * 256 + threadIdx.x;
     if (cell >= a.fast.N) return;
     if (WIDE) bulk_cell<S, FLIP, BULK, uint64_t>(a, cell);
-    else bulk_cell<S, FLIP, BULK, uint32_t>(a, cell);
+    else bulk_cell<S, FLIP, BULK, uint32_t, OPEN>(a, cell);
 }
 
 // Open-boundary cells (never wall-adjacent when this kernel is used): load,
@@ -375,10 +397,10 @@ __global__ __launch_bounds__(256) void k_open_groups(StepArgs<S> a)
 #pragma unroll
     for (int q = 0; q < 27; ++q)
         f[q] = StoreTraits<S>::bload(r, off[pre_nb(q)], pre_plane(q, FLIP) * LBM3_TILE * int(sizeof(S)), q, a.cd);
-    if (code <= -8 && a.open.pending) {
+    if (code <= -8 && (a.param ? a.param->pending : a.open.pending)) {
         const int x = -(code + 8);
         open_dispatch<T>(f, x, a.open.meta[x], a.open.records, a.open.dRhoRef, a.open.uNormalRef, a.open.dRhoCum,
-                         a.open.uNormalCum, a.open.iter, a.open.scale);
+                         a.open.uNormalCum, a.param ? a.param->iter : a.open.iter, a.open.scale);
     }
     collide<T, BULK>(f, relax_of<T>(a.bulk));
 #pragma unroll
@@ -399,26 +421,50 @@ __global__ __launch_bounds__(256) void k_mark_groups(const int *groups, int coun
 
 // the bulk kernel's view of the cell records: per 64 cells a mask of the cells
 // it skips (record >= 0 or <= -3, and the padding past N), per 256 cells the
-// row of the first one
-__global__ __launch_bounds__(256) void k_bulk_mask(const int2 *cellRow, int N, uint64_t *skip, int *tileRow)
+// row of the first one; with `open` non-null the open cells (record <= -8) are
+// the bulk kernel's too, marked in `open`
+__global__ __launch_bounds__(256) void k_bulk_mask(const int2 *cellRow, int N, uint64_t *skip, int *tileRow,
+                                                   uint64_t *open)
 {
     const int cell = blockIdx.x * 256 + threadIdx.x;
-    bool sk = true;
+    bool sk = true, op = false;
     int row = 0;
     if (cell < N) {
         const int2 r = cellRow[cell];
-        sk = r.x >= 0 || r.x <= -3;
+        op = open && r.x <= -8;
+        sk = (r.x >= 0 || r.x <= -3) && !op;
         row = r.y;
     }
-    const unsigned long long m = __ballot(sk);
-    if ((threadIdx.x & 63) == 0) skip[cell >> 6] = m;
+    const unsigned long long m = __ballot(sk), o = __ballot(op);
+    if ((threadIdx.x & 63) == 0) {
+        skip[cell >> 6] = m;
+        if (open) open[cell >> 6] = o;
+    }
     if (threadIdx.x == 0) tileRow[blockIdx.x] = row;
 }
 
-void launch_bulk_mask(const int2 *cellRow, int N, uint64_t *skip, int *tileRow, hipStream_t s)
+void launch_bulk_mask(const int2 *cellRow, int N, uint64_t *skip, int *tileRow, uint64_t *open, hipStream_t s)
 {
     if (N <= 0) return;
-    k_bulk_mask<<<(N + 255) / 256, 256, 0, s>>>(cellRow, N, skip, tileRow);
+    k_bulk_mask<<<(N + 255) / 256, 256, 0, s>>>(cellRow, N, skip, tileRow, open);
+}
+
+// the cells of the open groups leave the bulk kernel's view: skip bits set,
+// open bits cleared (a group of 8 or 16 aligned cells lies in one 64-bit word)
+__global__ __launch_bounds__(256) void k_group_bits(const int *groups, int count, int G, uint64_t *skip, uint64_t *open)
+{
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= count) return;
+    const long long first = (long long)groups[t] * G;
+    const unsigned long long bits = (G >= 64 ? ~0ull : ((1ull << G) - 1ull)) << (first & 63);
+    atomicOr(reinterpret_cast<unsigned long long *>(skip + (first >> 6)), bits);
+    atomicAnd(reinterpret_cast<unsigned long long *>(open + (first >> 6)), ~bits);
+}
+
+void launch_group_bits(const int *groups, int count, int G, uint64_t *skip, uint64_t *open, hipStream_t s)
+{
+    if (count <= 0) return;
+    k_group_bits<<<(count + 255) / 256, 256, 0, s>>>(groups, count, G, skip, open);
 }
 
 void launch_mark_groups(const int *groups, int count, int G, int N, int2 *cellRow, hipStream_t s)
@@ -632,8 +678,8 @@ __device__ __forceinline__ void open_one(const OpenArgs<S> &a, int x, const int 
     for (int q = 0; q < 27; ++q)
         f[q] = unknown_dir(nx, ny, nz, q) ? T(0)
                                           : StoreTraits<S>::load(a.f + pop_index(pre_plane(q, FLIP), nb[pre_nb(q)]), q, a.cd);
-    open_fill<NI, T>(f, x, meta, a.records, a.dRhoRef, a.uNormalRef, a.dRhoCum, a.uNormalCum, a.iterationsFinished,
-                     a.scale);
+    open_fill<NI, T>(f, x, meta, a.records, a.dRhoRef, a.uNormalRef, a.dRhoCum, a.uNormalCum,
+                     a.param ? a.param->iterNow : a.iterationsFinished, a.scale);
 #pragma unroll
     for (int q = 0; q < 27; ++q)
         if (unknown_dir(nx, ny, nz, q))
@@ -660,9 +706,9 @@ __device__ __forceinline__ void open_entry(const OpenArgs<S> &a, int x)
 template <class S, bool FLIP>
 __global__ __launch_bounds__(256) void k_open(OpenArgs<S> a)
 {
-    const int x = blockIdx.x * 256 + threadIdx.x;
-    if (x >= a.count) return;
-    open_entry<S, FLIP>(a, x);
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= a.count) return;
+    open_entry<S, FLIP>(a, a.list ? a.list[t] : t);
 }
 
 // the stored values are copied as they are: source and ghost read them as the same directions
@@ -852,29 +898,49 @@ __global__ __launch_bounds__(256) void k_wall_force(const double *gWall, int W, 
                                                     Topology topo, LevelGeom g, double ox, double oy, double oz, int nBodies,
                                                     double *partial)
 {
-    __shared__ double acc[LBM3_MAX_BODIES * 6];
-    for (int i = threadIdx.x; i < nBodies * 6; i += 256) acc[i] = 0.0;
-    __syncthreads();
-    const int w = blockIdx.x * 256 + threadIdx.x;
+    // a fixed-order tree sum per body and component: the same bits on every
+    // run, whatever the order the block's waves run in
+    __shared__ double red[6][256];
+    const int t = threadIdx.x;
+    const int w = blockIdx.x * 256 + t;
+    int body = -1;
+    double v[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     if (w < W) {
-        const int body = (wallData[w] >> 27) & 31u;
+        body = (wallData[w] >> 27) & 31u;
         if (body < nBodies) {
             int nb[8], ijk[3];
             decode(topo, wallIndex[w], nb, ijk);
             const double x = double(float(ijk[0]) * g.res + g.ox) - ox;
             const double y = double(float(ijk[1]) * g.res + g.oy) - oy;
             const double z = double(float(ijk[2]) * g.res + g.oz) - oz;
-            const double fx = gWall[w], fy = gWall[(size_t)W + w], fz = gWall[2 * (size_t)W + w];
-            atomicAdd(&acc[body * 6 + 0], fx);
-            atomicAdd(&acc[body * 6 + 1], fy);
-            atomicAdd(&acc[body * 6 + 2], fz);
-            atomicAdd(&acc[body * 6 + 3], y * fz - z * fy);
-            atomicAdd(&acc[body * 6 + 4], z * fx - x * fz);
-            atomicAdd(&acc[body * 6 + 5], x * fy - y * fx);
+            v[0] = gWall[w];
+            v[1] = gWall[(size_t)W + w];
+            v[2] = gWall[2 * (size_t)W + w];
+            v[3] = y * v[2] - z * v[1];
+            v[4] = z * v[0] - x * v[2];
+            v[5] = x * v[1] - y * v[0];
+        } else {
+            body = -1;
         }
     }
-    __syncthreads();
-    for (int i = threadIdx.x; i < nBodies * 6; i += 256) partial[(size_t)blockIdx.x * nBodies * 6 + i] = acc[i];
+    double *out = partial + (size_t)blockIdx.x * nBodies * 6;
+    for (int b = 0; b < nBodies; ++b) {
+        if (!__syncthreads_or(body == b)) {      // no cell of this body in the block
+            if (t < 6) out[b * 6 + t] = 0.0;
+            continue;
+        }
+#pragma unroll
+        for (int k = 0; k < 6; ++k) red[k][t] = body == b ? v[k] : 0.0;
+        __syncthreads();
+        for (int s = 128; s > 0; s >>= 1) {
+            if (t < s) {
+#pragma unroll
+                for (int k = 0; k < 6; ++k) red[k][t] += red[k][t + s];
+            }
+            __syncthreads();
+        }
+        if (t < 6) out[b * 6 + t] = red[t][0];
+    }
 }
 
 // unit-level entry: collide n cells stored cell-major (27 per cell)
@@ -994,7 +1060,11 @@ void launch_collide_stream(const StepArgs<S> &a, bool flip, hipStream_t s)
     const bool k17 = relax_of<typename StoreTraits<S>::compute>(a.bulk).k17 != 0;
     const bool wide = wide_level<S>(a.fast.N);
 #define LBM3_BULK_LAUNCH(F, K, W) hipLaunchKernelGGL((k_collide_stream<S, F, K, W>), g, b, lds, s, a)
-    if (wide) {
+#define LBM3_BULK_OPEN(F, K) hipLaunchKernelGGL((k_collide_stream<S, F, K, false, true>), g, b, lds, s, a)
+    if (a.fast.openBits && !wide) {
+        if (flip) { if (k17) LBM3_BULK_OPEN(true, 1); else LBM3_BULK_OPEN(true, 2); }
+        else { if (k17) LBM3_BULK_OPEN(false, 1); else LBM3_BULK_OPEN(false, 2); }
+    } else if (wide) {
         if (flip) { if (k17) LBM3_BULK_LAUNCH(true, 1, true); else LBM3_BULK_LAUNCH(true, 2, true); }
         else { if (k17) LBM3_BULK_LAUNCH(false, 1, true); else LBM3_BULK_LAUNCH(false, 2, true); }
     } else {
@@ -1002,6 +1072,7 @@ void launch_collide_stream(const StepArgs<S> &a, bool flip, hipStream_t s)
         else { if (k17) LBM3_BULK_LAUNCH(false, 1, false); else LBM3_BULK_LAUNCH(false, 2, false); }
     }
 #undef LBM3_BULK_LAUNCH
+#undef LBM3_BULK_OPEN
 }
 
 template <class S>
@@ -1076,6 +1147,19 @@ void launch_scatter_index(const int *cell, int count, int *openOf, hipStream_t s
 {
     if (count <= 0) return;
     hipLaunchKernelGGL(k_scatter_index, dim3(blocks_for(count)), dim3(256), 0, s, cell, count, openOf);
+}
+
+// openOf[cell of entry] = entry, for the listed open-list entries only
+__global__ __launch_bounds__(256) void k_scatter_entries(const int *cell, const int *list, int count, int *openOf)
+{
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t < count) openOf[cell[list[t]]] = list[t];
+}
+
+void launch_scatter_entries(const int *cell, const int *list, int count, int *openOf, hipStream_t s)
+{
+    if (count <= 0) return;
+    hipLaunchKernelGGL(k_scatter_entries, dim3(blocks_for(count)), dim3(256), 0, s, cell, list, count, openOf);
 }
 
 template <class S>
@@ -1358,11 +1442,25 @@ template void launch_collide_batch<double>(double *, int, const Relax<double> &,
 // plain streaming copy: the bandwidth probe used to calibrate HBM counters
 // ---------------------------------------------------------------------------
 namespace lbm3 {
-__global__ __launch_bounds__(256) void k_copy(const float4 *__restrict__ a, float4 *__restrict__ b, long long n4,
+typedef float f4v __attribute__((ext_vector_type(4)));
+__global__ __launch_bounds__(256) void k_copy(const float4 *__restrict__ a4, float4 *__restrict__ b4, long long n4,
                                               const float *__restrict__ at, float *__restrict__ bt, int tail)
 {
+    // four independent 16-byte loads in flight per thread before the stores,
+    // non-temporal: a plain stream with no reuse
+    const f4v *__restrict__ a = reinterpret_cast<const f4v *>(a4);
+    f4v *__restrict__ b = reinterpret_cast<f4v *>(b4);
     const long long stride = (long long)gridDim.x * 256;
-    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n4; i += stride) b[i] = a[i];
+    long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    for (; i + 3 * stride < n4; i += 4 * stride) {
+        const f4v v0 = __builtin_nontemporal_load(a + i), v1 = __builtin_nontemporal_load(a + i + stride);
+        const f4v v2 = __builtin_nontemporal_load(a + i + 2 * stride), v3 = __builtin_nontemporal_load(a + i + 3 * stride);
+        __builtin_nontemporal_store(v0, b + i);
+        __builtin_nontemporal_store(v1, b + i + stride);
+        __builtin_nontemporal_store(v2, b + i + 2 * stride);
+        __builtin_nontemporal_store(v3, b + i + 3 * stride);
+    }
+    for (; i < n4; i += stride) b[i] = a[i];
     if (blockIdx.x == 0 && (int)threadIdx.x < tail) bt[threadIdx.x] = at[threadIdx.x];
 }
 

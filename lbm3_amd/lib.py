@@ -253,6 +253,8 @@ class Sim:
         n = max(cnt.value, 0)
         if name in ("rayHitScan", "cellRow"):
             dt = np.int64
+        elif name in ("bulkSkip", "openBits"):
+            dt = np.uint64
         elif name.startswith("rotor") and name.endswith("Force"):
             dt = np.float64
         elif name in ("wallForce",):
@@ -270,13 +272,20 @@ class Sim:
 
     def cell_kinds(self, level=0):
         """Counts of the cells each per-step kernel updates on a level:
-        {"bulk", "wall", "open", "rotor", "open_group", "solid"} (open, rotor and the
-        fluid cells grouped with open cells have kernels of their own;
-        where the open boundary runs as a separate pass they count as bulk)."""
+        {"bulk", "wall", "open", "rotor", "open_group", "solid", "open_in_bulk"}
+        ("bulk": the cells the bulk kernel updates, from its skip mask; open,
+        rotor and the fluid cells grouped with open cells have kernels of their
+        own, but with open groups of 1 cell the bulk kernel steps the open cells
+        it marks too, counted in "open_in_bulk" and in "bulk")."""
         code = (self.array("cellRow", level) & 0xFFFFFFFF).astype(np.uint32).view(np.int32)
-        return {"bulk": int(((code == -1) | (code == -2)).sum()), "wall": int((code >= 0).sum()),
+        n = code.size
+
+        def bits(name):
+            w = self.array(name, level)
+            return np.unpackbits(w.view(np.uint8), bitorder="little")[:n] if w.size else np.zeros(n, np.uint8)
+        return {"bulk": int((bits("bulkSkip") == 0).sum()), "wall": int((code >= 0).sum()),
                 "open": int((code <= -8).sum()), "rotor": int((code == -4).sum()), "open_group": int((code == -5).sum()),
-                "solid": int((code == -3).sum())}
+                "solid": int((code == -3).sum()), "open_in_bulk": int(bits("openBits").sum())}
 
     def kinetic_energy(self):
         v = C.c_double()
@@ -292,20 +301,22 @@ class Sim:
     def autotune(self, steps=10):
         """lbm3_autotune: time every launch choice over `steps` coarse steps,
         keep the fastest, leave the state as it was.  Returns (chosen
-        {bulk_lds, xcd_remap, overlap, open_group}, [(lds, remap, overlap,
-        group, ms per step), ...])."""
-        out = np.zeros((64, 5))
+        {bulk_lds, xcd_remap, overlap, open_group, open_split}, [(lds, remap,
+        overlap, group, split, ms per step), ...])."""
+        out = np.zeros((128, 6))
         n = C.c_int()
-        check(load().lbm3_autotune(self.h, steps, out.ctypes.data, 64, C.byref(n)))
-        return self.tune(), [tuple(r) for r in out[:min(n.value, 64)]]
+        check(load().lbm3_autotune(self.h, steps, out.ctypes.data, 128, C.byref(n)))
+        return self.tune(), [tuple(r) for r in out[:min(n.value, 128)]]
+
+    TUNE = ("bulk_lds", "xcd_remap", "overlap", "open_group", "open_split")
 
     def tune(self):
-        t = np.zeros(4, np.int32)
+        t = np.zeros(5, np.int32)
         check(load().lbm3_get_tune(self.h, t.ctypes.data))
-        return dict(zip(("bulk_lds", "xcd_remap", "overlap", "open_group"), (int(x) for x in t)))
+        return dict(zip(self.TUNE, (int(x) for x in t)))
 
-    def set_tune(self, bulk_lds, xcd_remap, overlap, open_group):
-        t = np.array([bulk_lds, xcd_remap, overlap, open_group], np.int32)
+    def set_tune(self, bulk_lds, xcd_remap, overlap, open_group, open_split=0):
+        t = np.array([bulk_lds, xcd_remap, overlap, open_group, open_split], np.int32)
         check(load().lbm3_set_tune(self.h, t.ctypes.data))
 
     def wall_forces(self, level=0, origin=(0.0, 0.0, 0.0), bodies=1):
@@ -353,8 +364,8 @@ class Sim:
         return out
 
     def run_graph(self, steps):
-        """run(steps) through a captured HIP graph of two coarse steps (no open
-        boundaries or rotors); returns when the steps are done."""
+        """run(steps) through a captured HIP graph of two coarse steps; returns
+        when the steps are done."""
         check(load().lbm3_run_graph(self.h, int(steps)))
 
     def save_checkpoint(self, path):
