@@ -276,6 +276,16 @@ int lbm3_mass(lbm3_sim *s, double *out2);
  * `origin` (trackerFunctions.h:356-474, FP64): out[body*6 + {Fx,Fy,Fz,Tx,Ty,Tz}] */
 int lbm3_wall_forces(lbm3_sim *s, int level, const double *origin, int nBodies, double *out);
 int lbm3_reset_wall_forces(lbm3_sim *s);   /* wall and rotor force accumulators */
+/* lbm3_wall_forces without any allocation (no counterpart in the reference):
+ * for callers that cannot allocate -- code under HIP graph capture, tight
+ * tracker or serving loops that forbid hipMalloc.  Same arguments and result
+ * (FP64, equal to lbm3_wall_forces up to the order of the partial sums).  The
+ * partial sums live in a fixed device array of 2^20 doubles per GPU, held
+ * until the process ends: wall-adjacent cells of the level, rounded up to a
+ * multiple of 256, times 6 * nBodies / 256, plus 6 * nBodies, must fit, else
+ * LBM3_E_ARG.  Every call on one GPU shares that array: calls that overlap on
+ * one GPU (other host threads, other streams) race. */
+int lbm3_wall_forces_noalloc(lbm3_sim *s, int level, const double *origin, int nBodies, double *out);
 /* Open-boundary flow statistics of updateTracker (trackerFunctions.h:127-223)
  * for boundary openBCID over every level that has it, from the per-cell
  * accumulators time-averaged over the steps since the last

@@ -55,6 +55,7 @@ def load():
         "lbm3_kinetic_energy": (i32, [vp, C.POINTER(f64)]),
         "lbm3_mass": (i32, [vp, C.POINTER(f64)]),
         "lbm3_wall_forces": (i32, [vp, i32, C.POINTER(f64), i32, C.POINTER(f64)]),
+        "lbm3_wall_forces_noalloc": (i32, [vp, i32, C.POINTER(f64), i32, C.POINTER(f64)]),
         "lbm3_autotune": (i32, [vp, i32, vp, i32, C.POINTER(i32)]),
         "lbm3_get_tune": (i32, [vp, vp]),
         "lbm3_set_tune": (i32, [vp, vp]),
