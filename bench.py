@@ -76,8 +76,11 @@ def measure(storage, steps, warmup, k17=False):
         "bytes_per_launch": bytes_per_launch, "build_s": round(build_s, 3),
         "steps_done": warmup + 2 * steps,
         "tune": {"chosen": tuned, "candidates": len(table),
-                 "ms_per_step_best": round(min(r[5] for r in table), 4),
-                 "ms_per_step_worst": round(max(r[5] for r in table), 4)},
+                 "ms_per_step_best": round(min(r[-1] for r in table), 4),
+                 "ms_per_step_worst": round(max(r[-1] for r in table), 4),
+                 # fastest time of each layout of the open and rotor cells (group, x-face choice, rotor)
+                 "layouts": {"%d/%d/%d" % (g, sp, ro): round(min(r[-1] for r in table if (r[3], r[4], r[5]) == (g, sp, ro)), 4)
+                             for g, sp, ro in sorted({(int(r[3]), int(r[4]), int(r[5])) for r in table})}},
     }
     u = sim.macros(0)[:, 1:].astype(np.float64)
     u[sim.array("wallMap") == -3] = np.nan     # solid cells hold no flow

@@ -132,24 +132,29 @@ int lbm3_synchronize(lbm3_sim *s);
  * level's step: 2) and, where a level completes its open boundary when it next
  * loads the open cells, the width of the aligned cell groups the open-cell
  * kernel takes (8 or 16 cells; 1: no groups, the bulk kernel completes and
- * steps the open cells itself) and whether the cells on the x faces are
- * instead stepped by the bulk kernel and completed by a separate pass after
- * streaming; keeps the fastest and restores the
+ * steps the open cells itself) and what becomes of the cells on the x faces
+ * (as the others; stepped by the bulk kernel and completed by a separate pass
+ * after streaming; in groups while the bulk kernel takes the other open
+ * cells), and, where a level has rotors, whether the bulk kernel steps the
+ * rotor cells too instead of a kernel of their own; keeps the fastest and
+ * restores the
  * simulation's state as it was (the populations and every step counter; a
  * pending open-boundary condition stays pending unless the chosen open-cell
- * layout differs, which completes it first).  out[6 r + {LDS bytes,
- * XCD order, side stream, group cells, x faces separate, ms per step}] for the
- * first `cap` candidates; *nOut the candidates timed. */
+ * layout differs, which completes it first).  out[7 r + {LDS bytes,
+ * XCD order, side streams, group cells, x-face choice, rotor cells in the bulk
+ * kernel, ms per step}] for the first `cap` candidates; *nOut the candidates
+ * timed. */
 int lbm3_autotune(lbm3_sim *s, int steps, double *out, int cap, int *nOut);
 /* The launch choices in use: {bulk LDS bytes, XCD order, side streams, open
- * group cells, x faces separate}; lbm3_set_tune sets them (LDS 0..65536, side
- * streams 0..2, group 1, 8 or 16; a new group width or x-face choice
- * completes a pending open-boundary condition first).  Every choice gives the
+ * group cells, x-face choice, rotor cells in the bulk kernel}; lbm3_set_tune
+ * sets them (LDS 0..65536, side streams 0..2, group 1, 8 or 16, x-face choice
+ * 0..2, rotor cells 0..1; a new layout of the open or rotor cells completes a
+ * pending open-boundary condition first).  Every choice gives the
  * same results bit for bit, except with 16-bit storage: that completion, and
  * every step's separate pass for the x faces, store the completed populations
  * rounded to the storage type. */
-int lbm3_get_tune(lbm3_sim *s, int *out5);
-int lbm3_set_tune(lbm3_sim *s, const int *in5);
+int lbm3_get_tune(lbm3_sim *s, int *out6);
+int lbm3_set_tune(lbm3_sim *s, const int *in6);
 
 /* Timed run: `steps` coarse steps between HIP events on the sim's stream.
  * out[0] = total ms; with kernelEvents != 0 also out[1] = summed ms of the

@@ -70,7 +70,9 @@ static int refresh_bulk_mask(Sim &s, Level &l)
     if (l.tileRow.n < blocks) HIPCHK(l.tileRow.alloc(blocks > 0 ? blocks : 1));
     if (l.bulkSkip.n < 4 * blocks) HIPCHK(l.bulkSkip.alloc(blocks > 0 ? 4 * blocks : 4));
     if (l.openInBulk && l.openBits.n < 4 * blocks) HIPCHK(l.openBits.alloc(blocks > 0 ? 4 * blocks : 4));
-    launch_bulk_mask(l.cellRow.p, l.N, l.bulkSkip.p, l.tileRow.p, l.openInBulk ? l.openBits.p : nullptr, s.stream);
+    if (l.rotorInBulk && l.rotorBits.n < 4 * blocks) HIPCHK(l.rotorBits.alloc(blocks > 0 ? 4 * blocks : 4));
+    launch_bulk_mask(l.cellRow.p, l.N, l.bulkSkip.p, l.tileRow.p, l.openInBulk ? l.openBits.p : nullptr,
+                     l.rotorInBulk ? l.rotorBits.p : nullptr, s.stream);
     return check_hip(hipGetLastError(), "bulk mask");
 }
 
@@ -141,6 +143,7 @@ int finish_level_setup(Sim &s, int level)
     l.openFusionChecked = false;
     l.fusedOpen = false;
     l.openInBulk = false;
+    l.rotorInBulk = false;
     l.openSettled = true;
     HIPCHK(l.openNb.alloc(2LL * (l.nOpen > 0 ? l.nOpen : 1)));
     launch_open_nb(topo_of(l), l.nOpen, l.openCell.p, l.openNb.p, s.stream);
@@ -194,6 +197,7 @@ static int prepare_open_fusion(Sim &s, int L)
     l.openFusionChecked = true;
     l.fusedOpen = false;
     l.openInBulk = false;
+    l.rotorInBulk = false;
     l.openSettled = true;
     l.rotorCellCount = 0;
     const uint8_t *cand = nullptr;
@@ -262,7 +266,11 @@ static int prepare_open_fusion(Sim &s, int L)
     launch_fast_topo(topo_of(l), l.wallMap.p, fuse ? openOf.p : nullptr, cand, l.cellRow.p, l.rowTab.p, s.stream);
     HIPCHK(hipGetLastError());
     l.nOpenGroups = 0;
+    // the rotor cells in the bulk kernel (the kernel takes open cells or rotor
+    // cells, not both: open cells then stay in groups)
+    l.rotorInBulk = cand && s.tune.rotorInBulk && !wide_of(s, l);
     int G = s.tune.openGroup ? s.tune.openGroup : open_group_cells((int)storage_bytes(s.storage));
+    if (G == 1 && l.rotorInBulk) G = open_group_cells((int)storage_bytes(s.storage));
     l.openInBulk = fuse && G == 1;   // the bulk kernel completes and steps the open cells
     // the bulk kernel's open cells but those on the x faces, which stay in
     // groups of the open-cell kernel (one x-face cell per row end would hold a
@@ -304,7 +312,7 @@ static int prepare_open_fusion(Sim &s, int L)
         launch_mark_groups(l.openGroups.p, l.nOpenGroups, G, l.N, l.cellRow.p, s.stream);
         HIPCHK(hipGetLastError());
     }
-    if (cand) {
+    if (cand && !l.rotorInBulk) {
         // the cells with record -4, in cell order (coalesced rows)
         std::vector<int2> rec((size_t)l.N);
         HIPCHK(hipMemcpyAsync(rec.data(), l.cellRow.p, sizeof(int2) * rec.size(), hipMemcpyDeviceToHost, s.stream));
@@ -1018,15 +1026,17 @@ static int restore_snapshot(Sim &s, Snapshot &z)
 static bool same_open_layout(const Sim &s, const Sim::Tune &a, const Sim::Tune &b)
 {
     const int g = open_group_cells((int)storage_bytes(s.storage));
-    return (a.openGroup ? a.openGroup : g) == (b.openGroup ? b.openGroup : g) && a.openSplit == b.openSplit;
+    return (a.openGroup ? a.openGroup : g) == (b.openGroup ? b.openGroup : g) && a.openSplit == b.openSplit &&
+           a.rotorInBulk == b.rotorInBulk;
 }
 
-// build every level's open-cell groups again for the current Sim::Tune width
+// build every level's open-cell groups and rotor-cell records again for the
+// current Sim::Tune layout
 static int regroup(Sim &s)
 {
     for (int L = 0; L < s.levels; ++L) {
         Level &l = s.L[L];
-        if (!l.fusedOpen || !l.built) continue;
+        if (!(l.fusedOpen || !l.rotors.empty()) || !l.built) continue;
         if (const int rc = settle_open(s, L)) return rc;
         l.openFusionChecked = false;
         if (const int rc = prepare_open_fusion(s, L)) return rc;
@@ -1047,18 +1057,25 @@ int lbm3_autotune(lbm3_sim *h, int steps, double *out, int cap, int *nOut)
     // layout it was taken in, so that an unchanged layout continues bit for bit
     Snapshot z;
     if (const int rc = take_snapshot(s, z)) return rc;
-    bool fused = false;
-    for (int L = 0; L < s.levels; ++L) fused = fused || s.L[L].fusedOpen;
-    const int storageGroup = open_group_cells((int)storage_bytes(s.storage));
-    // (group width, x faces in a separate pass) layouts of the open cells
-    std::vector<std::pair<int, int>> groups = {{storageGroup, 0}};
-    if (fused) {
-        groups.push_back({storageGroup == 8 ? 16 : 8, 0});
-        groups.push_back({1, 0});                     // the open cells in the bulk kernel
-        groups.push_back({1, 2});                     // ... but those on the x faces, in groups
-        groups.push_back({storageGroup, 1});
-        groups.push_back({storageGroup == 8 ? 16 : 8, 1});
+    bool fused = false, rotors = false;
+    for (int L = 0; L < s.levels; ++L) {
+        fused = fused || s.L[L].fusedOpen;
+        rotors = rotors || !s.L[L].rotors.empty();
     }
+    const int storageGroup = open_group_cells((int)storage_bytes(s.storage));
+    // (group width, x-face choice) layouts of the open cells
+    std::vector<std::pair<int, int>> open = {{storageGroup, 0}};
+    if (fused) {
+        open.push_back({storageGroup == 8 ? 16 : 8, 0});
+        open.push_back({1, 0});                     // the open cells in the bulk kernel
+        open.push_back({1, 2});                     // ... but those on the x faces, in groups
+        open.push_back({storageGroup, 1});
+        open.push_back({storageGroup == 8 ? 16 : 8, 1});
+    }
+    struct Layout { int group, split, rotor; };
+    std::vector<Layout> groups;
+    for (int r = 0; r < (rotors ? 2 : 1); ++r)
+        for (const auto &o : open) groups.push_back({o.first, o.second, r});
     const int ldsChoices[4] = {0, 24576, 40960, 65536};
     const Sim::Tune start = s.tune;
     Sim::Tune best = start;
@@ -1068,10 +1085,11 @@ int lbm3_autotune(lbm3_sim *h, int steps, double *out, int cap, int *nOut)
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
     int rc = 0;
-    for (const auto &gs : groups) {
-        const int g = gs.first;
+    for (const Layout &gs : groups) {
+        const int g = gs.group;
         s.tune.openGroup = g;
-        s.tune.openSplit = gs.second;
+        s.tune.openSplit = gs.split;
+        s.tune.rotorInBulk = gs.rotor;
         if ((rc = regroup(s))) break;
         for (int lds : ldsChoices)
             for (int remap = 0; remap < 2 && !rc; ++remap)
@@ -1088,8 +1106,8 @@ int lbm3_autotune(lbm3_sim *h, int steps, double *out, int cap, int *nOut)
                     if ((rc = check_hip(hipEventElapsedTime(&ms, e0, e1), "event time"))) break;
                     const double per = (double)ms / steps;
                     if (rows < cap) {
-                        double *o = out + 6 * (size_t)rows;
-                        o[0] = lds; o[1] = remap; o[2] = overlap; o[3] = g; o[4] = gs.second; o[5] = per;
+                        double *o = out + 7 * (size_t)rows;
+                        o[0] = lds; o[1] = remap; o[2] = overlap; o[3] = g; o[4] = gs.split; o[5] = gs.rotor; o[6] = per;
                     }
                     ++rows;
                     if (per < bestMs) { bestMs = per; best = s.tune; }
@@ -1112,32 +1130,35 @@ int lbm3_autotune(lbm3_sim *h, int steps, double *out, int cap, int *nOut)
     return relayout ? regroup(s) : 0;
 }
 
-int lbm3_get_tune(lbm3_sim *h, int *out5)
+int lbm3_get_tune(lbm3_sim *h, int *out6)
 {
-    if (!h || !out5) return set_error(LBM3_E_ARG, "null argument");
+    if (!h || !out6) return set_error(LBM3_E_ARG, "null argument");
     const Sim::Tune &t = h->s.tune;
-    out5[0] = t.bulkLds; out5[1] = t.xcdRemap; out5[2] = t.overlap;
-    out5[3] = t.openGroup ? t.openGroup : open_group_cells((int)storage_bytes(h->s.storage));
-    out5[4] = t.openSplit;
+    out6[0] = t.bulkLds; out6[1] = t.xcdRemap; out6[2] = t.overlap;
+    out6[3] = t.openGroup ? t.openGroup : open_group_cells((int)storage_bytes(h->s.storage));
+    out6[4] = t.openSplit;
+    out6[5] = t.rotorInBulk;
     return 0;
 }
 
-int lbm3_set_tune(lbm3_sim *h, const int *in5)
+int lbm3_set_tune(lbm3_sim *h, const int *in6)
 {
-    if (!h || !in5) return set_error(LBM3_E_ARG, "null argument");
-    if (in5[0] < 0 || in5[0] > 65536 || in5[2] < 0 || in5[2] > 2 || (in5[3] != 1 && in5[3] != 8 && in5[3] != 16) ||
-        in5[4] < 0 || in5[4] > 2)
+    if (!h || !in6) return set_error(LBM3_E_ARG, "null argument");
+    if (in6[0] < 0 || in6[0] > 65536 || in6[2] < 0 || in6[2] > 2 || (in6[3] != 1 && in6[3] != 8 && in6[3] != 16) ||
+        in6[4] < 0 || in6[4] > 2 || in6[5] < 0 || in6[5] > 1)
         return set_error(LBM3_E_ARG, "bulk LDS in 0..65536 bytes, side streams 0..2, open groups of 1 (in the bulk "
-                                     "kernel), 8 or 16 cells, x-face choice 0..2");
+                                     "kernel), 8 or 16 cells, x-face choice 0..2, rotor cells 0..1");
     Sim &s = h->s;
     if (s.timing) return set_error(LBM3_E_STATE, "launch choices changed during a timed run");
     const Sim::Tune was = s.tune;
-    s.tune.bulkLds = in5[0];
-    s.tune.xcdRemap = in5[1] != 0;
-    s.tune.overlap = in5[2];
-    s.tune.openGroup = in5[3];
-    s.tune.openSplit = in5[4];
-    // only a new open-cell layout completes a pending condition (regroup)
+    s.tune.bulkLds = in6[0];
+    s.tune.xcdRemap = in6[1] != 0;
+    s.tune.overlap = in6[2];
+    s.tune.openGroup = in6[3];
+    s.tune.openSplit = in6[4];
+    s.tune.rotorInBulk = in6[5];
+    // only a new layout of the open or rotor cells rebuilds the records, which
+    // completes a pending open condition first (regroup)
     return same_open_layout(s, was, s.tune) ? 0 : regroup(s);
 }
 
@@ -1406,6 +1427,7 @@ int lbm3_download_array(lbm3_sim *h, int level, const char *name, void *out, lon
     ARR("wallMap", l.wallMap, l.N) ARR("wallIndex", l.wallIndex, l.W) ARR("wallData", l.wallData, l.W)
     ARR("cellRow", l.cellRow, l.N)
     ARR("bulkSkip", l.bulkSkip, (l.N + 63) / 64) ARR("openBits", l.openBits, l.openInBulk ? (l.N + 63) / 64 : 0)
+    ARR("rotorBits", l.rotorBits, l.rotorInBulk ? (l.N + 63) / 64 : 0)
     ARR("linkLength", l.linkLength, 7LL * l.W) ARR("periodicIndex", l.perIndex, l.nPeriodic)
     ARR("periodicSource", l.perSource, l.nPeriodic) ARR("openIndex", l.openCell, l.nOpen)
     ARR("openSource", l.openSource, l.nOpen) ARR("openMeta", l.openMeta, l.nOpen)

@@ -51,6 +51,7 @@ struct FastTopo {
     const int *tileRow;    // [N / 256 rounded up] row of cell 256 t
     int nRows;
     const uint64_t *openBits;  // as skip: open cells the bulk kernel completes and steps itself; null: none
+    const uint64_t *rotorBits; // as skip: cells the rotors can reach, stepped by the bulk kernel; null: none
 };
 
 __device__ __forceinline__ void decode_fast(const FastTopo &t, int cell, int row, int (&nb)[8])

@@ -15,7 +15,7 @@ template <class S> inline StepArgs<S> step_args(Sim &s, Level &l)
     a.f = reinterpret_cast<S *>(l.f.p);
     a.topo = topo_of(l);
     a.fast = FastTopo{l.N, l.cellRow.p, l.rowTab.p, l.bulkSkip.p, l.tileRow.p, l.nRows,
-                      l.openInBulk ? l.openBits.p : nullptr};
+                      l.openInBulk ? l.openBits.p : nullptr, l.rotorInBulk ? l.rotorBits.p : nullptr};
     a.wallMap = l.wallMap.p;
     a.wallIndex = l.wallIndex.p;
     a.wallData = l.wallData.p;

@@ -88,6 +88,9 @@ struct Level {
     // cells itself (openBits marks them)
     bool openInBulk = false;
     DBuf<uint64_t> openBits;
+    // Sim::Tune::rotorInBulk: the bulk kernel steps the rotor cells (rotorBits)
+    bool rotorInBulk = false;
+    DBuf<uint64_t> rotorBits;
     // with Sim::Tune::openSplit: the open-list entries the open-cell kernel
     // completes (fused) and those on the x faces, completed by a pass after
     // the step's streaming (separate)
@@ -153,6 +156,7 @@ struct Sim {
         // and completed by a pass after streaming; 2 (with openGroup 1) in
         // groups of the open-cell kernel while the bulk kernel takes the others
         int openSplit = 0;
+        int rotorInBulk = 0;          // the rotor cells stepped by the bulk kernel instead of their own
     } tune;
     // graph capture: each level step takes the next slot of this device table
     // for the values that change between replays; a dry run only collects them

@@ -214,11 +214,9 @@ __device__ __forceinline__ bool rotor_locate(const RotorDev &R, T c, T s, T x, T
 // capped at 1 and scales the collision limiter down; the forcing is tracked,
 // turned into the rotor frame, per force cell in FP64
 template <class S, class T>
-__device__ __forceinline__ void rotors_apply(const StepArgs<S> &a, int cell, T dRho, T ux, T uy, T uz, Relax<T> &R,
-                                             bool track)
+__device__ __forceinline__ void rotors_apply_at(const StepArgs<S> &a, const int (&ijk)[3], T dRho, T ux, T uy, T uz,
+                                                Relax<T> &R, bool track)
 {
-    int nbx[8], ijk[3];
-    decode(a.topo, cell, nbx, ijk);
     const T x = T(float(ijk[0]) * a.geom.res + a.geom.ox), y = T(float(ijk[1]) * a.geom.res + a.geom.oy),
             z = T(float(ijk[2]) * a.geom.res + a.geom.oz);
     const T rho = dRho + T(1), rhoInv = T(1) / rho;
@@ -263,6 +261,50 @@ __device__ __forceinline__ void rotors_apply(const StepArgs<S> &a, int cell, T d
     }
 }
 
+template <class S, class T>
+__device__ __forceinline__ void rotors_apply(const StepArgs<S> &a, int cell, T dRho, T ux, T uy, T uz, Relax<T> &R,
+                                             bool track)
+{
+    int nbx[8], ijk[3];
+    decode(a.topo, cell, nbx, ijk);
+    rotors_apply_at<S, T>(a, ijk, dRho, ux, uy, uz, R, track);
+}
+
+// grid coordinates of a cell from its row (FastTopo and Topology number rows alike)
+__device__ __forceinline__ void cell_ijk(const Topology &t, const FastTopo &ft, int cell, int row, int (&ijk)[3])
+{
+    ijk[0] = t.ri[row] + (cell - ft.rows[row].x);
+    ijk[1] = t.rj[row];
+    ijk[2] = t.rk[row];
+}
+
+// the collision of a bulk cell; ROTOR: the level's rotor cells are the bulk
+// kernel's too (FastTopo::rotorBits): a wave holding one takes the collision
+// with run-time K15/K17 choice, and its rotor lanes add the rotors' forcing
+// and scaled limiter first, as k_rotor_cells does
+template <class S, int BULK, bool ROTOR>
+__device__ __forceinline__ void bulk_collide(const StepArgs<S> &a, int cell, int row,
+                                             typename StoreTraits<S>::compute (&f)[27])
+{
+    using T = typename StoreTraits<S>::compute;
+    if constexpr (ROTOR) {
+        const uint64_t m = a.fast.rotorBits[__builtin_amdgcn_readfirstlane(cell >> 6)];
+        if (m) {
+            Relax<T> R = relax_of<T>(a.bulk);
+            if ((m >> (cell & 63)) & 1ull) {
+                T dRho, ux, uy, uz;
+                moments(f, dRho, ux, uy, uz);
+                int ijk[3];
+                cell_ijk(a.topo, a.fast, cell, row, ijk);
+                rotors_apply_at<S, T>(a, ijk, dRho, ux, uy, uz, R, a.trackRotor && a.wallMap[cell] != -2);
+            }
+            collide<T, 0>(f, R);
+            return;
+        }
+    }
+    collide<T, BULK>(f, relax_of<T>(a.bulk));
+}
+
 // Cells are independent under esotwist streaming (each one reads and writes
 // only its own 27 slots), so the level is updated by kernels over disjoint cell
 // sets: one over all cells that skips the special ones, one over the short list
@@ -299,7 +341,7 @@ __device__ __forceinline__ void open_in_bulk(const StepArgs<S> &a, int cell, typ
     }
 }
 
-template <class S, bool FLIP, int BULK, class Off, bool OPEN = false>
+template <class S, bool FLIP, int BULK, class Off, bool OPEN = false, bool ROTOR = false>
 __device__ __forceinline__ void bulk_cell(const StepArgs<S> &a, int cell)
 {
     using T = typename StoreTraits<S>::compute;
@@ -334,7 +376,7 @@ __device__ __forceinline__ void bulk_cell(const StepArgs<S> &a, int cell)
         for (int q = 0; q < 27; ++q)
             f[q] = StoreTraits<S>::bload(r, off[pre_nb(q)], pre_plane(q, FLIP) * LBM3_TILE * int(sizeof(S)), q, a.cd);
         open_in_bulk<S, OPEN>(a, cell, f);
-        collide<T, BULK>(f, relax_of<T>(a.bulk));
+        bulk_collide<S, BULK, ROTOR>(a, cell, row, f);
         if constexpr (sizeof(S) >= 4 || LBM3_BUF16 == 1) {
 #pragma unroll
             for (int q = 0; q < 27; ++q)
@@ -349,7 +391,7 @@ __device__ __forceinline__ void bulk_cell(const StepArgs<S> &a, int cell)
         for (int q = 0; q < 27; ++q)
             f[q] = StoreTraits<S>::load(at(a.f + pre_plane(q, FLIP) * LBM3_TILE, off[pre_nb(q)]), q, a.cd);
         open_in_bulk<S, OPEN>(a, cell, f);
-        collide<T, BULK>(f, relax_of<T>(a.bulk));
+        bulk_collide<S, BULK, ROTOR>(a, cell, row, f);
 #pragma unroll
         for (int q = 0; q < 27; ++q)
             StoreTraits<S>::store(at(a.f + post_plane(q, FLIP) * LBM3_TILE, off[post_nb(q)]), f[q], q, a.cd);
@@ -357,14 +399,15 @@ __device__ __forceinline__ void bulk_cell(const StepArgs<S> &a, int cell)
 }
 
 // WIDE: the level's population array exceeds 4 GiB (64-bit lane offsets);
-// OPEN: the open cells are stepped here too (never with WIDE)
-template <class S, bool FLIP, int BULK, bool WIDE, bool OPEN = false>
+// OPEN: the open cells are stepped here too; ROTOR: the rotor cells (neither
+// with WIDE)
+template <class S, bool FLIP, int BULK, bool WIDE, bool OPEN = false, bool ROTOR = false>
 __global__ __launch_bounds__(256) void k_collide_stream(StepArgs<S> a)
 {
     const int cell = xcd_block(blockIdx.x, gridDim.x, a.xcdRemap) * 256 + threadIdx.x;
     if (cell >= a.fast.N) return;
     if (WIDE) bulk_cell<S, FLIP, BULK, uint64_t>(a, cell);
-    else bulk_cell<S, FLIP, BULK, uint32_t, OPEN>(a, cell);
+    else bulk_cell<S, FLIP, BULK, uint32_t, OPEN, ROTOR>(a, cell);
 }
 
 // Open-boundary cells (never wall-adjacent when this kernel is used): load,
@@ -424,29 +467,32 @@ __global__ __launch_bounds__(256) void k_mark_groups(const int *groups, int coun
 // row of the first one; with `open` non-null the open cells (record <= -8) are
 // the bulk kernel's too, marked in `open`
 __global__ __launch_bounds__(256) void k_bulk_mask(const int2 *cellRow, int N, uint64_t *skip, int *tileRow,
-                                                   uint64_t *open)
+                                                   uint64_t *open, uint64_t *rotor)
 {
     const int cell = blockIdx.x * 256 + threadIdx.x;
-    bool sk = true, op = false;
+    bool sk = true, op = false, rt = false;
     int row = 0;
     if (cell < N) {
         const int2 r = cellRow[cell];
         op = open && r.x <= -8;
-        sk = (r.x >= 0 || r.x <= -3) && !op;
+        rt = rotor && r.x == -4;
+        sk = (r.x >= 0 || r.x <= -3) && !op && !rt;
         row = r.y;
     }
-    const unsigned long long m = __ballot(sk), o = __ballot(op);
+    const unsigned long long m = __ballot(sk), o = __ballot(op), q = __ballot(rt);
     if ((threadIdx.x & 63) == 0) {
         skip[cell >> 6] = m;
         if (open) open[cell >> 6] = o;
+        if (rotor) rotor[cell >> 6] = q;
     }
     if (threadIdx.x == 0) tileRow[blockIdx.x] = row;
 }
 
-void launch_bulk_mask(const int2 *cellRow, int N, uint64_t *skip, int *tileRow, uint64_t *open, hipStream_t s)
+void launch_bulk_mask(const int2 *cellRow, int N, uint64_t *skip, int *tileRow, uint64_t *open, uint64_t *rotor,
+                      hipStream_t s)
 {
     if (N <= 0) return;
-    k_bulk_mask<<<(N + 255) / 256, 256, 0, s>>>(cellRow, N, skip, tileRow, open);
+    k_bulk_mask<<<(N + 255) / 256, 256, 0, s>>>(cellRow, N, skip, tileRow, open, rotor);
 }
 
 // the cells of the open groups leave the bulk kernel's view: skip bits set,
@@ -483,8 +529,9 @@ __global__ __launch_bounds__(256) void k_rotor_cells(StepArgs<S> a)
     const int x = blockIdx.x * 256 + threadIdx.x;
     if (x >= a.rotorCellCount) return;
     const int cell = a.rotorCells[x];
+    const int row = a.fast.cellRow[cell].y;
     int nb[8];
-    decode_fast(a.fast, cell, a.fast.cellRow[cell].y, nb);
+    decode_fast(a.fast, cell, row, nb);
     T f[27];
 #pragma unroll
     for (int q = 0; q < 27; ++q) f[q] = StoreTraits<S>::load(a.f + pop_index(pre_plane(q, FLIP), nb[pre_nb(q)]), q, a.cd);
@@ -492,7 +539,9 @@ __global__ __launch_bounds__(256) void k_rotor_cells(StepArgs<S> a)
     moments(f, dRho, ux, uy, uz);
     Relax<T> R = relax_of<T>(a.bulk);
     const int wm = a.wallMap[cell];
-    rotors_apply<S, T>(a, cell, dRho, ux, uy, uz, R, a.trackRotor && wm != -2);
+    int ijk[3];
+    cell_ijk(a.topo, a.fast, cell, row, ijk);
+    rotors_apply_at<S, T>(a, ijk, dRho, ux, uy, uz, R, a.trackRotor && wm != -2);
     collide<T, 0>(f, R);
 #pragma unroll
     for (int q = 0; q < 27; ++q)
@@ -1061,7 +1110,11 @@ void launch_collide_stream(const StepArgs<S> &a, bool flip, hipStream_t s)
     const bool wide = wide_level<S>(a.fast.N);
 #define LBM3_BULK_LAUNCH(F, K, W) hipLaunchKernelGGL((k_collide_stream<S, F, K, W>), g, b, lds, s, a)
 #define LBM3_BULK_OPEN(F, K) hipLaunchKernelGGL((k_collide_stream<S, F, K, false, true>), g, b, lds, s, a)
-    if (a.fast.openBits && !wide) {
+#define LBM3_BULK_ROTOR(F, K) hipLaunchKernelGGL((k_collide_stream<S, F, K, false, false, true>), g, b, lds, s, a)
+    if (a.fast.rotorBits && !wide) {
+        if (flip) { if (k17) LBM3_BULK_ROTOR(true, 1); else LBM3_BULK_ROTOR(true, 2); }
+        else { if (k17) LBM3_BULK_ROTOR(false, 1); else LBM3_BULK_ROTOR(false, 2); }
+    } else if (a.fast.openBits && !wide) {
         if (flip) { if (k17) LBM3_BULK_OPEN(true, 1); else LBM3_BULK_OPEN(true, 2); }
         else { if (k17) LBM3_BULK_OPEN(false, 1); else LBM3_BULK_OPEN(false, 2); }
     } else if (wide) {
@@ -1073,6 +1126,7 @@ void launch_collide_stream(const StepArgs<S> &a, bool flip, hipStream_t s)
     }
 #undef LBM3_BULK_LAUNCH
 #undef LBM3_BULK_OPEN
+#undef LBM3_BULK_ROTOR
 }
 
 template <class S>

@@ -91,7 +91,8 @@ void launch_fast_topo(const Topology &t, const int *wallMap, const int *openOf, 
                       int4 *rows, hipStream_t s);
 void launch_scatter_index(const int *cell, int count, int *openOf, hipStream_t s);
 void launch_scatter_entries(const int *cell, const int *list, int count, int *openOf, hipStream_t s);
-void launch_bulk_mask(const int2 *cellRow, int N, uint64_t *skip, int *tileRow, uint64_t *open, hipStream_t s);
+void launch_bulk_mask(const int2 *cellRow, int N, uint64_t *skip, int *tileRow, uint64_t *open, uint64_t *rotor,
+                      hipStream_t s);
 void launch_group_bits(const int *groups, int count, int G, uint64_t *skip, uint64_t *open, hipStream_t s);
 void launch_mark_groups(const int *groups, int count, int G, int N, int2 *cellRow, hipStream_t s);
 void launch_rotor_reach(const Topology &t, const LevelGeom &g, const RotorReachSet &R, uint8_t *cand, hipStream_t s);

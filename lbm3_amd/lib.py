@@ -302,22 +302,22 @@ class Sim:
     def autotune(self, steps=10):
         """lbm3_autotune: time every launch choice over `steps` coarse steps,
         keep the fastest, leave the state as it was.  Returns (chosen
-        {bulk_lds, xcd_remap, overlap, open_group, open_split}, [(lds, remap,
-        overlap, group, split, ms per step), ...])."""
-        out = np.zeros((128, 6))
+        {bulk_lds, xcd_remap, overlap, open_group, open_split, rotor_in_bulk},
+        [(lds, remap, overlap, group, split, rotor, ms per step), ...])."""
+        out = np.zeros((256, 7))
         n = C.c_int()
-        check(load().lbm3_autotune(self.h, steps, out.ctypes.data, 128, C.byref(n)))
-        return self.tune(), [tuple(r) for r in out[:min(n.value, 128)]]
+        check(load().lbm3_autotune(self.h, steps, out.ctypes.data, 256, C.byref(n)))
+        return self.tune(), [tuple(r) for r in out[:min(n.value, 256)]]
 
-    TUNE = ("bulk_lds", "xcd_remap", "overlap", "open_group", "open_split")
+    TUNE = ("bulk_lds", "xcd_remap", "overlap", "open_group", "open_split", "rotor_in_bulk")
 
     def tune(self):
-        t = np.zeros(5, np.int32)
+        t = np.zeros(6, np.int32)
         check(load().lbm3_get_tune(self.h, t.ctypes.data))
         return dict(zip(self.TUNE, (int(x) for x in t)))
 
-    def set_tune(self, bulk_lds, xcd_remap, overlap, open_group, open_split=0):
-        t = np.array([bulk_lds, xcd_remap, overlap, open_group, open_split], np.int32)
+    def set_tune(self, bulk_lds, xcd_remap, overlap, open_group, open_split=0, rotor_in_bulk=0):
+        t = np.array([bulk_lds, xcd_remap, overlap, open_group, open_split, rotor_in_bulk], np.int32)
         check(load().lbm3_set_tune(self.h, t.ctypes.data))
 
     def wall_forces(self, level=0, origin=(0.0, 0.0, 0.0), bodies=1):

@@ -22,7 +22,7 @@ def test_autotune_keeps_the_state(storage):
     chosen, table = s.autotune(steps=3)
     assert s.info(0).iterationsFinished == 7
     assert len(table) in (16, 96)
-    assert any(tuple(int(v) for v in r[:5]) == tuple(chosen[k] for k in lib.Sim.TUNE) for r in table)
+    assert any(tuple(int(v) for v in r[:6]) == tuple(chosen[k] for k in lib.Sim.TUNE) for r in table)
     s.run(13)
     s.sync()
     got = s.macros(0)
@@ -38,7 +38,7 @@ def test_autotune_keeps_the_state(storage):
     ref.close()
     err = max(rel_max(1 + got[:, 0], 1 + want[:, 0]), rel_max(got[:, 1:], want[:, 1:]))
     print("%s autotune chose %s (best %.4f ms/step of %d); tuned vs untuned run %.2e" %
-          (storage, chosen, min(r[4] for r in table), len(table), err))
+          (storage, chosen, min(r[-1] for r in table), len(table), err))
     assert err < 1e-6
 
 
@@ -49,7 +49,8 @@ def test_set_tune_round_trip():
     s.init()
     s.run(2)
     s.set_tune(24576, 1, 0, 16, 1)
-    assert s.tune() == {"bulk_lds": 24576, "xcd_remap": 1, "overlap": 0, "open_group": 16, "open_split": 1}
+    assert s.tune() == {"bulk_lds": 24576, "xcd_remap": 1, "overlap": 0, "open_group": 16, "open_split": 1,
+                        "rotor_in_bulk": 0}
     s.run(3)
     s.sync()
     assert np.isfinite(s.macros(0)).all()

@@ -70,18 +70,22 @@ def _errors(o, g, o64=None):
 LIMITER_NU = [(0.0, 0.05), (0.1, 0.01)]
 
 
-@pytest.mark.parametrize("limiter,nu", LIMITER_NU)
-def test_turning_rotor_f64(limiter, nu):
+@pytest.mark.parametrize("limiter,nu,in_bulk", [lnu + (b,) for lnu in LIMITER_NU for b in (0, 1)])
+def test_turning_rotor_f64(limiter, nu, in_bulk):
+    """in_bulk 1: the rotor cells stepped by the bulk kernel (lbm3_set_tune)."""
     case, dom, _ = cases.periodic_box(24, u=(0.01, 0.0, 0.005), limiter=limiter, nu=nu)
     case.trackRotorForce = 1
     o, g = _pair(case, dom, [], _star(), "f64", T.AXIS_Z, AXIS_ORIGIN)
+    g.set_tune(40960, 0, 1, 8, 0, in_bulk)
     o.run(100)
     g.run(100)
     g.sync()
     r, u = _errors(o, g)
     print("rotor f64 limiter %.1f: rho %.2e u %.2e" % (limiter, r, u))
     assert r < 1e-10 and u < 1e-10
-    assert g.cell_kinds(0)["rotor"] > 0
+    k = g.cell_kinds(0)
+    assert k["rotor"] > 0
+    assert k["bulk"] == g.info(0).cellCount - k["wall"] - k["solid"] - (0 if in_bulk else k["rotor"])
     fo, fg = o.rotor_forces(0, 0), g.rotor_forces(0, 0)
     assert np.max(np.abs(fg - fo)) < 1e-9 * np.max(np.abs(fo))
     assert abs(g.rotor_info(0, 0)[10] - o.rotor_info(0, 0)[10]) < 1e-12
@@ -190,3 +194,44 @@ def test_seed_rotor_of_the_duct_config_bit_exact():
     print("seed-0 rotor: %d triangles, marked blocks per level %s" %
           (info["triangles"], [int(o.rotor_info(L, 0)[11]) for L in range(2)]))
     g.close()
+
+
+@pytest.mark.parametrize("storage", ["f32", "f16"])
+def test_rotor_cells_in_the_bulk_kernel_equal_their_own_kernel(storage):
+    """The rotor duct on two levels (rotor on both) with the rotor cells in the
+    bulk kernel or in k_rotor_cells: the same arithmetic per cell (collision
+    with run-time K15/K17 choice, forcing and limiter scaling on the rotor
+    cells), compiled into two kernels whose multiply-adds the compiler may
+    fuse differently, so the fields agree to the storage's rounding (FP64
+    equals the oracle to 1e-15 either way: test_turning_rotor_f64); the rotor
+    torque likewise, the rotor angles exactly."""
+    case, dom, statics, rotor, info = cases.rotor_duct(48, 24, 24, seed=1, levels=2, n_theta=128, n_axial=8)
+    runs = []
+    for in_bulk in (0, 1):
+        s = lib.Sim(case, storage)
+        s.build(dom, statics)
+        s.build_rotors([rotor])
+        for L in range(2):
+            s.set_rotor(L, 0, info["omega"], info["axis"], info["origin"])
+        s.init()
+        t = s.tune()
+        t["rotor_in_bulk"] = in_bulk
+        s.set_tune(*(t[k] for k in lib.Sim.TUNE))
+        s.run(15)
+        s.sync()
+        k = s.cell_kinds(1)
+        assert k["rotor"] > 0
+        runs.append(([s.macros(L) for L in range(2)], [s.rotor_forces(L, 0) for L in range(2)],
+                     [s.rotor_info(L, 0) for L in range(2)], k))
+        s.close()
+    (m0, f0, i0, k0), (m1, f1, i1, k1) = runs
+    assert k1["bulk"] == k0["bulk"] + k0["rotor"]
+    tol = 1e-5 if storage == "f32" else 2e-3
+    for L, (x, y) in enumerate(zip(m0, m1)):
+        err = max(rel_max(1 + y[:, 0], 1 + x[:, 0]), rel_max(y[:, 1:], x[:, 1:]))
+        print("%s level %d: rotor cells in the bulk kernel vs their own kernel %.2e" % (storage, L, err))
+        assert err < tol
+    for x, y in zip(i0, i1):
+        assert np.array_equal(x, y)
+    for x, y in zip(f0, f1):
+        assert np.abs(x - y).max() <= 50 * tol * np.abs(x).max()
