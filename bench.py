@@ -293,32 +293,40 @@ def cpu_baseline(threads, seconds=4.0, runs=3, single_seconds=6.0):
 
 
 def traffic_from_profiles(storage, bulk_cells):
-    """Measured HBM bytes per bulk launch, if the counters were taken on the same cell split."""
+    """Measured HBM bytes per bulk launch, if the counters were taken on the same
+    cell split (open cells in groups: key `storage`; in the bulk kernel:
+    `storage`_in_bulk)."""
     path = os.path.join(ROOT, "profiles", "hbm_traffic_sphere.json")
     if not os.path.exists(path):
         return None
     with open(path) as fh:
         d = json.load(fh)
-    e = d.get(storage)
-    return None if e is None or e.get("cells") != bulk_cells else e.get("bytes_per_launch")
+    for key in (storage, storage + "_in_bulk"):
+        e = d.get(key)
+        if e is not None and e.get("cells") == bulk_cells:
+            return e.get("bytes_per_launch")
+    return None
 
 
 def counters_workload():
-    """A short fixed sequence for rocprofv3 --pmc runs: the sphere workload for
-    10 steps in FP32, FP16 and BF16, then three 1 GiB copies (the calibration
-    kernel whose bytes are known)."""
+    """A short sequence for rocprofv3 --pmc runs: the sphere workload in FP32,
+    FP16 and BF16, each autotuned as the bench does (2 steps per candidate),
+    then 10 steps whose dispatches are the last 10 of each step kernel; then
+    three 1 GiB copies (the calibration kernel whose bytes are known)."""
     from lbm3_amd import lib
     for st in ("f32", "f16", "bf16"):
         case, dom, tris = workload()
         s = lib.Sim(case, st)
         s.build(dom, tris)
         s.init()
+        s.run(2)
+        tuned, _ = s.autotune(steps=2)
         s.run(10)
         s.sync()
         k = s.cell_kinds(0)
         s.close()
-        print("counters workload: %s sphere, 10 steps, cells bulk %d open %d open_group %d"
-              % (st, k["bulk"], k["open"], k["open_group"]), flush=True)
+        print("counters workload: %s sphere, tune %s, then 10 steps; cells bulk %d (open in it %d) open %d open_group %d"
+              % (st, tuned, k["bulk"], k["open_in_bulk"], k["open"], k["open_group"]), flush=True)
     lib.copy_probe(1 << 30, 3)
     print("counters workload: 3 copies of 1 GiB", flush=True)
 
