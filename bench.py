@@ -30,7 +30,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBS = 8000.0
-SBYTES = {"f32": 4, "f16": 2, "bf16": 2, "f64": 8}
+SBYTES = {"f32": 4, "f16": 2, "bf16": 2, "f64": 8, "f16c": 2}
 
 
 def workload(k17=False):
@@ -111,7 +111,7 @@ def sphere_run(storage, steps, reynolds=100.0, window=1000):
     return u, float(cd)
 
 
-def accuracy_runs(steps, storages=("f32", "f16", "bf16")):
+def accuracy_runs(steps, storages=("f32", "f16", "bf16", "f16c")):
     """Relative L2 velocity error of each storage precision against the FP64
     run of the same kernels after `steps` steps of the sphere workload, and the
     FP64 drag coefficient at Re 100 and 300 (Schiller-Naumann: 1.09 and 0.69
@@ -353,7 +353,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=100)
-    ap.add_argument("--variants", default="f16,bf16,k17")
+    ap.add_argument("--variants", default="f16,bf16,f16c,k17")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--accuracy-steps", type=int, default=5000,

@@ -191,6 +191,11 @@ typedef struct lbm3_case {
 #define LBM3_STORE_F16 1
 #define LBM3_STORE_BF16 2
 #define LBM3_STORE_F64 3
+/* 16 bits: sign, 4 exponent bits, 11 mantissa bits -- one mantissa bit more
+ * than F16 over the range the shifted encoding uses (|deviation| < 2, larger
+ * values saturate); slower to encode and decode than F16's hardware
+ * conversions.  No counterpart in the reference. */
+#define LBM3_STORE_F16C 4
 
 #ifdef __cplusplus
 }

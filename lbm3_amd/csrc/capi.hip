@@ -35,7 +35,7 @@ int check_hip(hipError_t e, const char *what)
 size_t storage_bytes(int storage)
 {
     switch (storage) {
-    case LBM3_STORE_F16: case LBM3_STORE_BF16: return 2;
+    case LBM3_STORE_F16: case LBM3_STORE_BF16: case LBM3_STORE_F16C: return 2;
     case LBM3_STORE_F64: return 8;
     default: return 4;
     }
@@ -44,13 +44,17 @@ size_t compute_bytes(int storage) { return storage == LBM3_STORE_F64 ? 8 : 4; }
 
 // 16-bit encoding: deviation from the shifted equilibrium at rho = 1 and the
 // case's reference velocity, in units of the weight
-Codec make_codec(const lbm3_case &c)
+Codec make_codec(const lbm3_case &c, int storage)
 {
     Codec k;
     for (int q = 0; q < 27; ++q) {
         k.off[q] = feq_shifted<float>(q, 0.f, c.storeRef[0], c.storeRef[1], c.storeRef[2]);
         k.w[q] = weight<float>(q);
         k.iw[q] = 1.f / k.w[q];
+        if (storage == LBM3_STORE_F16C) {   // the f16c bit pattern is that of the value times 2^-112
+            k.w[q] = std::ldexp(k.w[q], 112);
+            k.iw[q] = std::ldexp(k.iw[q], -112);
+        }
     }
     return k;
 }
@@ -370,6 +374,7 @@ static int settle_open(Sim &s, int L)
     switch (s.storage) {
     case LBM3_STORE_F16: settle_typed<__half>(s, l); break;
     case LBM3_STORE_BF16: settle_typed<__hip_bfloat16>(s, l); break;
+    case LBM3_STORE_F16C: settle_typed<f16c>(s, l); break;
     case LBM3_STORE_F64: settle_typed<double>(s, l); break;
     default: settle_typed<float>(s, l); break;
     }
@@ -496,6 +501,7 @@ static void step_level_any(Sim &s, int level)
     switch (s.storage) {
     case LBM3_STORE_F16: step_level<__half>(s, level); break;
     case LBM3_STORE_BF16: step_level<__hip_bfloat16>(s, level); break;
+    case LBM3_STORE_F16C: step_level<f16c>(s, level); break;
     case LBM3_STORE_F64: step_level<double>(s, level); break;
     default: step_level<float>(s, level); break;
     }
@@ -555,6 +561,7 @@ static int init_level_any(Sim &s, int level, const float *fieldsDev)
     switch (s.storage) {
     case LBM3_STORE_F16: return init_level<__half>(s, level, fieldsDev);
     case LBM3_STORE_BF16: return init_level<__hip_bfloat16>(s, level, fieldsDev);
+    case LBM3_STORE_F16C: return init_level<f16c>(s, level, fieldsDev);
     case LBM3_STORE_F64: return init_level<double>(s, level, fieldsDev);
     default: return init_level<float>(s, level, fieldsDev);
     }
@@ -685,13 +692,13 @@ int lbm3_sim_create(const lbm3_case *c, int storage, lbm3_sim **out)
 {
     if (!c || !out) return set_error(LBM3_E_ARG, "null argument");
     if (c->gridLevelCount < 1 || c->gridLevelCount > LBM3_MAX_LEVELS) return set_error(LBM3_E_ARG, "gridLevelCount out of range");
-    if (storage < LBM3_STORE_F32 || storage > LBM3_STORE_F64) return set_error(LBM3_E_ARG, "unknown storage");
+    if (storage < LBM3_STORE_F32 || storage > LBM3_STORE_F16C) return set_error(LBM3_E_ARG, "unknown storage");
     lbm3_sim *h = new (std::nothrow) lbm3_sim();
     if (!h) return set_error(LBM3_E_MEMORY, "host allocation");
     h->s.C = *c;
     h->s.storage = storage;
     h->s.levels = c->gridLevelCount;
-    h->s.codec = make_codec(*c);
+    h->s.codec = make_codec(*c, storage);
     int rc = check_hip(hipStreamCreateWithFlags(&h->s.stream, hipStreamNonBlocking), "stream");
     if (!rc) rc = check_hip(hipStreamCreateWithFlags(&h->s.aux, hipStreamNonBlocking), "stream");
     if (!rc) rc = check_hip(hipEventCreateWithFlags(&h->s.fork, hipEventDisableTiming), "event");
@@ -1322,6 +1329,7 @@ int lbm3_download_macros(lbm3_sim *h, int level, void *out)
     switch (s.storage) {
     case LBM3_STORE_F16: launch_macros<__half>(reinterpret_cast<__half *>(l.f.p), t, reinterpret_cast<float *>(d.p), s.codec, flip, s.stream); break;
     case LBM3_STORE_BF16: launch_macros<__hip_bfloat16>(reinterpret_cast<__hip_bfloat16 *>(l.f.p), t, reinterpret_cast<float *>(d.p), s.codec, flip, s.stream); break;
+    case LBM3_STORE_F16C: launch_macros<f16c>(reinterpret_cast<f16c *>(l.f.p), t, reinterpret_cast<float *>(d.p), s.codec, flip, s.stream); break;
     case LBM3_STORE_F64: launch_macros<double>(reinterpret_cast<double *>(l.f.p), t, reinterpret_cast<double *>(d.p), s.codec, flip, s.stream); break;
     default: launch_macros<float>(reinterpret_cast<float *>(l.f.p), t, reinterpret_cast<float *>(d.p), s.codec, flip, s.stream); break;
     }
@@ -1500,6 +1508,7 @@ int lbm3_kinetic_energy(lbm3_sim *h, double *out)
         switch (s.storage) {
         case LBM3_STORE_F16: launch_kinetic_energy<__half>(reinterpret_cast<__half *>(l.f.p), t, l.wallMap.p, l.wallData.p, s.partial.p, s.scalar.p, s.codec, flip, s.stream); break;
         case LBM3_STORE_BF16: launch_kinetic_energy<__hip_bfloat16>(reinterpret_cast<__hip_bfloat16 *>(l.f.p), t, l.wallMap.p, l.wallData.p, s.partial.p, s.scalar.p, s.codec, flip, s.stream); break;
+        case LBM3_STORE_F16C: launch_kinetic_energy<f16c>(reinterpret_cast<f16c *>(l.f.p), t, l.wallMap.p, l.wallData.p, s.partial.p, s.scalar.p, s.codec, flip, s.stream); break;
         case LBM3_STORE_F64: launch_kinetic_energy<double>(reinterpret_cast<double *>(l.f.p), t, l.wallMap.p, l.wallData.p, s.partial.p, s.scalar.p, s.codec, flip, s.stream); break;
         default: launch_kinetic_energy<float>(reinterpret_cast<float *>(l.f.p), t, l.wallMap.p, l.wallData.p, s.partial.p, s.scalar.p, s.codec, flip, s.stream); break;
         }
@@ -1547,6 +1556,7 @@ static int mass_sums(Sim &s, double *out2)
         switch (s.storage) {
         case LBM3_STORE_F16: launch_mass<__half>(reinterpret_cast<__half *>(l.f.p), t, l.wallMap.p, l.wallData.p, s.partial.p, s.scalar.p, s.codec, flip, s.stream, w, acc); break;
         case LBM3_STORE_BF16: launch_mass<__hip_bfloat16>(reinterpret_cast<__hip_bfloat16 *>(l.f.p), t, l.wallMap.p, l.wallData.p, s.partial.p, s.scalar.p, s.codec, flip, s.stream, w, acc); break;
+        case LBM3_STORE_F16C: launch_mass<f16c>(reinterpret_cast<f16c *>(l.f.p), t, l.wallMap.p, l.wallData.p, s.partial.p, s.scalar.p, s.codec, flip, s.stream, w, acc); break;
         case LBM3_STORE_F64: launch_mass<double>(reinterpret_cast<double *>(l.f.p), t, l.wallMap.p, l.wallData.p, s.partial.p, s.scalar.p, s.codec, flip, s.stream, w, acc); break;
         default: launch_mass<float>(reinterpret_cast<float *>(l.f.p), t, l.wallMap.p, l.wallData.p, s.partial.p, s.scalar.p, s.codec, flip, s.stream, w, acc); break;
         }
@@ -1744,6 +1754,7 @@ static int section_image(Sim &s, int plane, float cut, const lbm3_bounds *bounds
         switch (s.storage) {
         case LBM3_STORE_F16: cut_level<__half>(s, L, d, img->p); break;
         case LBM3_STORE_BF16: cut_level<__hip_bfloat16>(s, L, d, img->p); break;
+        case LBM3_STORE_F16C: cut_level<f16c>(s, L, d, img->p); break;
         case LBM3_STORE_F64: cut_level<double>(s, L, d, img->p); break;
         default: cut_level<float>(s, L, d, img->p); break;
         }
@@ -1909,6 +1920,7 @@ int lbm3_export_vtk(lbm3_sim *h, const char *path)
         switch (s.storage) {
         case LBM3_STORE_F16: launch_vtk_fields<__half>(reinterpret_cast<const __half *>(l.f.p), t, l.wallMap.p, l.wallData.p, s.codec, flip, l.geom, s.C.rhoPhys, d.p, s.stream); break;
         case LBM3_STORE_BF16: launch_vtk_fields<__hip_bfloat16>(reinterpret_cast<const __hip_bfloat16 *>(l.f.p), t, l.wallMap.p, l.wallData.p, s.codec, flip, l.geom, s.C.rhoPhys, d.p, s.stream); break;
+        case LBM3_STORE_F16C: launch_vtk_fields<f16c>(reinterpret_cast<const f16c *>(l.f.p), t, l.wallMap.p, l.wallData.p, s.codec, flip, l.geom, s.C.rhoPhys, d.p, s.stream); break;
         case LBM3_STORE_F64: launch_vtk_fields<double>(reinterpret_cast<const double *>(l.f.p), t, l.wallMap.p, l.wallData.p, s.codec, flip, l.geom, s.C.rhoPhys, d.p, s.stream); break;
         default: launch_vtk_fields<float>(reinterpret_cast<const float *>(l.f.p), t, l.wallMap.p, l.wallData.p, s.codec, flip, l.geom, s.C.rhoPhys, d.p, s.stream); break;
         }

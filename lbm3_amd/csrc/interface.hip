@@ -339,6 +339,7 @@ void update_interface(Sim &s, int coarse)
     case LBM3_STORE_F64: interface_typed<double>(s, coarse); break;
     case LBM3_STORE_F16: interface_typed<__half>(s, coarse); break;
     case LBM3_STORE_BF16: interface_typed<__hip_bfloat16>(s, coarse); break;
+    case LBM3_STORE_F16C: interface_typed<f16c>(s, coarse); break;
     default: interface_typed<float>(s, coarse); break;
     }
 }

@@ -76,7 +76,7 @@ struct Server {
     std::vector<Request *> active;
     std::map<long long, std::unique_ptr<Request>> all;
     hipStream_t stream = nullptr;
-    DBuf<unsigned char> desc[4];      // device descriptors and block prefixes, one per storage type
+    DBuf<unsigned char> desc[5];      // device descriptors and block prefixes, one per storage type
     Stage stage[8];
     int nextStage = 0;
     long long stepsRun = 0, updatesRun = 0;
@@ -256,7 +256,7 @@ int lbm3_server_submit(lbm3_server *h, const lbm3_case *c, int storage, const fl
     if (!h || !c || !id || steps <= 0 || nBodies < 0 || (nBodies > 0 && (!triangles || !triCounts)))
         return set_error(LBM3_E_ARG, "bad request");
     if (c->gridLevelCount != 1) return set_error(LBM3_E_ARG, "the server runs single-level cases");
-    if (storage < LBM3_STORE_F32 || storage > LBM3_STORE_F64) return set_error(LBM3_E_ARG, "unknown storage");
+    if (storage < LBM3_STORE_F32 || storage > LBM3_STORE_F16C) return set_error(LBM3_E_ARG, "unknown storage");
     Server &sv = h->s;
     auto r = std::make_unique<Request>();
     r->id = sv.nextId++;
@@ -298,12 +298,13 @@ int lbm3_server_step(lbm3_server *h, long long *completed, int cap, int *nComple
         sv.active.push_back(r);
     }
     if (sv.active.empty()) return 0;
-    std::vector<Request *> groups[4];
+    std::vector<Request *> groups[5];
     for (Request *r : sv.active) groups[r->storage].push_back(r);
     int rc = batched_step<float>(sv, LBM3_STORE_F32, groups[LBM3_STORE_F32]);
     if (!rc) rc = batched_step<__half>(sv, LBM3_STORE_F16, groups[LBM3_STORE_F16]);
     if (!rc) rc = batched_step<__hip_bfloat16>(sv, LBM3_STORE_BF16, groups[LBM3_STORE_BF16]);
     if (!rc) rc = batched_step<double>(sv, LBM3_STORE_F64, groups[LBM3_STORE_F64]);
+    if (!rc) rc = batched_step<f16c>(sv, LBM3_STORE_F16C, groups[LBM3_STORE_F16C]);
     if (rc) return rc;
     sv.stepsRun++;
     bool any = false;

@@ -1487,6 +1487,7 @@ LBM3_INSTANTIATE(float)
 LBM3_INSTANTIATE(double)
 LBM3_INSTANTIATE(__half)
 LBM3_INSTANTIATE(__hip_bfloat16)
+LBM3_INSTANTIATE(f16c)
 template void launch_collide_batch<float>(float *, int, const Relax<float> &, hipStream_t);
 template void launch_collide_batch<double>(double *, int, const Relax<double> &, hipStream_t);
 

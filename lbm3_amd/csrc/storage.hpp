@@ -1,5 +1,5 @@
 // Population storage in HBM: shifted populations f - w as float, half,
-// bfloat16 or double.  Arithmetic always runs in the compute type (float for
+// bfloat16, a 16-bit float with 11 mantissa bits (f16c) or double.  Arithmetic always runs in the compute type (float for
 // the 32/16-bit stores, double for the 64-bit store); conversion happens at the
 // load and the store only.
 #pragma once
@@ -171,6 +171,56 @@ template <> struct StoreTraits<__half> {
     {
         const unsigned short bits = __half_as_ushort(__float2half_rn((v - c.off[q]) * c.iw[q]));
         __builtin_amdgcn_raw_buffer_store_b16((short)bits, r, (int)voff, soff, 0);
+    }
+};
+
+// f16c: a 16-bit float with one sign bit, 4 exponent bits (bias 15, no
+// infinities: the largest finite value is just below 2) and 11 mantissa bits,
+// subnormal below 2^-14 -- one mantissa bit more than IEEE half, paid for with
+// the exponent range the deviations of the shifted encoding never use (|x| <
+// 2).  Its bits are those of the float x * 2^-112 from bit 12 to bit 26, whose
+// exponent field is then the 4-bit exponent: decoding shifts them back into a
+// float and takes the 2^112 in the codec's weight (Codec::w holds w * 2^112,
+// Codec::iw holds 2^-112 / w for this storage); encoding rounds the float to
+// 11 mantissa bits (nearest, ties to even) and saturates at the largest
+// finite value.  Both need float subnormals (the kernels keep them).
+struct f16c {
+    unsigned short bits;
+};
+
+__device__ __forceinline__ float f16c_decode(unsigned int h)
+{
+    return __uint_as_float(((h & 0x7FFFu) << 12) | ((h & 0x8000u) << 16));
+}
+
+__device__ __forceinline__ unsigned short f16c_encode(float y)
+{
+    const unsigned int u = __float_as_uint(y);
+    const unsigned int mag = u & 0x7FFFFFFFu;
+    const unsigned int r = min((mag + 0x7FFu + ((mag >> 12) & 1u)) >> 12, 0x7FFFu);
+    return (unsigned short)(r | ((u >> 16) & 0x8000u));
+}
+
+template <> struct StoreTraits<f16c> {
+    using compute = float;
+    static __device__ __forceinline__ float load(const f16c *p, int q, const Codec &c)
+    {
+        const unsigned short bits = ld(reinterpret_cast<const unsigned short *>(p));
+        return fmaf(f16c_decode(bits), c.w[q], c.off[q]);
+    }
+    static __device__ __forceinline__ void store(f16c *p, float v, int q, const Codec &c)
+    {
+        st(reinterpret_cast<unsigned short *>(p), f16c_encode((v - c.off[q]) * c.iw[q]));
+    }
+    static __device__ __forceinline__ float
+    bload(__amdgpu_buffer_rsrc_t r, uint32_t voff, int soff, int q, const Codec &c)
+    {
+        const unsigned short bits = (unsigned short)__builtin_amdgcn_raw_buffer_load_b16(r, (int)voff, soff, 0);
+        return fmaf(f16c_decode(bits), c.w[q], c.off[q]);
+    }
+    static __device__ __forceinline__ void bstore(__amdgpu_buffer_rsrc_t r, uint32_t voff, int soff, float v, int q, const Codec &c)
+    {
+        __builtin_amdgcn_raw_buffer_store_b16((short)f16c_encode((v - c.off[q]) * c.iw[q]), r, (int)voff, soff, 0);
     }
 };
 

@@ -8,7 +8,7 @@ FACE_LOW, FACE_MID, FACE_HIGH, FACE_ANY = 1, 2, 4, 7
 SET_VELOCITY, SET_ROTATION, SET_FORCE, SET_LIMITER, SET_NU_MULTIPLIER, SET_USE_IBB = 1, 2, 4, 8, 16, 32
 INIT_UNIFORM, INIT_TAYLOR_GREEN = 0, 1
 REFINE_SET, REFINE_ADD, REFINE_REMOVE = 0, 1, 2
-STORE_F32, STORE_F16, STORE_BF16, STORE_F64 = 0, 1, 2, 3
+STORE_F32, STORE_F16, STORE_BF16, STORE_F64, STORE_F16C = 0, 1, 2, 3, 4
 
 
 class Bounds(C.Structure):

@@ -154,7 +154,7 @@ def validate_level(arrays, info, counts=None):
     check(rc)
 
 
-STORE = {"f32": T.STORE_F32, "f16": T.STORE_F16, "bf16": T.STORE_BF16, "f64": T.STORE_F64}
+STORE = {"f32": T.STORE_F32, "f16": T.STORE_F16, "bf16": T.STORE_BF16, "f64": T.STORE_F64, "f16c": T.STORE_F16C}
 _UINT_ARRAYS = ("wallData", "linkLength", "openMeta")
 
 
@@ -239,7 +239,7 @@ class Sim:
 
     def populations(self, level=0):
         n = self.info(level).cellCount
-        dt = {"f32": np.float32, "f64": np.float64, "f16": np.float16, "bf16": np.uint16}[self.storage]
+        dt = {"f32": np.float32, "f64": np.float64, "f16": np.float16, "bf16": np.uint16, "f16c": np.uint16}[self.storage]
         out = np.zeros((27, n), dt)
         check(load().lbm3_download_populations(self.h, level, out.ctypes.data, out.nbytes))
         return out

@@ -30,7 +30,7 @@ from . import cases
 from . import lib
 
 KINDS = ("tgv", "channel", "sphere")
-PRECISIONS = ("f32", "f16", "bf16", "f64")
+PRECISIONS = ("f32", "f16", "bf16", "f64", "f16c")
 MAX_CELLS = 1 << 30           # one request's grid, cells (fits one device with room to spare)
 
 # kind -> {field: (type, required, low, high)}; low / high inclusive, None: open

@@ -264,11 +264,12 @@ def test_multilevel_f32(name):
 
 
 # ---------------------------------------------------------------- 16-bit storage accuracy
-@pytest.mark.parametrize("storage,bar", [("f16", 1e-3), ("bf16", 1e-2)])
+@pytest.mark.parametrize("storage,bar", [("f16", 1e-3), ("bf16", 1e-2), ("f16c", 1e-4)])
 def test_16bit_storage_velocity_error_vs_fp64(storage, bar):
     """Shifted 16-bit populations (deviation from the free-stream equilibrium,
     in units of the weight): relative L2 velocity error against the FP64 run of
-    the same kernels after 1000 steps of the sphere case."""
+    the same kernels after 1000 steps of the sphere case.  f16c (11 mantissa
+    bits) holds half of FP16's error."""
     case, dom, tris = _small_sphere()
     runs = {}
     for st in (storage, "f64"):
@@ -286,13 +287,14 @@ def test_16bit_storage_velocity_error_vs_fp64(storage, bar):
     assert err < bar, err
 
 
-def test_16bit_multilevel_velocity_error_vs_fp64():
-    """FP16 shifted storage on the three-level sphere: relative L2 velocity
-    error of every level against the FP64 run of the same kernels after 400
-    coarse steps (1600 finest sub-steps)."""
+@pytest.mark.parametrize("storage", ["f16", "f16c"])
+def test_16bit_multilevel_velocity_error_vs_fp64(storage):
+    """FP16 (and f16c) shifted storage on the three-level sphere: relative L2
+    velocity error of every level against the FP64 run of the same kernels
+    after 400 coarse steps (1600 finest sub-steps)."""
     case, dom, tris = _small_sphere3()
     runs = {}
-    for st in ("f16", "f64"):
+    for st in (storage, "f64"):
         g = lib.Sim(case, st)
         g.build(dom, tris)
         g.init()
@@ -301,7 +303,45 @@ def test_16bit_multilevel_velocity_error_vs_fp64():
         runs[st] = [(g.macros(L)[:, 1:].astype(np.float64), g.array("wallMap", L) != -3) for L in range(3)]
         g.close()
     errs = []
-    for (u16, fluid), (u64, _) in zip(runs["f16"], runs["f64"]):
+    for (u16, fluid), (u64, _) in zip(runs[storage], runs["f64"]):
         errs.append(float(np.linalg.norm((u16 - u64)[fluid]) / np.linalg.norm(u64[fluid])))
-    print("FP16 three-level sphere, 400 coarse steps: rel L2 u vs FP64 per level %s" % ["%.2e" % e for e in errs])
-    assert max(errs) < 1e-3, errs
+    print("%s three-level sphere, 400 coarse steps: rel L2 u vs FP64 per level %s" % (storage, ["%.2e" % e for e in errs]))
+    assert max(errs) < (1e-3 if storage == "f16" else 5e-4), errs
+
+
+def _f16c_value(bits):
+    """The f16c format by its definition: sign bit 15, exponent bits 11-14
+    (bias 15; 0: subnormal, 2^-14 * m / 2^11), mantissa bits 0-10."""
+    b = bits.astype(np.int64)
+    sign = np.where(b & 0x8000, -1.0, 1.0)
+    e = (b >> 11) & 0xF
+    m = (b & 0x7FF).astype(np.float64)
+    mag = np.where(e == 0, np.ldexp(m / 2048.0, -14), np.ldexp(1.0 + m / 2048.0, e - 15))
+    return sign * mag
+
+
+def test_f16c_format_against_its_definition():
+    """The initial Taylor-Green populations (reference velocity 0: the stored
+    value is (f - w) / w) in f16c decode, by the format's definition, to the
+    FP32 run's values rounded to nearest at 11 mantissa bits: within half a
+    unit in the last place, and about half of them not exact (the rounding is
+    live); every code lies in the format's range."""
+    case, dom, tris = cases.taylor_green(24)
+    pops = {}
+    for st in ("f32", "f16c"):
+        g = lib.Sim(case, st)
+        g.build(dom, tris)
+        g.init()
+        pops[st] = g.populations(0)
+        g.close()
+    w = np.array([8 / 27] + [2 / 27] * 6 + [1 / 54] * 12 + [1 / 216] * 8)[:, None]
+    v32 = pops["f32"].astype(np.float64) / w
+    v = _f16c_value(pops["f16c"])
+    e = np.floor(np.log2(np.maximum(np.abs(v32), 2.0 ** -14)))
+    half_ulp = np.ldexp(1.0, (e - 12).astype(np.int64))
+    # (the library divides in FP32 before it rounds: allow that rounding too)
+    assert np.all(np.abs(v - v32) <= half_ulp * (1 + 2.0 ** -10))
+    inexact = np.mean(v != v32)
+    print("f16c initial populations: %d values, %.2f not exact, max |error| / half ulp %.3f"
+          % (v.size, inexact, np.max(np.abs(v - v32) / half_ulp)))
+    assert 0.2 < inexact and np.max(np.abs(v)) < 2.0

@@ -95,10 +95,10 @@ def test_served_taylor_green_matches_oracle():
 
 
 def test_mix_is_bitwise_identical_to_alone():
-    """Requests of every kind and two storage types, submitted while others
+    """Requests of every kind and four storage types, submitted while others
     run: each result equals the same request solved alone, bit for bit."""
     plan = [("sphere", "f32", 70, 0), ("tgv", "f32", 50, 0), ("channel", "f16", 40, 5), ("tgv", "f16", 30, 12),
-            ("channel", "f32", 45, 12), ("sphere", "f16", 25, 20), ("tgv", "f64", 20, 33)]
+            ("channel", "f32", 45, 12), ("sphere", "f16", 25, 20), ("tgv", "f64", 20, 33), ("sphere", "f16c", 30, 8)]
     alone = {}
     for k, (name, st, steps, _) in enumerate(plan):
         alone[k] = _alone(CASES[name], steps, st)[0]

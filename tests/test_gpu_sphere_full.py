@@ -11,7 +11,8 @@
 * FP16 shifted storage, 5000 steps: relative L2 velocity error against the
   FP64 run of the same kernels.  The target is 1e-3; measured 1.15e-3 on the
   MI355X, so the target is missed by 15% at this length: the assertion guards
-  the measured level (1.5e-3) and the drag coefficient to 1%."""
+  the measured level (1.5e-3) and the drag coefficient to 1%.  The f16c
+  storage (11 mantissa bits, 4 exponent bits) is held to the target."""
 import numpy as np
 import pytest
 
@@ -72,10 +73,13 @@ def test_full_sphere_drag_coefficient(reynolds, lo, hi):
     assert lo < cd < hi
 
 
-def test_full_sphere_fp16_velocity_error():
+@pytest.mark.parametrize("storage,bar", [("f16", 1.5e-3), ("f16c", 1e-3)])
+def test_full_sphere_16bit_velocity_error(storage, bar):
+    """FP16 misses the 1e-3 target (1.15e-3); f16c, one mantissa bit more over
+    the range the shifted encoding uses, meets it."""
     u64, fluid, cd64 = _run("f64", 5000)
-    u16, _, cd16 = _run("f16", 5000)
+    u16, _, cd16 = _run(storage, 5000)
     err = float(np.linalg.norm((u16 - u64)[fluid]) / np.linalg.norm(u64[fluid]))
-    print("FP16 vs FP64 after 5000 steps: rel L2 u %.2e, C_D %.5f vs %.5f" % (err, cd16, cd64))
-    assert err < 1.5e-3
+    print("%s vs FP64 after 5000 steps: rel L2 u %.2e, C_D %.5f vs %.5f" % (storage, err, cd16, cd64))
+    assert err < bar
     assert abs(cd16 / cd64 - 1.0) < 1e-2
