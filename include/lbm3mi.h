@@ -125,30 +125,34 @@ int lbm3_synchronize(lbm3_sim *s);
 /* Autotuner of the step's launch choices (no counterpart in the reference):
  * times `steps` whole coarse steps (after one untimed step) for every
  * combination of the bulk kernel's occupancy cap (dynamic LDS per block 0,
- * 24, 40 or 64 KiB), its block order (round-robin over the XCDs or one
- * contiguous part per XCD), the side kernels' placement (in order after the
+ * 24, 40 or 64 KiB), its block order (bit 0: round-robin over the XCDs or one
+ * contiguous part per XCD; bit 1: the tiles in reverse order every other
+ * step, so that a step starts on the lines the previous one left in the
+ * last-level cache), the side kernels' placement (in order after the
  * bulk kernel: 0, on a second stream beside it: 1, and with several levels
  * also each finer level's sub-cycle on a stream of its own beside its coarser
  * level's step: 2) and, where a level completes its open boundary when it next
  * loads the open cells, the width of the aligned cell groups the open-cell
  * kernel takes (8 or 16 cells; 1: no groups, the bulk kernel completes and
  * steps the open cells itself) and what becomes of the cells on the x faces
- * (as the others; stepped by the bulk kernel and completed by a separate pass
- * after streaming; in groups while the bulk kernel takes the other open
- * cells), and, where a level has rotors, whether the bulk kernel steps the
- * rotor cells too instead of a kernel of their own; keeps the fastest and
- * restores the
- * simulation's state as it was (the populations and every step counter; a
+ * (0: as the others; 1: stepped by the bulk kernel and completed by a
+ * separate pass after streaming; 2: in groups while the bulk kernel takes the
+ * other open cells; 3: in the bulk kernel with a block's lanes rotated by 32
+ * cells, so that a row's two x-face cells share a wave), and, where a level
+ * has rotors, whether the bulk kernel steps the rotor cells too instead of a
+ * kernel of their own; keeps the fastest and restores the simulation's state
+ * as it was (the populations and every step counter; a
  * pending open-boundary condition stays pending unless the chosen open-cell
  * layout differs, which completes it first).  out[7 r + {LDS bytes,
- * XCD order, side streams, group cells, x-face choice, rotor cells in the bulk
- * kernel, ms per step}] for the first `cap` candidates; *nOut the candidates
- * timed. */
+ * block order, side streams, group cells, x-face choice, rotor cells in the
+ * bulk kernel, ms per step}] for the first `cap` candidates; *nOut the
+ * candidates timed. */
 int lbm3_autotune(lbm3_sim *s, int steps, double *out, int cap, int *nOut);
-/* The launch choices in use: {bulk LDS bytes, XCD order, side streams, open
+/* The launch choices in use: {bulk LDS bytes, block order, side streams, open
  * group cells, x-face choice, rotor cells in the bulk kernel}; lbm3_set_tune
- * sets them (LDS 0..65536, side streams 0..2, group 1, 8 or 16, x-face choice
- * 0..2, rotor cells 0..1; a new layout of the open or rotor cells completes a
+ * sets them (LDS 0..65536, block order 0..3, side streams 0..2, group 1, 8 or
+ * 16, x-face choice 0..3, rotor cells 0..1; a new layout of the open or rotor
+ * cells completes a
  * pending open-boundary condition first).  Every choice gives the
  * same results bit for bit, except with 16-bit storage: that completion, and
  * every step's separate pass for the x faces, store the completed populations

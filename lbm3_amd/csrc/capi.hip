@@ -201,6 +201,7 @@ static int prepare_open_fusion(Sim &s, int L)
     l.openFusionChecked = true;
     l.fusedOpen = false;
     l.openInBulk = false;
+    l.laneRot = false;
     l.rotorInBulk = false;
     l.openSettled = true;
     l.rotorCellCount = 0;
@@ -276,6 +277,7 @@ static int prepare_open_fusion(Sim &s, int L)
     int G = s.tune.openGroup ? s.tune.openGroup : open_group_cells((int)storage_bytes(s.storage));
     if (G == 1 && l.rotorInBulk) G = open_group_cells((int)storage_bytes(s.storage));
     l.openInBulk = fuse && G == 1;   // the bulk kernel completes and steps the open cells
+    l.laneRot = l.openInBulk && s.tune.openSplit == 3;
     // the bulk kernel's open cells but those on the x faces, which stay in
     // groups of the open-cell kernel (one x-face cell per row end would hold a
     // bulk wave for one lane; eight in a group share one wave's condition code)
@@ -1076,6 +1078,7 @@ int lbm3_autotune(lbm3_sim *h, int steps, double *out, int cap, int *nOut)
         open.push_back({storageGroup == 8 ? 16 : 8, 0});
         open.push_back({1, 0});                     // the open cells in the bulk kernel
         open.push_back({1, 2});                     // ... but those on the x faces, in groups
+        open.push_back({1, 3});                     // ... a row's two x-face cells in one wave
         open.push_back({storageGroup, 1});
         open.push_back({storageGroup == 8 ? 16 : 8, 1});
     }
@@ -1099,7 +1102,7 @@ int lbm3_autotune(lbm3_sim *h, int steps, double *out, int cap, int *nOut)
         s.tune.rotorInBulk = gs.rotor;
         if ((rc = regroup(s))) break;
         for (int lds : ldsChoices)
-            for (int remap = 0; remap < 2 && !rc; ++remap)
+            for (int remap = 0; remap < 4 && !rc; ++remap)
                 for (int overlap = s.levels > 1 ? 2 : 1; overlap >= 0 && !rc; --overlap) {
                     s.tune.bulkLds = lds;
                     s.tune.xcdRemap = remap;
@@ -1151,15 +1154,16 @@ int lbm3_get_tune(lbm3_sim *h, int *out6)
 int lbm3_set_tune(lbm3_sim *h, const int *in6)
 {
     if (!h || !in6) return set_error(LBM3_E_ARG, "null argument");
-    if (in6[0] < 0 || in6[0] > 65536 || in6[2] < 0 || in6[2] > 2 || (in6[3] != 1 && in6[3] != 8 && in6[3] != 16) ||
-        in6[4] < 0 || in6[4] > 2 || in6[5] < 0 || in6[5] > 1)
-        return set_error(LBM3_E_ARG, "bulk LDS in 0..65536 bytes, side streams 0..2, open groups of 1 (in the bulk "
-                                     "kernel), 8 or 16 cells, x-face choice 0..2, rotor cells 0..1");
+    if (in6[0] < 0 || in6[0] > 65536 || in6[1] < 0 || in6[1] > 3 || in6[2] < 0 || in6[2] > 2 ||
+        (in6[3] != 1 && in6[3] != 8 && in6[3] != 16) ||
+        in6[4] < 0 || in6[4] > 3 || in6[5] < 0 || in6[5] > 1)
+        return set_error(LBM3_E_ARG, "bulk LDS in 0..65536 bytes, block order 0..3, side streams 0..2, open groups of 1 (in the bulk "
+                                     "kernel), 8 or 16 cells, x-face choice 0..3, rotor cells 0..1");
     Sim &s = h->s;
     if (s.timing) return set_error(LBM3_E_STATE, "launch choices changed during a timed run");
     const Sim::Tune was = s.tune;
     s.tune.bulkLds = in6[0];
-    s.tune.xcdRemap = in6[1] != 0;
+    s.tune.xcdRemap = in6[1];
     s.tune.overlap = in6[2];
     s.tune.openGroup = in6[3];
     s.tune.openSplit = in6[4];

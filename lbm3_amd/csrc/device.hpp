@@ -181,8 +181,9 @@ template <class S> struct StepArgs {
     int rotorCellCount;
     double rotorCS[2 * LBM3_MAX_ROTORS];   // cos, sin of each rotor's angle this step, in the compute type
     const StepParam *param;       // this step's values in device memory (graph replay) or null
-    int xcdRemap;                 // bulk blocks dealt so that each XCD sweeps one contiguous part (else round-robin)
+    int xcdRemap;                 // bulk block order (xcd_block)
     int bulkLds;                  // dynamic LDS bytes per bulk block (host side: caps resident bulk blocks per CU)
+    int laneRot;                  // host side: the bulk kernel with open cells inside takes its block's lanes rotated
 };
 
 // Open boundary cells of one level, all BCIDs concatenated (BCID order, then cell order)
@@ -223,12 +224,19 @@ template <class S> struct OpenArgs {
 #define LBM3_BULK_LDS 40960
 #endif
 
-__device__ __forceinline__ int xcd_block(int b, int n, int remap)
+// The tile a bulk block takes.  remap bit 0: blocks dealt so that each XCD
+// sweeps one contiguous part (else round-robin over the XCDs); bit 1: on odd
+// esotwist parity the tiles in reverse order, so that a step starts where the
+// previous one ended, on the lines the last-level cache still holds
+__device__ __forceinline__ int xcd_block(int b, int n, int remap, bool flip)
 {
-    if (!remap) return b;
-    const int per = n / 8;
-    if (b >= per * 8) return b;
-    return (b % 8) * per + b / 8;
+    int t = b;
+    if (remap & 1) {
+        const int per = n / 8;
+        if (b < per * 8) t = (b % 8) * per + b / 8;
+    }
+    if ((remap & 2) && flip) t = n - 1 - t;
+    return t;
 }
 
 } // namespace lbm3

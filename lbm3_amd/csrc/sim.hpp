@@ -88,6 +88,7 @@ struct Level {
     // cells itself (openBits marks them)
     bool openInBulk = false;
     DBuf<uint64_t> openBits;
+    bool laneRot = false;            // Sim::Tune::openSplit 3: the bulk kernel's lanes rotated by 32 cells
     // Sim::Tune::rotorInBulk: the bulk kernel steps the rotor cells (rotorBits)
     bool rotorInBulk = false;
     DBuf<uint64_t> rotorBits;
@@ -154,7 +155,9 @@ struct Sim {
         int openGroup = 0;            // cells per open-boundary group (0: open_group_cells of the storage)
         // cells on the x faces: 0 as the others; 1 stepped by the bulk kernel
         // and completed by a pass after streaming; 2 (with openGroup 1) in
-        // groups of the open-cell kernel while the bulk kernel takes the others
+        // groups of the open-cell kernel while the bulk kernel takes the others;
+        // 3 (with openGroup 1) in the bulk kernel with its lanes rotated so that
+        // a row's two x-face cells share a wave
         int openSplit = 0;
         int rotorInBulk = 0;          // the rotor cells stepped by the bulk kernel instead of their own
     } tune;

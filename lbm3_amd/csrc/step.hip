@@ -327,13 +327,23 @@ __device__ __forceinline__ void bulk_collide(const StepArgs<S> &a, int cell, int
 // OPEN: the level's open cells are in the bulk kernel too (FastTopo::openBits):
 // such a cell first completes the populations the previous step left unknown,
 // as k_open_groups does, then collides and streams like its neighbours
-template <class S, bool OPEN>
+// the bit of `cell` in a mask of one bit per cell, 64 to a word: a wave of
+// consecutive 64-aligned cells reads one word (a scalar load); with its
+// block's lanes rotated (ROT) a wave may span two words, and each lane reads
+// its own
+template <bool ROT>
+__device__ __forceinline__ bool mask_bit(const uint64_t *m, int cell)
+{
+    const uint64_t w = ROT ? m[cell >> 6] : m[__builtin_amdgcn_readfirstlane(cell >> 6)];
+    return (w >> (cell & 63)) & 1ull;
+}
+
+template <class S, bool OPEN, bool ROT = false>
 __device__ __forceinline__ void open_in_bulk(const StepArgs<S> &a, int cell, typename StoreTraits<S>::compute (&f)[27])
 {
     if constexpr (OPEN) {
         using T = typename StoreTraits<S>::compute;
-        const uint64_t m = a.fast.openBits[__builtin_amdgcn_readfirstlane(cell >> 6)];
-        if (!((m >> (cell & 63)) & 1ull)) return;
+        if (!mask_bit<ROT>(a.fast.openBits, cell)) return;
         if (!(a.param ? a.param->pending : a.open.pending)) return;
         const int x = -(a.fast.cellRow[cell].x + 8);
         open_dispatch<T>(f, x, a.open.meta[x], a.open.records, a.open.dRhoRef, a.open.uNormalRef, a.open.dRhoCum,
@@ -341,7 +351,7 @@ __device__ __forceinline__ void open_in_bulk(const StepArgs<S> &a, int cell, typ
     }
 }
 
-template <class S, bool FLIP, int BULK, class Off, bool OPEN = false, bool ROTOR = false>
+template <class S, bool FLIP, int BULK, class Off, bool OPEN = false, bool ROTOR = false, bool ROT = false>
 __device__ __forceinline__ void bulk_cell(const StepArgs<S> &a, int cell)
 {
     using T = typename StoreTraits<S>::compute;
@@ -349,8 +359,7 @@ __device__ __forceinline__ void bulk_cell(const StepArgs<S> &a, int cell)
     if (a.fast.skip) {
         // a wave's 64 cells are 64-aligned and a block's 256: one mask word per
         // wave and one row per block, both scalar loads
-        const uint64_t m = a.fast.skip[__builtin_amdgcn_readfirstlane(cell >> 6)];
-        if ((m >> (cell & 63)) & 1ull) return;   // handled by another kernel, or solid
+        if (mask_bit<ROT>(a.fast.skip, cell)) return;   // handled by another kernel, or solid
         int r = a.fast.tileRow[__builtin_amdgcn_readfirstlane(cell >> 8)];
         while (r + 1 < a.fast.nRows && a.fast.rows[r + 1].x <= cell) ++r;
         row = r;
@@ -375,7 +384,7 @@ __device__ __forceinline__ void bulk_cell(const StepArgs<S> &a, int cell)
 #pragma unroll
         for (int q = 0; q < 27; ++q)
             f[q] = StoreTraits<S>::bload(r, off[pre_nb(q)], pre_plane(q, FLIP) * LBM3_TILE * int(sizeof(S)), q, a.cd);
-        open_in_bulk<S, OPEN>(a, cell, f);
+        open_in_bulk<S, OPEN, ROT>(a, cell, f);
         bulk_collide<S, BULK, ROTOR>(a, cell, row, f);
         if constexpr (sizeof(S) >= 4 || LBM3_BUF16 == 1) {
 #pragma unroll
@@ -390,7 +399,7 @@ __device__ __forceinline__ void bulk_cell(const StepArgs<S> &a, int cell)
 #pragma unroll
         for (int q = 0; q < 27; ++q)
             f[q] = StoreTraits<S>::load(at(a.f + pre_plane(q, FLIP) * LBM3_TILE, off[pre_nb(q)]), q, a.cd);
-        open_in_bulk<S, OPEN>(a, cell, f);
+        open_in_bulk<S, OPEN, ROT>(a, cell, f);
         bulk_collide<S, BULK, ROTOR>(a, cell, row, f);
 #pragma unroll
         for (int q = 0; q < 27; ++q)
@@ -400,14 +409,19 @@ __device__ __forceinline__ void bulk_cell(const StepArgs<S> &a, int cell)
 
 // WIDE: the level's population array exceeds 4 GiB (64-bit lane offsets);
 // OPEN: the open cells are stepped here too; ROTOR: the rotor cells (neither
-// with WIDE)
-template <class S, bool FLIP, int BULK, bool WIDE, bool OPEN = false, bool ROTOR = false>
+// with WIDE).  ROT: lane t of a block takes cell (t + 32) mod 256 of its tile,
+// so that where rows are whole tiles the last 32 and the first 32 cells of a
+// row -- both x-face cells -- share one wave, which then runs the open
+// condition once for the two instead of two waves once each; every access of
+// that wave still fills whole 128-byte lines
+template <class S, bool FLIP, int BULK, bool WIDE, bool OPEN = false, bool ROTOR = false, bool ROT = false>
 __global__ __launch_bounds__(256) void k_collide_stream(StepArgs<S> a)
 {
-    const int cell = xcd_block(blockIdx.x, gridDim.x, a.xcdRemap) * 256 + threadIdx.x;
+    const int lane = ROT ? int((threadIdx.x + 32) & 255) : int(threadIdx.x);
+    const int cell = xcd_block(blockIdx.x, gridDim.x, a.xcdRemap, FLIP) * 256 + lane;
     if (cell >= a.fast.N) return;
     if (WIDE) bulk_cell<S, FLIP, BULK, uint64_t>(a, cell);
-    else bulk_cell<S, FLIP, BULK, uint32_t, OPEN, ROTOR>(a, cell);
+    else bulk_cell<S, FLIP, BULK, uint32_t, OPEN, ROTOR, ROT>(a, cell);
 }
 
 // Open-boundary cells (never wall-adjacent when this kernel is used): load,
@@ -1110,10 +1124,14 @@ void launch_collide_stream(const StepArgs<S> &a, bool flip, hipStream_t s)
     const bool wide = wide_level<S>(a.fast.N);
 #define LBM3_BULK_LAUNCH(F, K, W) hipLaunchKernelGGL((k_collide_stream<S, F, K, W>), g, b, lds, s, a)
 #define LBM3_BULK_OPEN(F, K) hipLaunchKernelGGL((k_collide_stream<S, F, K, false, true>), g, b, lds, s, a)
+#define LBM3_BULK_OPEN_ROT(F, K) hipLaunchKernelGGL((k_collide_stream<S, F, K, false, true, false, true>), g, b, lds, s, a)
 #define LBM3_BULK_ROTOR(F, K) hipLaunchKernelGGL((k_collide_stream<S, F, K, false, false, true>), g, b, lds, s, a)
     if (a.fast.rotorBits && !wide) {
         if (flip) { if (k17) LBM3_BULK_ROTOR(true, 1); else LBM3_BULK_ROTOR(true, 2); }
         else { if (k17) LBM3_BULK_ROTOR(false, 1); else LBM3_BULK_ROTOR(false, 2); }
+    } else if (a.fast.openBits && !wide && a.laneRot) {
+        if (flip) { if (k17) LBM3_BULK_OPEN_ROT(true, 1); else LBM3_BULK_OPEN_ROT(true, 2); }
+        else { if (k17) LBM3_BULK_OPEN_ROT(false, 1); else LBM3_BULK_OPEN_ROT(false, 2); }
     } else if (a.fast.openBits && !wide) {
         if (flip) { if (k17) LBM3_BULK_OPEN(true, 1); else LBM3_BULK_OPEN(true, 2); }
         else { if (k17) LBM3_BULK_OPEN(false, 1); else LBM3_BULK_OPEN(false, 2); }
@@ -1126,6 +1144,7 @@ void launch_collide_stream(const StepArgs<S> &a, bool flip, hipStream_t s)
     }
 #undef LBM3_BULK_LAUNCH
 #undef LBM3_BULK_OPEN
+#undef LBM3_BULK_OPEN_ROT
 #undef LBM3_BULK_ROTOR
 }
 

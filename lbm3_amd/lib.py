@@ -304,10 +304,10 @@ class Sim:
         keep the fastest, leave the state as it was.  Returns (chosen
         {bulk_lds, xcd_remap, overlap, open_group, open_split, rotor_in_bulk},
         [(lds, remap, overlap, group, split, rotor, ms per step), ...])."""
-        out = np.zeros((256, 7))
+        out = np.zeros((1024, 7))
         n = C.c_int()
-        check(load().lbm3_autotune(self.h, steps, out.ctypes.data, 256, C.byref(n)))
-        return self.tune(), [tuple(r) for r in out[:min(n.value, 256)]]
+        check(load().lbm3_autotune(self.h, steps, out.ctypes.data, 1024, C.byref(n)))
+        return self.tune(), [tuple(r) for r in out[:min(n.value, 1024)]]
 
     TUNE = ("bulk_lds", "xcd_remap", "overlap", "open_group", "open_split", "rotor_in_bulk")
 
