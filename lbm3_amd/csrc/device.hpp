@@ -204,14 +204,17 @@ template <class S> struct OpenArgs {
     const int *list;              // the entries to complete ([count]), or null: entries 0..count-1
 };
 
-// Blocks are dispatched round-robin over the 8 XCDs.  With LBM3_XCD_REMAP=1
-// each XCD gets a contiguous range of blocks (its own L2 then sees whole
+// Blocks are dispatched round-robin over the 8 XCDs.  With bit 0 of the block
+// order each XCD gets a contiguous range of blocks (its own L2 then sees whole
 // neighbouring rows); with the tiled population layout the plain order, in
 // which the whole chip works on one narrow window of tiles at a time, measured
-// 4% faster on the 256x128x128 sphere bulk kernel (162 vs 169 us), so it is
-// the default.
+// 4% faster on the 256x128x128 sphere bulk kernel (162 vs 169 us).  Bit 1
+// sweeps the tiles backwards on odd esotwist parity, so that each step starts
+// on the lines the previous step wrote last, still in the 256 MB last-level
+// cache: the FP32 sphere bulk kernel (452 MB of populations) 186 -> 158 us.
+// Default: plain order, reversed every other step.
 #ifndef LBM3_XCD_REMAP
-#define LBM3_XCD_REMAP 0
+#define LBM3_XCD_REMAP 2
 #endif
 
 // Dynamic LDS per bulk block, unused by the kernel: a cap on resident bulk
