@@ -446,11 +446,16 @@ template <class S> static void step_level(Sim &s, int level)
     }
     const StepParam *slot = s.captureTable ? s.captureTable + s.captureSlot++ : nullptr;
     a.param = slot;
-    if (timed || !s.tune.overlap || (l.W == 0 && a.open.count == 0 && a.rotorCellCount == 0)) {
+    // side-stream choice 3: the wall-adjacent cells in blocks ahead of the bulk
+    // blocks of one launch (not with rotor cells in the bulk kernel, rotated
+    // lanes or 64-bit offsets)
+    const bool walls = s.tune.overlap == 3 && l.W > 0 && !a.fast.rotorBits && !a.laneRot && !wide_level<S>(l.N);
+    a.wallBlocks = walls ? (int)((l.W + 255) / 256) : 0;
+    if (timed || !s.tune.overlap || ((l.W == 0 || walls) && a.open.count == 0 && a.rotorCellCount == 0)) {
         // in order on one stream, so that each kernel's events bracket it alone
         launch_collide_stream<S>(a, flip, s.stream);
         if (timed) (void)hipEventRecord(s.ev[s.evUsed][1], s.stream);
-        launch_collide_wall<S>(a, flip, s.stream);
+        if (!walls) launch_collide_wall<S>(a, flip, s.stream);
         launch_open_cells<S>(a, flip, s.stream);
         launch_rotor_cells<S>(a, flip, s.stream);
         if (timed) {
@@ -469,7 +474,7 @@ template <class S> static void step_level(Sim &s, int level)
             launch_rotor_cells<S>(a, flip, s.aux2);
             (void)hipEventRecord(s.join2, s.aux2);
         }
-        launch_collide_wall<S>(a, flip, s.aux);
+        if (!walls) launch_collide_wall<S>(a, flip, s.aux);
         launch_open_cells<S>(a, flip, s.aux);
         if (!third) launch_rotor_cells<S>(a, flip, s.aux);
         (void)hipEventRecord(s.join, s.aux);
@@ -1103,7 +1108,8 @@ int lbm3_autotune(lbm3_sim *h, int steps, double *out, int cap, int *nOut)
         if ((rc = regroup(s))) break;
         for (int lds : ldsChoices)
             for (int remap = 0; remap < 4 && !rc; ++remap)
-                for (int overlap = s.levels > 1 ? 2 : 1; overlap >= 0 && !rc; --overlap) {
+                for (int overlap = 3; overlap >= 0 && !rc; --overlap) {
+                    if (overlap == 2 && s.levels == 1) continue;   // no finer levels to run beside
                     s.tune.bulkLds = lds;
                     s.tune.xcdRemap = remap;
                     s.tune.overlap = overlap;
@@ -1154,10 +1160,10 @@ int lbm3_get_tune(lbm3_sim *h, int *out6)
 int lbm3_set_tune(lbm3_sim *h, const int *in6)
 {
     if (!h || !in6) return set_error(LBM3_E_ARG, "null argument");
-    if (in6[0] < 0 || in6[0] > 65536 || in6[1] < 0 || in6[1] > 3 || in6[2] < 0 || in6[2] > 2 ||
+    if (in6[0] < 0 || in6[0] > 65536 || in6[1] < 0 || in6[1] > 3 || in6[2] < 0 || in6[2] > 3 ||
         (in6[3] != 1 && in6[3] != 8 && in6[3] != 16) ||
         in6[4] < 0 || in6[4] > 3 || in6[5] < 0 || in6[5] > 1)
-        return set_error(LBM3_E_ARG, "bulk LDS in 0..65536 bytes, block order 0..3, side streams 0..2, open groups of 1 (in the bulk "
+        return set_error(LBM3_E_ARG, "bulk LDS in 0..65536 bytes, block order 0..3, side streams 0..3, open groups of 1 (in the bulk "
                                      "kernel), 8 or 16 cells, x-face choice 0..3, rotor cells 0..1");
     Sim &s = h->s;
     if (s.timing) return set_error(LBM3_E_STATE, "launch choices changed during a timed run");

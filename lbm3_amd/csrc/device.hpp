@@ -184,6 +184,7 @@ template <class S> struct StepArgs {
     int xcdRemap;                 // bulk block order (xcd_block)
     int bulkLds;                  // dynamic LDS bytes per bulk block (host side: caps resident bulk blocks per CU)
     int laneRot;                  // host side: the bulk kernel with open cells inside takes its block's lanes rotated
+    int wallBlocks;               // blocks of wall-adjacent cells ahead of the bulk blocks in one launch (0: none)
 };
 
 // Open boundary cells of one level, all BCIDs concatenated (BCID order, then cell order)

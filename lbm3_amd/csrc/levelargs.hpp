@@ -30,6 +30,7 @@ template <class S> inline StepArgs<S> step_args(Sim &s, Level &l)
     a.xcdRemap = s.tune.xcdRemap;
     a.bulkLds = s.tune.bulkLds;
     a.laneRot = l.openInBulk && l.laneRot ? 1 : 0;
+    a.wallBlocks = 0;
     a.geom = l.geom;
     a.cd = s.codec;
     a.open = FusedOpen<typename StoreTraits<S>::compute>{};

@@ -150,7 +150,9 @@ struct Sim {
         int xcdRemap = LBM3_XCD_REMAP;
         // 1: wall / open / rotor kernels on the second stream beside the bulk
         // kernel; 2: also each finer level's sub-cycle on a stream of its own,
-        // beside its coarser level's step; 0: everything in order
+        // beside its coarser level's step; 3: the wall-adjacent cells in
+        // blocks ahead of the bulk blocks of one launch, the other side kernels
+        // on the second stream; 0: everything in order
         int overlap = 1;
         int openGroup = 0;            // cells per open-boundary group (0: open_group_cells of the storage)
         // cells on the x faces: 0 as the others; 1 stepped by the bulk kernel

@@ -649,6 +649,25 @@ __global__ __launch_bounds__(256) void k_collide_wall(StepArgs<S> a)
     wall_cell<S, FLIP>(a, wm);
 }
 
+// The bulk kernel with the wall-adjacent cells in blocks of their own ahead of
+// the bulk blocks, in one launch: no second stream and no join per step.  The
+// wall path needs about twice the bulk path's registers; the register budget is
+// held to six waves per SIMD (the bulk path fits it, the wall path spills), so
+// the bulk blocks keep their occupancy.
+template <class S, bool FLIP, int BULK, bool OPEN>
+__global__ __launch_bounds__(256, 6) void k_collide_stream_walls(StepArgs<S> a)
+{
+    const int b = blockIdx.x;
+    if (b < a.wallBlocks) {
+        const int wm = b * 256 + threadIdx.x;
+        if (wm < a.W) wall_cell<S, FLIP>(a, wm);
+        return;
+    }
+    const int cell = xcd_block(b - a.wallBlocks, gridDim.x - a.wallBlocks, a.xcdRemap, FLIP) * 256 + threadIdx.x;
+    if (cell >= a.fast.N) return;
+    bulk_cell<S, FLIP, BULK, uint32_t, OPEN>(a, cell);
+}
+
 // derive the hot-loop topology records from the compressed IJK/NBR arrays;
 // with rotorCand, the fluid cells the rotors can reach get -4, and with openOf
 // the open-boundary cells get -(8 + open-list index), in place of
@@ -1122,6 +1141,19 @@ void launch_collide_stream(const StepArgs<S> &a, bool flip, hipStream_t s)
     const unsigned lds = (unsigned)a.bulkLds;
     const bool k17 = relax_of<typename StoreTraits<S>::compute>(a.bulk).k17 != 0;
     const bool wide = wide_level<S>(a.fast.N);
+    if (a.wallBlocks > 0) {
+        // the caller combines only levels without rotor cells in the bulk
+        // kernel, rotated lanes or 64-bit offsets
+        const dim3 gw(blocks_for(a.fast.N) + (unsigned)a.wallBlocks);
+#define LBM3_BULK_WALLS(F, K, O) hipLaunchKernelGGL((k_collide_stream_walls<S, F, K, O>), gw, b, lds, s, a)
+        const bool o = a.fast.openBits != nullptr;
+        if (flip) { if (k17) { if (o) LBM3_BULK_WALLS(true, 1, true); else LBM3_BULK_WALLS(true, 1, false); }
+                    else { if (o) LBM3_BULK_WALLS(true, 2, true); else LBM3_BULK_WALLS(true, 2, false); } }
+        else { if (k17) { if (o) LBM3_BULK_WALLS(false, 1, true); else LBM3_BULK_WALLS(false, 1, false); }
+               else { if (o) LBM3_BULK_WALLS(false, 2, true); else LBM3_BULK_WALLS(false, 2, false); } }
+#undef LBM3_BULK_WALLS
+        return;
+    }
 #define LBM3_BULK_LAUNCH(F, K, W) hipLaunchKernelGGL((k_collide_stream<S, F, K, W>), g, b, lds, s, a)
 #define LBM3_BULK_OPEN(F, K) hipLaunchKernelGGL((k_collide_stream<S, F, K, false, true>), g, b, lds, s, a)
 #define LBM3_BULK_OPEN_ROT(F, K) hipLaunchKernelGGL((k_collide_stream<S, F, K, false, true, false, true>), g, b, lds, s, a)

@@ -21,7 +21,7 @@ def test_autotune_keeps_the_state(storage):
     s.run(7)
     chosen, table = s.autotune(steps=3)
     assert s.info(0).iterationsFinished == 7
-    assert len(table) in (32, 224)
+    assert len(table) in (48, 336)
     assert any(tuple(int(v) for v in r[:6]) == tuple(chosen[k] for k in lib.Sim.TUNE) for r in table)
     s.run(13)
     s.sync()
@@ -92,12 +92,13 @@ def test_open_cells_in_the_bulk_kernel_equal_the_open_cell_kernel(storage):
     the open-cell kernel; 3: with its lanes rotated so that a row's two x-face
     cells share a wave), with the same arithmetic as the open-cell kernel
     (the condition applied to the loaded populations, then the collision): the
-    same bits in every storage type, with the side stream on or off; the skip
-    and open masks count every fluid cell once."""
+    same bits in every storage type, with the side stream on or off or the
+    wall cells in blocks of the bulk kernel's launch (side-stream choice 3);
+    the skip and open masks count every fluid cell once."""
     case, dom, tris = _fused_sphere()
     case.trackOpenBoundaries = 1
     runs = []
-    for g, overlap, split in ((0, 1, 0), (1, 1, 0), (1, 0, 0), (1, 1, 2), (1, 0, 2), (1, 1, 3)):
+    for g, overlap, split in ((0, 1, 0), (1, 1, 0), (1, 0, 0), (1, 1, 2), (1, 0, 2), (1, 1, 3), (1, 3, 0), (0, 3, 0)):
         s = lib.Sim(case, storage)
         s.build(dom, tris)
         s.init()
@@ -105,7 +106,8 @@ def test_open_cells_in_the_bulk_kernel_equal_the_open_cell_kernel(storage):
             s.set_tune(40960, 0, overlap, g, split)
         s.run(30)
         s.sync()
-        runs.append([s.macros(0), s.array("openRefs", 0), np.array([list(s.open_boundary_stats(b).values()) for b in range(3)])])
+        runs.append([s.macros(0), s.array("openRefs", 0), np.array([list(s.open_boundary_stats(b).values()) for b in range(3)]),
+                     s.wall_forces(0)])
         k = s.cell_kinds(0)
         assert k["bulk"] + k["wall"] + k["open_group"] + k["solid"] + k["rotor"] + (k["open"] - k["open_in_bulk"]) == s.info(0).cellCount
         assert k["open_in_bulk"] == (0 if not g else (k["open"] if split in (0, 3) else k["open_in_bulk"])) and \
@@ -179,7 +181,7 @@ def test_finer_levels_on_their_own_streams_equal_in_order(storage):
     bit on the three-level sphere."""
     case, dom, tris = cases.multilevel_sphere(seed=0)
     runs = []
-    for overlap in (0, 1, 2):
+    for overlap in (0, 1, 2, 3):
         s = lib.Sim(case, storage)
         s.build(dom, tris)
         s.init()

@@ -43,7 +43,7 @@ def _with_overlap(s, overlap):
     s.set_tune(*(t[k] for k in lib.Sim.TUNE))
 
 
-@pytest.mark.parametrize("storage,overlap", [("f32", 1), ("f16", 1), ("f32", 2)])
+@pytest.mark.parametrize("storage,overlap", [("f32", 1), ("f16", 1), ("f32", 2), ("f32", 3)])
 def test_graph_replay_two_level_taylor_green(storage, overlap):
     """overlap 2: the fine level sub-cycles on a stream of its own, captured
     into the same graph through the fork and join events."""
@@ -78,10 +78,13 @@ def _state_open(s, levels, n_open):
     return out
 
 
-def test_graph_replay_sphere_open_cells():
-    """The sphere with its open boundaries completed in the open-cell kernel
-    (a deferred condition pending across replays) and tracked flow: replays
-    equal eager steps bit for bit, also around eager steps and reads."""
+@pytest.mark.parametrize("group,overlap", [(8, 1), (1, 3)])
+def test_graph_replay_sphere_open_cells(group, overlap):
+    """The sphere with its open boundaries completed when the cells are next
+    loaded (in the open-cell kernel, or in the bulk kernel with the wall cells
+    in the same launch; a deferred condition pending across replays) and
+    tracked flow: replays equal eager steps bit for bit, also around eager
+    steps and reads."""
     case, dom, tris = cases.sphere(48, 24, 24, diameter=6.0, center=(14.0, 12.0, 12.0), segments=(16, 32))
     case.trackOpenBoundaries = 1
 
@@ -89,6 +92,7 @@ def test_graph_replay_sphere_open_cells():
         s = lib.Sim(case, "f32")
         s.build(dom, tris)
         s.init()
+        s.set_tune(40960, 2, overlap, group, 0, 0)
         return s
 
     a, b = make(), make()
