@@ -1143,9 +1143,10 @@ void launch_collide_stream(const StepArgs<S> &a, bool flip, hipStream_t s)
     const bool wide = wide_level<S>(a.fast.N);
     if (a.wallBlocks > 0) {
         // the caller combines only levels without rotor cells in the bulk
-        // kernel, rotated lanes or 64-bit offsets
+        // kernel, rotated lanes or 64-bit offsets, and 16-bit levels only
+        // without open cells in the bulk kernel
         const dim3 gw(blocks_for(a.fast.N) + (unsigned)a.wallBlocks);
-#define LBM3_BULK_WALLS(F, K, O) hipLaunchKernelGGL((k_collide_stream_walls<S, F, K, O>), gw, b, lds, s, a)
+#define LBM3_BULK_WALLS(F, K, O) hipLaunchKernelGGL((k_collide_stream_walls<S, F, K, O && sizeof(S) != 2>), gw, b, lds, s, a)
         const bool o = a.fast.openBits != nullptr;
         if (flip) { if (k17) { if (o) LBM3_BULK_WALLS(true, 1, true); else LBM3_BULK_WALLS(true, 1, false); }
                     else { if (o) LBM3_BULK_WALLS(true, 2, true); else LBM3_BULK_WALLS(true, 2, false); } }
