@@ -319,9 +319,11 @@ __device__ __forceinline__ void bulk_collide(const StepArgs<S> &a, int cell, int
 // open cells, <= -8: open cell (each with its own kernel).
 // 16-bit storage in the bulk kernel: 0 global loads and stores, 1 raw buffer
 // loads and stores, 2 buffer loads and global stores.  Measured on the
-// 256x128x128 sphere (FP16 bulk kernel): 93.4 us, 97.6 us, 94.4 us.
+// 256x128x128 sphere (FP16 bulk kernel with the open cells inside, after the
+// stores' plain conversion): 103.2 us, 98.0 us, 99.8 us (with the fused
+// conversion the order was reversed: 93.4, 97.6, 94.4 us, open cells in groups).
 #ifndef LBM3_BUF16
-#define LBM3_BUF16 0
+#define LBM3_BUF16 1
 #endif
 
 // OPEN: the level's open cells are in the bulk kernel too (FastTopo::openBits):
