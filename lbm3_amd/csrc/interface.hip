@@ -92,6 +92,7 @@ template <class T> struct IfArgs {
     void *fc, *ff;                 // populations of the two levels (storage type)
     T w1c, w1f;                    // omega1 of the two levels
     int count;
+    int loCount;                   // leftovers of a combined C->F and leftover launch
     const int *index, *child;      // interface lists of the coarse level
     const int *loIndex, *loParent, *loI, *loJ, *loK;
     const int *wallMapC;
@@ -152,13 +153,13 @@ __global__ __launch_bounds__(256) void k_fine_to_coarse(IfArgs<typename StoreTra
 }
 
 // C->F: the eight fine cells inside a 2x2x2 coarse block, at +-1/4
-// (updateInterface.h:362-687)
+// (updateInterface.h:362-687); `blk` the block's index among the C->F blocks
 template <class S, bool FLIPC, bool FLIPF, bool LINEAR>
-__global__ __launch_bounds__(256) void k_coarse_to_fine(IfArgs<typename StoreTraits<S>::compute> a)
+__device__ __forceinline__ void c2f_block(const IfArgs<typename StoreTraits<S>::compute> &a, int blk)
 {
     using T = typename StoreTraits<S>::compute;
     __shared__ CellValues<T> lds[256];
-    const int gid = blockIdx.x * 256 + threadIdx.x;
+    const int gid = blk * 256 + threadIdx.x;
     const int item = gid >> 3, n = gid & 7;
     const bool active = item < a.count;
     if (active) {
@@ -192,13 +193,20 @@ __global__ __launch_bounds__(256) void k_coarse_to_fine(IfArgs<typename StoreTra
     for (int q = 0; q < 27; ++q) StoreTraits<S>::store(ff + pop_index(pre_plane(q, FLIPF), nb[pre_nb(q)]), f[q], q, a.cd);
 }
 
-// fine cells without a full coarse block (updateInterface.h:689-884)
+template <class S, bool FLIPC, bool FLIPF, bool LINEAR>
+__global__ __launch_bounds__(256) void k_coarse_to_fine(IfArgs<typename StoreTraits<S>::compute> a)
+{
+    c2f_block<S, FLIPC, FLIPF, LINEAR>(a, blockIdx.x);
+}
+
+// fine cells without a full coarse block (updateInterface.h:689-884), `count`
+// of them; `blk` the block's index among the leftover blocks
 template <class S, bool FLIPC, bool FLIPF>
-__global__ __launch_bounds__(256) void k_leftovers(IfArgs<typename StoreTraits<S>::compute> a)
+__device__ __forceinline__ void leftover_block(const IfArgs<typename StoreTraits<S>::compute> &a, int blk, int count)
 {
     using T = typename StoreTraits<S>::compute;
-    const int x = blockIdx.x * 256 + threadIdx.x;
-    if (x >= a.count) return;
+    const int x = blk * 256 + threadIdx.x;
+    if (x >= count) return;
     const S *fc = reinterpret_cast<const S *>(a.fc);
     const int centre = a.loParent[x];
     T v[27];
@@ -262,6 +270,22 @@ __global__ __launch_bounds__(256) void k_leftovers(IfArgs<typename StoreTraits<S
     for (int m = 0; m < 27; ++m) StoreTraits<S>::store(ff + pop_index(pre_plane(m, FLIPF), nb[pre_nb(m)]), f[m], m, a.cd);
 }
 
+template <class S, bool FLIPC, bool FLIPF>
+__global__ __launch_bounds__(256) void k_leftovers(IfArgs<typename StoreTraits<S>::compute> a)
+{
+    leftover_block<S, FLIPC, FLIPF>(a, blockIdx.x, a.count);
+}
+
+// C->F and the leftovers in one launch: they read the coarse level and write
+// disjoint fine cells, so nothing orders them; the first `c2fBlocks` blocks
+// take the C->F items, the others the leftovers
+template <class S, bool FLIPC, bool FLIPF, bool LINEAR>
+__global__ __launch_bounds__(256) void k_c2f_leftovers(IfArgs<typename StoreTraits<S>::compute> a, int c2fBlocks)
+{
+    if ((int)blockIdx.x < c2fBlocks) c2f_block<S, FLIPC, FLIPF, LINEAR>(a, blockIdx.x);
+    else leftover_block<S, FLIPC, FLIPF>(a, blockIdx.x - c2fBlocks, a.loCount);
+}
+
 inline unsigned groups_of_8(long long items) { return (unsigned)((8 * items + 255) / 256); }
 
 template <class S, bool FC, bool FF> void launch_all(const IfArgs<typename StoreTraits<S>::compute> &base, const Level &c,
@@ -274,13 +298,21 @@ template <class S, bool FC, bool FF> void launch_all(const IfArgs<typename Store
         if (linear) hipLaunchKernelGGL((k_fine_to_coarse<S, FC, FF, true>), dim3(groups_of_8(a.count)), dim3(256), 0, s, a);
         else hipLaunchKernelGGL((k_fine_to_coarse<S, FC, FF, false>), dim3(groups_of_8(a.count)), dim3(256), 0, s, a);
     }
-    if (c.C2F.interfaceCount > 0) {
+    const bool c2f = c.C2F.interfaceCount > 0, lo = c.C2F.leftoverCount > 0;
+    if (c2f) {
         a.count = c.C2F.interfaceCount; a.index = c.C2F.index.p; a.child = c.C2F.childMap.p;
         a.srcNb = c.C2F.srcNb.p; a.dstNb = c.C2F.dstNb.p;
+    }
+    if (c2f && lo) {
+        a.loCount = c.C2F.leftoverCount;
+        const int nb = (int)groups_of_8(a.count);
+        const dim3 g((unsigned)nb + (unsigned)((a.loCount + 255) / 256));
+        if (linear) hipLaunchKernelGGL((k_c2f_leftovers<S, FC, FF, true>), g, dim3(256), 0, s, a, nb);
+        else hipLaunchKernelGGL((k_c2f_leftovers<S, FC, FF, false>), g, dim3(256), 0, s, a, nb);
+    } else if (c2f) {
         if (linear) hipLaunchKernelGGL((k_coarse_to_fine<S, FC, FF, true>), dim3(groups_of_8(a.count)), dim3(256), 0, s, a);
         else hipLaunchKernelGGL((k_coarse_to_fine<S, FC, FF, false>), dim3(groups_of_8(a.count)), dim3(256), 0, s, a);
-    }
-    if (c.C2F.leftoverCount > 0) {
+    } else if (lo) {
         a.count = c.C2F.leftoverCount;
         hipLaunchKernelGGL((k_leftovers<S, FC, FF>), dim3((a.count + 255) / 256), dim3(256), 0, s, a);
     }
