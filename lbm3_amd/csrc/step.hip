@@ -1046,6 +1046,12 @@ __global__ __launch_bounds__(256) void k_collide_batch(T *f, int n, Relax<T> R)
 // the domain d with start[d] <= b < start[d + 1]; each domain carries its own
 // arguments, esotwist parity and collision variant
 // ---------------------------------------------------------------------------
+// The domain is the same for every lane of a block; readfirstlane states it,
+// so that the domain's descriptor is read by scalar loads and the buffer
+// resource made from its population pointer sits in scalar registers.  Left
+// to the compiler, the index counted as per-lane, and every raw buffer access
+// became a loop over the wave's distinct resources (168 such loops in the
+// bulk kernel).
 __device__ __forceinline__ int domain_of(const int *start, int n, int b)
 {
     int lo = 0, hi = n;
@@ -1054,11 +1060,11 @@ __device__ __forceinline__ int domain_of(const int *start, int n, int b)
         if (start[mid] <= b) lo = mid;
         else hi = mid;
     }
-    return lo;
+    return __builtin_amdgcn_readfirstlane(lo);
 }
 
 template <class S>
-__global__ __launch_bounds__(256) void k_multi_bulk(const MultiDesc<S> *D, const int *start, int n)
+__global__ __launch_bounds__(256) void k_multi_bulk(const MultiDesc<S> *__restrict__ D, const int *__restrict__ start, int n)
 {
     const int d = domain_of(start, n, blockIdx.x);
     const MultiDesc<S> &m = D[d];
@@ -1075,7 +1081,7 @@ __global__ __launch_bounds__(256) void k_multi_bulk(const MultiDesc<S> *D, const
 }
 
 template <class S>
-__global__ __launch_bounds__(256) void k_multi_wall(const MultiDesc<S> *D, const int *start, int n)
+__global__ __launch_bounds__(256) void k_multi_wall(const MultiDesc<S> *__restrict__ D, const int *__restrict__ start, int n)
 {
     const int d = domain_of(start, n, blockIdx.x);
     const MultiDesc<S> &m = D[d];
@@ -1087,7 +1093,7 @@ __global__ __launch_bounds__(256) void k_multi_wall(const MultiDesc<S> *D, const
 
 // after streaming: the open boundary and the periodic copy at the new parity
 template <class S>
-__global__ __launch_bounds__(256) void k_multi_open(const MultiDesc<S> *D, const int *start, int n)
+__global__ __launch_bounds__(256) void k_multi_open(const MultiDesc<S> *__restrict__ D, const int *__restrict__ start, int n)
 {
     const int d = domain_of(start, n, blockIdx.x);
     const MultiDesc<S> &m = D[d];
@@ -1111,7 +1117,7 @@ __device__ __forceinline__ void periodic_entry(S *f, const Topology &topo, const
 }
 
 template <class S>
-__global__ __launch_bounds__(256) void k_multi_periodic(const MultiDesc<S> *D, const int *start, int n)
+__global__ __launch_bounds__(256) void k_multi_periodic(const MultiDesc<S> *__restrict__ D, const int *__restrict__ start, int n)
 {
     const int d = domain_of(start, n, blockIdx.x);
     const MultiDesc<S> &m = D[d];

@@ -107,10 +107,19 @@ struct Codec {
 // per slot plane, so that the 54 population accesses of a cell need no vector
 // address arithmetic (buffer_load/store ... offen with an SGPR soffset).
 // Word 3 of the resource is the raw-buffer format of gfx9-class GPUs.
+// `base` and `bytes` must be the same in every lane of the wave (a kernel
+// argument, or the descriptor of the one domain a block belongs to): they are
+// read from the first lane, so that the resource sits in scalar registers even
+// where the compiler cannot prove them uniform (a descriptor read through a
+// pointer), instead of every access looping over the wave's resources.
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t pop_rsrc(const void *base, long long bytes)
 {
-    const int num = bytes >= 0xFFFFFFFFLL ? int(0xFFFFFFFFu) : int((unsigned)bytes);
-    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, num, 0x00020000);
+    const unsigned long long p = reinterpret_cast<unsigned long long>(base);
+    const unsigned lo = __builtin_amdgcn_readfirstlane(unsigned(p));
+    const unsigned hi = __builtin_amdgcn_readfirstlane(unsigned(p >> 32));
+    const int num = __builtin_amdgcn_readfirstlane(bytes >= 0xFFFFFFFFLL ? int(0xFFFFFFFFu) : int((unsigned)bytes));
+    void *b = reinterpret_cast<void *>((static_cast<unsigned long long>(hi) << 32) | lo);
+    return __builtin_amdgcn_make_buffer_rsrc(b, (short)0, num, 0x00020000);
 }
 
 template <class S> struct StoreTraits;
