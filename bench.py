@@ -31,6 +31,8 @@ sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBS = 8000.0
 SBYTES = {"f32": 4, "f16": 2, "bf16": 2, "f64": 8, "f16c": 2}
+# cells of the sphere workload the bulk kernel updates with the open cells inside
+BULK_CELLS_SPHERE = 4171784
 
 
 def workload(k17=False):
@@ -292,43 +294,158 @@ def cpu_baseline(threads, seconds=4.0, runs=3, single_seconds=6.0):
             "runs_mlups": [round(s[0], 3) for s in samples], "single_thread": single, "host": _host_cpu()}
 
 
-def traffic_from_profiles(storage, bulk_cells):
-    """Measured HBM bytes per bulk launch, if the counters were taken on the same
-    cell split (open cells in groups: key `storage`; in the bulk kernel:
-    `storage`_in_bulk)."""
+def traffic_from_profiles(storage, bulk_cells, tune):
+    """Measured HBM bytes per launch of the kernel the tuned step runs, if the
+    counters were taken on the same cell split and kernel: open cells in groups
+    (key `storage`), in the bulk kernel (`storage`_in_bulk), with the block's
+    lanes rotated (`_in_bulk_rotated`), or with the wall cells' blocks in the
+    same launch (`_in_bulk_walls`: side-stream choice 3 with the open cells
+    inside and the lanes in order)."""
     path = os.path.join(ROOT, "profiles", "hbm_traffic_sphere.json")
     if not os.path.exists(path):
         return None
     with open(path) as fh:
         d = json.load(fh)
-    for key in (storage, storage + "_in_bulk"):
-        e = d.get(key)
-        if e is not None and e.get("cells") == bulk_cells:
-            return e.get("bytes_per_launch")
-    return None
+    if tune["open_group"] != 1:
+        key = storage
+    elif tune["open_split"] == 3:
+        key = storage + "_in_bulk_rotated"
+    elif tune["overlap"] == 3:
+        key = storage + "_in_bulk_walls"
+    else:
+        key = storage + "_in_bulk"
+    e = d.get(key)
+    if e is None or e.get("cells") != bulk_cells:
+        return None
+    b = e.get("by_block_order", {}).get(str(tune["xcd_remap"]))
+    return b["bytes_per_launch"] if b else e.get("bytes_per_launch")
+
+
+# launch layouts the counter passes measure: (key suffix, side-stream choice,
+# open group width, x-face choice); each at every block order
+COUNTER_LAYOUTS = {
+    "f32": (("_in_bulk_walls", 3, 1, 0), ("_in_bulk", 1, 1, 0), ("_in_bulk_rotated", 1, 1, 3)),
+    "f16": (("_in_bulk_walls", 3, 1, 0), ("_in_bulk", 1, 1, 0), ("_in_bulk_rotated", 1, 1, 3)),
+    "bf16": (("_in_bulk_walls", 3, 1, 0), ("_in_bulk", 1, 1, 0), ("_in_bulk_rotated", 1, 1, 3)),
+}
+COUNTER_STEPS = 10
+
+
+def counter_segments():
+    """The measured blocks of the counters workload, in order:
+    (storage, key, block order)."""
+    return [(st, st + suffix, remap) for st, layouts in COUNTER_LAYOUTS.items()
+            for suffix, _, _, _ in layouts for remap in range(4)]
 
 
 def counters_workload():
     """A short sequence for rocprofv3 --pmc runs: the sphere workload in FP32,
-    FP16 and BF16, each autotuned as the bench does (2 steps per candidate),
-    then 10 steps whose dispatches are the last 10 of each step kernel; then
-    three 1 GiB copies (the calibration kernel whose bytes are known)."""
+    FP16 and BF16, each launch layout of COUNTER_LAYOUTS at every block order:
+    3 unmeasured steps, then COUNTER_STEPS steps between two 4 KiB copies (the
+    markers around a measured block); at the end three 1 GiB copies, the
+    calibration kernel whose bytes are known."""
     from lbm3_amd import lib
-    for st in ("f32", "f16", "bf16"):
+    for st, layouts in COUNTER_LAYOUTS.items():
         case, dom, tris = workload()
         s = lib.Sim(case, st)
         s.build(dom, tris)
         s.init()
         s.run(2)
-        tuned, _ = s.autotune(steps=2)
-        s.run(10)
-        s.sync()
-        k = s.cell_kinds(0)
+        for suffix, overlap, group, split in layouts:
+            for remap in range(4):
+                s.set_tune(0, remap, overlap, group, split, 0)
+                s.run(3)
+                s.sync()
+                lib.copy_probe(4096, 1)
+                s.run(COUNTER_STEPS)
+                s.sync()
+                lib.copy_probe(4096, 1)
+                k = s.cell_kinds(0)
+                print("counters workload: %s%s block order %d: bulk cells %d (open cells in it %d)"
+                      % (st, suffix, remap, k["bulk"], k["open_in_bulk"]), flush=True)
         s.close()
-        print("counters workload: %s sphere, tune %s, then 10 steps; cells bulk %d (open in it %d) open %d open_group %d"
-              % (st, tuned, k["bulk"], k["open_in_bulk"], k["open"], k["open_group"]), flush=True)
     lib.copy_probe(1 << 30, 3)
     print("counters workload: 3 copies of 1 GiB", flush=True)
+
+
+def _counter_dispatches(directory, counter):
+    """[(dispatch id, kernel name, counter bytes)] in launch order from a
+    rocprofv3 --pmc csv directory (the counter reads KiB, summed over its
+    instances)."""
+    import csv
+    import glob
+    files = glob.glob(os.path.join(directory, "*counter_collection.csv"))
+    if len(files) != 1:
+        raise SystemExit("expected one counter_collection.csv in %s, found %d" % (directory, len(files)))
+    per, names = {}, {}
+    with open(files[0]) as fh:
+        for r in csv.DictReader(fh):
+            if r["Counter_Name"] != counter:
+                continue
+            d = int(r["Dispatch_Id"])
+            per[d] = per.get(d, 0.0) + float(r["Counter_Value"]) * 1024.0
+            names[d] = r["Kernel_Name"]
+    return [(d, names[d], per[d]) for d in sorted(per)]
+
+
+def _segment_means(dispatches):
+    """The runs of dispatches between runs of k_copy dispatches, in order, as
+    the mean counter bytes per dispatch of each kernel of the run (what comes
+    before the first copy is left out); and the last three k_copy dispatches
+    (the 1 GiB calibration copies)."""
+    segs, cur, copies = [], None, []
+    for _, name, v in dispatches:
+        if "k_copy" in name:
+            copies.append(v)
+            if cur:
+                segs.append(cur)
+            cur = {}
+            continue
+        if cur is not None:
+            cur.setdefault(name, []).append(v)
+    return [{k: sum(v) / len(v) for k, v in seg.items()} for seg in segs], copies[-3:]
+
+
+def counters_summary(fetch_dir, write_dir, out_path):
+    """HBM bytes per launch of each measured layout from the FETCH_SIZE and
+    WRITE_SIZE passes of the counters workload, scaled by what each counter
+    reads for the 1 GiB copies; written as JSON to out_path."""
+    fe, fcal = _segment_means(_counter_dispatches(fetch_dir, "FETCH_SIZE"))
+    wr, wcal = _segment_means(_counter_dispatches(write_dir, "WRITE_SIZE"))
+    order = counter_segments()
+    # measured blocks alternate with the unmeasured steps between them; the
+    # calibration's own fill may add one run after the last block
+    n = 2 * len(order) - 1
+    if not (n <= len(fe) <= n + 1 and n <= len(wr) <= n + 1) or len(fcal) != 3 or len(wcal) != 3:
+        raise SystemExit("counter passes do not match the workload: %d / %d runs between copies for %d blocks"
+                         % (len(fe), len(wr), len(order)))
+    fe, wr = fe[:n:2], wr[:n:2]
+    gib = float(1 << 30)
+    ff, fw = gib / (sum(fcal) / 3), gib / (sum(wcal) / 3)
+    out = {"calibration": {"kernel": "k_copy (1 GiB read + 1 GiB written)", "fetch_factor": round(ff, 3),
+                           "write_factor": round(fw, 3)},
+           "workload": "bench.py --counters-workload: each layout at each block order, %d steps, "
+                       "open cells completed inside the bulk kernel, LDS cap 0" % COUNTER_STEPS}
+    for (st, key, remap), f, w in zip(order, fe, wr):
+        # the step kernel of the block: the one moving the most bytes
+        name = max(f, key=lambda k: f[k])
+        fb, wb = f[name] * ff, w.get(name, 0.0) * fw
+        e = out.setdefault(key, {"kernel": name.split("(")[0].replace("void ", ""), "by_block_order": {}})
+        e["by_block_order"][str(remap)] = {"fetch_bytes": int(round(fb)), "write_bytes": int(round(wb)),
+                                           "bytes_per_launch": int(round(fb + wb))}
+    sb = {"f32": 4, "f16": 2, "bf16": 2}
+    for st, key, _ in order:
+        e = out[key]
+        e["cells"] = BULK_CELLS_SPHERE
+        walls = 5264 if key.endswith("_walls") else 0
+        e["wall_cells"] = walls
+        e["algorithmic_population_bytes"] = (BULK_CELLS_SPHERE + walls) * 2 * 27 * sb[st]
+    with open(out_path, "w") as fh:
+        json.dump(out, fh, indent=1)
+    for key in dict.fromkeys(k for _, k, _ in order):
+        e = out[key]
+        print("%-22s %s" % (key, "  ".join("order %s: %.1f MB (%.3f)" % (r, v["bytes_per_launch"] / 1e6,
+              v["bytes_per_launch"] / e["algorithmic_population_bytes"]) for r, v in sorted(e["by_block_order"].items()))))
 
 
 def serving(n, steps_scale, storages, seed=0):
@@ -367,11 +484,16 @@ def main():
                     help="skip the multilevel-sphere and rotor-duct MLUPS lines")
     ap.add_argument("--counters-workload", action="store_true",
                     help="run only the short sequence that the HBM byte counters are collected on")
+    ap.add_argument("--counters-summary", nargs=3, metavar=("FETCH_DIR", "WRITE_DIR", "OUT_JSON"),
+                    help="HBM bytes per launch from the FETCH_SIZE and WRITE_SIZE passes of --counters-workload")
     ap.add_argument("--case", choices=["multilevel", "rotor", "graph"],
                     help="measure only this configuration (FP32) and print its JSON object")
     args = ap.parse_args()
     if args.counters_workload:
         counters_workload()
+        return
+    if args.counters_summary:
+        counters_summary(*args.counters_summary)
         return
     if args.case:
         run = {"multilevel": lambda: measure_multilevel("f32", steps=args.steps, warmup=args.warmup),
@@ -399,7 +521,7 @@ def main():
     copy_ms = lib.copy_probe(copy_bytes, 20)
     # measured HBM bytes of the K15 bulk kernel (the K17 variant has no counter run of its own)
     for run in [main_run] + [v for k, v in variants.items() if k != "k17"]:
-        run["roofline"]["traffic"] = traffic_from_profiles(run["storage"], run["bulk_cells"])
+        run["roofline"]["traffic"] = traffic_from_profiles(run["storage"], run["bulk_cells"], run["tune"]["chosen"])
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_threads)
     other = None
     if not args.no_other_cases:

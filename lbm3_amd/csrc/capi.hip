@@ -448,10 +448,8 @@ template <class S> static void step_level(Sim &s, int level)
     a.param = slot;
     // side-stream choice 3: the wall-adjacent cells in blocks ahead of the bulk
     // blocks of one launch (not with rotor cells in the bulk kernel, rotated
-    // lanes or 64-bit offsets, nor with the open cells inside a 16-bit bulk
-    // kernel, whose 110 registers would spill under the combined budget)
-    const bool walls = s.tune.overlap == 3 && l.W > 0 && !a.fast.rotorBits && !a.laneRot && !wide_level<S>(l.N) &&
-                       !(a.fast.openBits && sizeof(S) == 2);
+    // lanes or 64-bit offsets)
+    const bool walls = s.tune.overlap == 3 && l.W > 0 && !a.fast.rotorBits && !a.laneRot && !wide_level<S>(l.N);
     a.wallBlocks = walls ? (int)((l.W + 255) / 256) : 0;
     if (timed || !s.tune.overlap || ((l.W == 0 || walls) && a.open.count == 0 && a.rotorCellCount == 0)) {
         // in order on one stream, so that each kernel's events bracket it alone

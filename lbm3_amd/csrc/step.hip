@@ -651,6 +651,21 @@ __global__ __launch_bounds__(256) void k_collide_wall(StepArgs<S> a)
     wall_cell<S, FLIP>(a, wm);
 }
 
+// The wall path of the combined launch below as a call of its own, for 16-bit
+// storage with the open cells in the bulk path: inlined there, the register
+// allocation of the two paths together slowed the bulk blocks about 3.5x; as a
+// callee, the wall path spills within its own frame.  The arguments stay in
+// the kernel's constant argument segment (address space 4), read by scalar
+// loads.
+#ifndef LBM3_WALL_CALL
+#define LBM3_WALL_CALL 1
+#endif
+template <class S, bool FLIP>
+__device__ __noinline__ void wall_cell_call(const __attribute__((address_space(4))) StepArgs<S> *a, int wm)
+{
+    wall_cell<S, FLIP>(*(const StepArgs<S> *)a, wm);
+}
+
 // The bulk kernel with the wall-adjacent cells in blocks of their own ahead of
 // the bulk blocks, in one launch: no second stream and no join per step.  The
 // wall path needs about twice the bulk path's registers; the register budget is
@@ -662,7 +677,15 @@ __global__ __launch_bounds__(256, 6) void k_collide_stream_walls(StepArgs<S> a)
     const int b = blockIdx.x;
     if (b < a.wallBlocks) {
         const int wm = b * 256 + threadIdx.x;
-        if (wm < a.W) wall_cell<S, FLIP>(a, wm);
+        if (wm < a.W) {
+            if constexpr (sizeof(S) == 2 && OPEN && LBM3_WALL_CALL)
+                // the kernel's one argument starts the argument segment; its
+                // address, not &a, which would copy the argument to the stack
+                wall_cell_call<S, FLIP>(
+                    (const __attribute__((address_space(4))) StepArgs<S> *)__builtin_amdgcn_kernarg_segment_ptr(), wm);
+            else
+                wall_cell<S, FLIP>(a, wm);
+        }
         return;
     }
     const int cell = xcd_block(b - a.wallBlocks, gridDim.x - a.wallBlocks, a.xcdRemap, FLIP) * 256 + threadIdx.x;
@@ -1151,10 +1174,9 @@ void launch_collide_stream(const StepArgs<S> &a, bool flip, hipStream_t s)
     const bool wide = wide_level<S>(a.fast.N);
     if (a.wallBlocks > 0) {
         // the caller combines only levels without rotor cells in the bulk
-        // kernel, rotated lanes or 64-bit offsets, and 16-bit levels only
-        // without open cells in the bulk kernel
+        // kernel, rotated lanes or 64-bit offsets
         const dim3 gw(blocks_for(a.fast.N) + (unsigned)a.wallBlocks);
-#define LBM3_BULK_WALLS(F, K, O) hipLaunchKernelGGL((k_collide_stream_walls<S, F, K, O && sizeof(S) != 2>), gw, b, lds, s, a)
+#define LBM3_BULK_WALLS(F, K, O) hipLaunchKernelGGL((k_collide_stream_walls<S, F, K, O>), gw, b, lds, s, a)
         const bool o = a.fast.openBits != nullptr;
         if (flip) { if (k17) { if (o) LBM3_BULK_WALLS(true, 1, true); else LBM3_BULK_WALLS(true, 1, false); }
                     else { if (o) LBM3_BULK_WALLS(true, 2, true); else LBM3_BULK_WALLS(true, 2, false); } }
