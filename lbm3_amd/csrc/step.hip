@@ -23,6 +23,7 @@
 // every slot run's offset within a tile a compile-time constant.
 #include "step.hpp"
 #include "hooks.hpp"
+#include <type_traits>
 
 namespace lbm3 {
 
@@ -670,9 +671,19 @@ __device__ __noinline__ void wall_cell_call(const __attribute__((address_space(4
 // the bulk blocks, in one launch: no second stream and no join per step.  The
 // wall path needs about twice the bulk path's registers; the register budget is
 // held to six waves per SIMD (the bulk path fits it, the wall path spills), so
-// the bulk blocks keep their occupancy.
+// the bulk blocks keep their occupancy.  BF16 with the open cells inside: seven
+// waves (72 registers, no spill in the bulk path), measured on the
+// 256x128x128 sphere at 0.0988-0.0996 ms per step against 0.1012-0.1025 with
+// six; FP16 measured the other way round (0.0976-0.0990 with six, and seven
+// lost to the wall kernel on the side stream).
+template <class S>
+constexpr int walls_waves(bool open)
+{
+    return open && std::is_same<S, __hip_bfloat16>::value ? 7 : 6;
+}
+
 template <class S, bool FLIP, int BULK, bool OPEN>
-__global__ __launch_bounds__(256, 6) void k_collide_stream_walls(StepArgs<S> a)
+__global__ __launch_bounds__(256, walls_waves<S>(OPEN)) void k_collide_stream_walls(StepArgs<S> a)
 {
     const int b = blockIdx.x;
     if (b < a.wallBlocks) {
