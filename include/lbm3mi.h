@@ -150,7 +150,7 @@ int lbm3_synchronize(lbm3_sim *s);
 int lbm3_autotune(lbm3_sim *s, int steps, double *out, int cap, int *nOut);
 /* The launch choices in use: {bulk LDS bytes, block order, side streams, open
  * group cells, x-face choice, rotor cells in the bulk kernel}; lbm3_set_tune
- * sets them (LDS 0..65536, block order 0..3, side streams 0..2, group 1, 8 or
+ * sets them (LDS 0..65536, block order 0..3, side streams 0..3, group 1, 8 or
  * 16, x-face choice 0..3, rotor cells 0..1; a new layout of the open or rotor
  * cells completes a
  * pending open-boundary condition first).  Every choice gives the
