@@ -140,12 +140,13 @@ int lbm3_synchronize(lbm3_sim *s);
  * other open cells; 3: in the bulk kernel with a block's lanes rotated by 32
  * cells, so that a row's two x-face cells share a wave), and, where a level
  * has rotors, whether the bulk kernel steps the rotor cells too instead of a
- * kernel of their own; keeps the fastest and restores the simulation's state
+ * kernel of their own; times the six fastest again, three rounds in turn, keeps
+ * the one with the best round and restores the simulation's state
  * as it was (the populations and every step counter; a
  * pending open-boundary condition stays pending unless the chosen open-cell
  * layout differs, which completes it first).  out[7 r + {LDS bytes,
  * block order, side streams, group cells, x-face choice, rotor cells in the
- * bulk kernel, ms per step}] for the first `cap` candidates; *nOut the
+ * bulk kernel, ms per step in the sweep}] for the first `cap` candidates; *nOut the
  * candidates timed. */
 int lbm3_autotune(lbm3_sim *s, int steps, double *out, int cap, int *nOut);
 /* The launch choices in use: {bulk LDS bytes, block order, side streams, open

@@ -1096,6 +1096,8 @@ int lbm3_autotune(lbm3_sim *h, int steps, double *out, int cap, int *nOut)
     Sim::Tune best = start;
     double bestMs = 1e300;
     int rows = 0;
+    struct Cand { Sim::Tune t; double ms; };
+    std::vector<Cand> cands;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
@@ -1126,9 +1128,34 @@ int lbm3_autotune(lbm3_sim *h, int steps, double *out, int cap, int *nOut)
                         o[0] = lds; o[1] = remap; o[2] = overlap; o[3] = g; o[4] = gs.split; o[5] = gs.rotor; o[6] = per;
                     }
                     ++rows;
-                    if (per < bestMs) { bestMs = per; best = s.tune; }
+                    cands.push_back({s.tune, per});
                 }
         if (rc) break;
+    }
+    // The fastest few candidates are timed again, in turns over three rounds,
+    // and the best of each candidate's rounds decides: a single noisy timing
+    // in the sweep (a launch that met another's traffic in the cache, a clock
+    // change) does not pick the configuration alone.
+    if (!rc && !cands.empty()) {
+        std::stable_sort(cands.begin(), cands.end(), [](const Cand &a, const Cand &b) { return a.ms < b.ms; });
+        const int K = (int)std::min<size_t>(6, cands.size());
+        std::vector<double> again(K, 1e300);
+        for (int round = 0; round < 3 && !rc; ++round)
+            for (int i = 0; i < K && !rc; ++i) {
+                const bool relayout = !same_open_layout(s, s.tune, cands[i].t);
+                s.tune = cands[i].t;
+                if (relayout && (rc = regroup(s))) break;
+                update_all(s, 0);
+                if ((rc = check_hip(hipEventRecord(e0, s.stream), "event"))) break;
+                for (int k = 0; k < steps; ++k) update_all(s, 0);
+                if ((rc = check_hip(hipEventRecord(e1, s.stream), "event"))) break;
+                if ((rc = check_hip(hipEventSynchronize(e1), "autotune step"))) break;
+                float ms = 0.f;
+                if ((rc = check_hip(hipEventElapsedTime(&ms, e0, e1), "event time"))) break;
+                again[i] = std::min(again[i], (double)ms / steps);
+            }
+        for (int i = 0; i < K; ++i)
+            if (again[i] < bestMs) { bestMs = again[i]; best = cands[i].t; }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
